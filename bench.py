@@ -1,0 +1,241 @@
+#!/usr/bin/env python3
+"""Throughput of the coupled 3D IB flow step on one MI355X.
+
+Workload (the plain method on the GPU at 256^3 in FP32): x_range 1, nu = 5e-4, free stream
+(1, 0, 0) every step, flow density 1, penalty width 2; initial vorticity = curl of a seeded
+random-phase vector potential (|k| <= 8, amplitude ~ k^-2, seed 42) scaled to
+max|omega| = 10, then one velocity recovery; a fixed sphere of radius 0.08 at the centre
+with 3000 Fibonacci-lattice forcing points, virtual-boundary stiffness -5e4 and damping
+-10 (the reference's sign convention), both rescaled by the Lagrangian spacing squared
+as ImmersedBodyFlowInteraction does.
+
+One step is the reference driver's loop body: ``dt = flow_sim.compute_stable_timestep()``
+(CFL 0.1), ``interactor.time_step(dt)``, ``interactor()``, ``flow_sim.time_step(dt, U)``.
+Inputs are resident on the device before timing starts; W untimed warm-up steps, then K
+steps between two device synchronisations.
+
+Prints one JSON line with the throughput in cell-updates/s, the per-kernel roofline of
+the dominant kernel (its average duration from HIP events recorded on its own stream
+during the timed steps) and a CPU baseline: the numpy/scipy restatement of the same loop
+timed on this host on a bounded sample of the same workload.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+WORKLOAD = "coupled IB flow step, 256^3 fp32, sphere r=0.08 (3000 Fibonacci points), nu=5e-4, U=(1,0,0), CFL 0.1"
+# frozen algorithmic traffic of the whole step in words per cell (survey section 8(d))
+STEP_WORDS_PER_CELL = 76
+
+
+def stage_words(nz, ny, nx):
+    """Algorithmic words moved per launch of each stage of sopht_ns3d_step, per cell
+    (reads + writes of the arrays each kernel must touch, each counted once)."""
+    c = (nx + 1) / nx  # complex columns kx = 0..nx per real row of nx
+    ghat = (nz + 1) * (ny + 1) * (nx + 1) / (nz * ny * nx)
+    return {
+        "vorticity_forcing_cross": 9 + 6,      # read w, u, F; write w, B
+        "vorticity_advect": 6 + 3,             # read w, B; write w2 (into psi)
+        "vorticity_diffuse_penalise": 3 + 6,   # read w2; write w, F = 0
+        "poisson_x_r2c": 3 * (1 + 2 * c),      # real rows in, half spectra out
+        "poisson_y_fwd": 3 * (2 * c + 4 * c),  # y < ny in, all 2ny out
+        "poisson_z_conv": 3 * (4 * c + 4 * c) + ghat,  # z < nz in and out, G^ octant
+        "poisson_y_inv": 3 * (4 * c + 2 * c),
+        "poisson_x_c2r": 3 * (2 * c + 1),
+        "velocity_curl": 3 + 3,                # read psi; write u
+    }
+
+
+def build(n, real_t, seed=42, n_lag=3000, radius=0.08, nu=5e-4, K=8):
+    from sopht_amd import simulator as sps
+    from sopht_amd.utils import synthetic
+
+    import torch
+
+    flow = sps.UnboundedNavierStokesFlowSimulator3D(
+        grid_size=(n, n, n), x_range=1.0, kinematic_viscosity=nu, cfl=0.1, real_t=real_t,
+        with_forcing=True, with_free_stream_flow=True,
+    )
+    w0 = synthetic.random_phase_vorticity((n, n, n), 1.0, K, seed, 10.0)
+    flow.vorticity_field.copy_(torch.from_numpy(w0.astype(real_t)))
+    del w0
+    U = np.array([1.0, 0.0, 0.0])
+    flow.recover_velocity(U)
+    body = sps.RigidBody((0.5, 0.5, 0.5), radius)
+    inter = sps.RigidBodyFlowInteraction(
+        body, flow.eul_grid_forcing_field, flow.velocity_field, -5e4, -10.0, flow.dx, 3,
+        sps.FibonacciSphereForcingGrid, real_t=real_t, num_lag_nodes=n_lag,
+    )
+    return flow, inter, U
+
+
+def step(flow, inter, U):
+    dt = flow.compute_stable_timestep()
+    inter.time_step(dt)
+    inter()
+    flow.time_step(dt, free_stream_velocity=U)
+    return dt
+
+
+def cpu_baseline(n, real_t, steps, n_lag=3000):
+    """The oracle (numpy + scipy.fft with all host cores for the FFTs) on the same loop."""
+    from oracle import flow as OF
+    from oracle import poisson as OP
+    from sopht_amd import simulator as sps
+    from sopht_amd.utils import synthetic
+
+    cores = len(os.sched_getaffinity(0))
+    OP.WORKERS = cores
+    o = OF.UnboundedNavierStokesFlowSimulator3D(
+        (n, n, n), 1.0, 5e-4, cfl=0.1, real_t=real_t, with_forcing=True, with_free_stream_flow=True
+    )
+    o.vorticity_field[...] = synthetic.random_phase_vorticity((n, n, n), 1.0, 8, 42, 10.0).astype(real_t)
+    U = np.array([1.0, 0.0, 0.0])
+    o.recover_velocity(U)
+    body = sps.RigidBody((0.5, 0.5, 0.5), 0.08)
+    oi = OF.ImmersedBodyFlowInteraction(o, sps.FibonacciSphereForcingGrid(3, body, n_lag), -5e4, -10.0, real_t)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        dt = o.compute_stable_timestep()
+        oi.time_step(dt)
+        oi()
+        o.time_step(dt, free_stream_velocity=U)
+    el = time.perf_counter() - t0
+    return {
+        "value": n**3 * steps / el,
+        "unit": "cell-updates/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"{steps} steps of the same {n}^3 loop (numpy stencils single-threaded, scipy.fft with {cores} workers), {el:.1f} s",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--grid", type=int, default=256)
+    ap.add_argument("--precision", choices=["single", "double"], default="single")
+    ap.add_argument("--cpu-steps", type=int, default=2, help="steps of the CPU baseline sample (0 = skip)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(
+            f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported "
+            "(multi-GPU sharding is designed, not built).",
+            file=sys.stderr,
+        )
+        return 2
+
+    import torch
+
+    from sopht_amd import _lib as L
+
+    torch.cuda.set_device(0)
+    real_t = np.float32 if args.precision == "single" else np.float64
+    n = args.grid
+    flow, inter, U = build(n, real_t)
+    for _ in range(args.warmup):
+        step(flow, inter, U)
+    lib = L.lib()
+    L.check(lib.sopht_ns3d_enable_timing(flow._plan, 1))
+    import ctypes as C
+
+    ms = (C.c_double * 16)()
+    cnt = C.c_int64()
+    lib.sopht_ns3d_stage_times(flow._plan, ms, 16, C.byref(cnt), 1)  # reset
+    inter.check_support_in_domain()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(flow, inter, U)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    nst = lib.sopht_ns3d_stage_times(flow._plan, ms, 16, C.byref(cnt), 0)
+    names = [lib.sopht_ns3d_stage_name(i).decode() for i in range(nst)]
+    avg_ms = {names[i]: ms[i] / max(cnt.value, 1) for i in range(nst)}
+    inter.check_support_in_domain()
+    if not torch.isfinite(flow.vorticity_field).all():
+        print("non-finite vorticity after the timed steps", file=sys.stderr)
+        return 1
+
+    cells = n**3
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = cells * args.steps / elapsed
+    word = np.dtype(real_t).itemsize
+    words = stage_words(n, n, n)
+    dom = max(avg_ms, key=avg_ms.get)
+    alg_bytes = words[dom] * cells * word
+    achieved = alg_bytes / (avg_ms[dom] * 1e-3) / 1e9
+    traffic = None
+    pmc = ROOT / "profiles" / f"{dom}_traffic.json"
+    if pmc.exists():
+        try:
+            traffic = json.loads(pmc.read_text()).get("hbm_bytes_per_launch")
+        except (ValueError, OSError):
+            traffic = None
+    roof = {
+        "bound": "hbm",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": traffic,
+        "kernel": dom,
+        "kernel_ms": round(avg_ms[dom], 4),
+        "algorithmic_bytes": int(alg_bytes),
+    }
+    step_gbs = STEP_WORDS_PER_CELL * word * cells / (ms_per_step * 1e-3) / 1e9
+    kernels_gbs = {
+        k: round(words[k] * cells * word / (avg_ms[k] * 1e-3) / 1e9, 1) for k in avg_ms if avg_ms[k] > 0
+    }
+    cpu = None
+    if args.cpu_steps > 0:
+        try:
+            cpu = cpu_baseline(n, real_t, args.cpu_steps)
+        except Exception as e:  # the baseline is reported, never required
+            cpu = {"value": None, "unit": "cell-updates/s", "cores": None, "kind": "port", "sample": f"failed: {e}"}
+    out = {
+        "metric": "3D IB flow cell-updates/s at 256^3 fp32 (% HBM peak); batched vs plain; adj/fwd",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if real_t == np.float32 else "f64",
+        "data": "synthetic (seeded random-phase vorticity, Fibonacci sphere)",
+        "config": {
+            "workload": WORKLOAD if n == 256 and real_t == np.float32 else f"coupled IB flow step, {n}^3 {args.precision}",
+            "grid": [n, n, n],
+            "precision": args.precision,
+            "lagrangian_nodes": 3000,
+            "parallelism": "single GPU",
+        },
+        "step_hbm_frac_76w": round(step_gbs / HBM_PEAK_GBS, 4),
+        "stage_ms": {k: round(v, 4) for k, v in avg_ms.items()},
+        "stage_gbs": kernels_gbs,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,268 @@
+/*
+ * sopht_amd.h — C ABI of the MI355X (gfx950) backend for SophT's 3D unbounded vorticity
+ * flow step with virtual-boundary immersed-boundary coupling.
+ *
+ * Conventions
+ *   - dtype: SOPHT_F32 or SOPHT_F64 (the reference's real_t = np.float32 / np.float64).
+ *   - Grid sizes are (nz, ny, nx); fields are x-fastest. A "strides" argument is three
+ *     int64 element strides (z, y, x) of a scalar field or of every component of a
+ *     vector field; vector fields are passed as three component pointers.
+ *   - stream: a hipStream_t passed as void* (0 = the null stream). Every call is
+ *     stream-ordered and asynchronous unless stated; no call synchronises the device
+ *     unless stated.
+ *   - Return value: 0 on success, a negative SOPHT_ERR_* code otherwise;
+ *     sopht_last_error() describes the last failure of the calling thread.
+ *   - The library holds no global mutable state besides the per-thread error string;
+ *     plans own their device scratch and are not shared across streams concurrently.
+ *
+ * Each entry point names the reference (SophT) interface it replaces as file:line,
+ * paths relative to the SophT repository root.
+ */
+#ifndef SOPHT_AMD_H
+#define SOPHT_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { SOPHT_F32 = 0, SOPHT_F64 = 1 };
+enum {
+  SOPHT_OK = 0,
+  SOPHT_ERR_INVALID = -1,     /* bad argument (maps to ValueError)            */
+  SOPHT_ERR_UNSUPPORTED = -2, /* size or option not supported (ValueError)    */
+  SOPHT_ERR_HIP = -3,         /* HIP runtime error (RuntimeError)             */
+  SOPHT_ERR_OUT_OF_DOMAIN = -4 /* Lagrangian support leaves the Eulerian grid */
+};
+
+const char* sopht_last_error(void);
+int sopht_version(void);
+
+/* ---------------------------------------------------------------- stencils (grid ops) */
+
+/* gen_curl_pyst_kernel_3d — sopht/numeric/eulerian_grid_ops/stencil_ops_3d/curl_3d.py:13-132.
+ * curl[1:-1]^3 = prefactor * curl_h(field); border set to 0 when reset_ghost_zone. */
+int sopht_curl_3d(int dtype, void* const curl[3], const int64_t* curl_strides,
+                  const void* const field[3], const int64_t* field_strides, int64_t nz,
+                  int64_t ny, int64_t nx, double prefactor, int reset_ghost_zone, void* stream);
+
+/* gen_update_vorticity_from_velocity_forcing_pyst_kernel_3d —
+ * .../update_vorticity_from_velocity_forcing_3d.py:12-132. w[1:-1]^3 += p * curl_h(f). */
+int sopht_update_vorticity_from_velocity_forcing_3d(int dtype, void* const vorticity[3],
+                                                    const int64_t* vorticity_strides,
+                                                    const void* const forcing[3],
+                                                    const int64_t* forcing_strides, int64_t nz,
+                                                    int64_t ny, int64_t nx, double prefactor,
+                                                    void* stream);
+
+/* gen_update_vorticity_from_penalised_velocity_pyst_kernel_3d —
+ * .../update_vorticity_from_velocity_forcing_3d.py:135-291. w[1:-1]^3 += p*curl_h(u_pen - u). */
+int sopht_update_vorticity_from_penalised_velocity_3d(
+    int dtype, void* const vorticity[3], const int64_t* vorticity_strides,
+    const void* const penalised_velocity[3], const int64_t* penalised_strides,
+    const void* const velocity[3], const int64_t* velocity_strides, int64_t nz, int64_t ny,
+    int64_t nx, double prefactor, void* stream);
+
+/* gen_elementwise_cross_product_pyst_kernel_3d — .../elementwise_ops_3d.py:390-449. */
+int sopht_elementwise_cross_product_3d(int dtype, void* const result[3],
+                                       const int64_t* result_strides,
+                                       const void* const field_1[3], const int64_t* f1_strides,
+                                       const void* const field_2[3], const int64_t* f2_strides,
+                                       int64_t nz, int64_t ny, int64_t nx, void* stream);
+
+/* gen_diffusion_flux_pyst_kernel_3d (scalar) — .../diffusion_flux_3d.py:14-75.
+ * flux[1:-1]^3 = p*(sum of 6 neighbours - 6 c); border 0 when reset_ghost_zone. */
+int sopht_diffusion_flux_3d(int dtype, void* flux, const int64_t* flux_strides,
+                            const void* field, const int64_t* field_strides, int64_t nz,
+                            int64_t ny, int64_t nx, double prefactor, int reset_ghost_zone,
+                            void* stream);
+
+/* gen_diffusion_timestep_euler_forward_pyst_kernel_3d (scalar) —
+ * .../diffusion_timestep_3d.py:12-80. flux (border reset) then field += flux. */
+int sopht_diffusion_timestep_euler_forward_3d(int dtype, void* field,
+                                              const int64_t* field_strides, void* flux,
+                                              const int64_t* flux_strides, int64_t nz,
+                                              int64_t ny, int64_t nx, double nu_dt_by_dx2,
+                                              void* stream);
+
+/* gen_penalise_field_boundary_pyst_kernel_3d (scalar) —
+ * .../penalise_field_boundary_3d.py:13-240. In place. sine_tables holds, for x then y then
+ * z, `width` front factors followed by `width` back factors (6*width doubles, host
+ * memory): sin(pi/(2 w dx) * distance of the slab cell from the first/last cell centre). */
+int sopht_penalise_field_boundary_3d(int dtype, void* field, const int64_t* field_strides,
+                                     int64_t nz, int64_t ny, int64_t nx, int width,
+                                     const double* sine_tables, void* stream);
+
+/* gen_divergence_pyst_kernel_3d — .../divergence_3d.py:13-96. */
+int sopht_divergence_3d(int dtype, void* divergence, const int64_t* divergence_strides,
+                        const void* const field[3], const int64_t* field_strides, int64_t nz,
+                        int64_t ny, int64_t nx, double inv_dx, int reset_ghost_zone,
+                        void* stream);
+
+/* Elementwise family — .../elementwise_ops_3d.py: set_fixed_val :60-119, add_fixed_val
+ * :271-334, elementwise_sum :13-57, elementwise_copy :122-143, saxpby :337-387,
+ * set_fixed_val_at_boundaries :200-268, complex_product :146-197 (interleaved complex). */
+int sopht_set_fixed_val_3d(int dtype, void* field, const int64_t* strides, int64_t nz,
+                           int64_t ny, int64_t nx, double value, void* stream);
+int sopht_add_fixed_val_3d(int dtype, void* sum_field, const int64_t* sum_strides,
+                           const void* field, const int64_t* field_strides, int64_t nz,
+                           int64_t ny, int64_t nx, double value, void* stream);
+int sopht_elementwise_saxpby_3d(int dtype, void* sum_field, const int64_t* sum_strides,
+                                const void* field_1, const int64_t* f1_strides,
+                                const void* field_2, const int64_t* f2_strides, int64_t nz,
+                                int64_t ny, int64_t nx, double field_1_prefac,
+                                double field_2_prefac, void* stream);
+int sopht_elementwise_sum_3d(int dtype, void* sum_field, const int64_t* sum_strides,
+                             const void* field_1, const int64_t* f1_strides,
+                             const void* field_2, const int64_t* f2_strides, int64_t nz,
+                             int64_t ny, int64_t nx, void* stream);
+int sopht_elementwise_copy_3d(int dtype, void* field, const int64_t* strides,
+                              const void* rhs_field, const int64_t* rhs_strides, int64_t nz,
+                              int64_t ny, int64_t nx, void* stream);
+int sopht_set_fixed_val_at_boundaries_3d(int dtype, void* field, const int64_t* strides,
+                                         int64_t nz, int64_t ny, int64_t nx, int width,
+                                         double value, void* stream);
+int sopht_elementwise_complex_product_3d(int dtype, void* product, const int64_t* product_strides,
+                                         const void* field_1, const int64_t* f1_strides,
+                                         const void* field_2, const int64_t* f2_strides,
+                                         int64_t nz, int64_t ny, int64_t nx, void* stream);
+
+/* gen_advection_flux_conservative_eno3_pyst_kernel_3d — .../advection_flux_3d.py:12-233.
+ * Accumulates into flux on [2:-2]^3 (six sub-stencils in the reference order). */
+int sopht_advection_flux_conservative_eno3_3d(int dtype, void* flux, const int64_t* flux_strides,
+                                              const void* field, const int64_t* field_strides,
+                                              const void* const velocity[3],
+                                              const int64_t* velocity_strides, int64_t nz,
+                                              int64_t ny, int64_t nx, double inv_dx,
+                                              void* stream);
+
+/* gen_vorticity_stretching_flux_pyst_kernel_3d — .../vorticity_stretching_flux_3d.py:13-107. */
+int sopht_vorticity_stretching_flux_3d(int dtype, void* const flux[3], const int64_t* flux_strides,
+                                       const void* const vorticity[3],
+                                       const int64_t* vorticity_strides,
+                                       const void* const velocity[3],
+                                       const int64_t* velocity_strides, int64_t nz, int64_t ny,
+                                       int64_t nx, double prefactor, void* stream);
+
+/* ---------------------------------------------------------------- free-space Poisson */
+
+typedef struct sopht_poisson3d sopht_poisson3d;
+
+/* UnboundedPoissonSolverPYFFTW3D.__init__ — .../poisson_solver_3d/UnboundedPoissonSolverPYFFTW3D.py:12-54.
+ * Builds the Green's function octant (FP64 on the device) and the pass scratch. Lengths
+ * 2*n per axis must factor into 2, 3 and 5 with 8 <= 2n <= 1024. */
+int sopht_poisson3d_create(int dtype, int64_t nz, int64_t ny, int64_t nx, double x_range,
+                           void* stream, sopht_poisson3d** plan);
+/* solve — :111-149 (scalar). rhs is read only. */
+int sopht_poisson3d_solve(sopht_poisson3d* plan, void* solution, const int64_t* solution_strides,
+                          const void* rhs, const int64_t* rhs_strides, void* stream);
+/* vector_field_solve — :151-172. Solves the three components in one pass sequence. */
+int sopht_poisson3d_vector_solve(sopht_poisson3d* plan, void* const solution[3],
+                                 const int64_t* solution_strides, const void* const rhs[3],
+                                 const int64_t* rhs_strides, void* stream);
+int sopht_poisson3d_destroy(sopht_poisson3d* plan);
+/* Bytes of device scratch the plan holds (for capacity planning). */
+int64_t sopht_poisson3d_scratch_bytes(const sopht_poisson3d* plan);
+
+/* ---------------------------------------------------------------- Eulerian <-> Lagrangian */
+
+/* local_eulerian_grid_support_of_lagrangian_grid_kernel_3d —
+ * sopht/numeric/immersed_boundary_ops/EulerianLagrangianGridCommunicator3D.py:68-135.
+ * lag_positions: (3, n) float64, device. nearest_index: (3, n) int64. support:
+ * (3, 4, 4, 4, n) real. */
+int sopht_lag_support_3d(int dtype, const double* lag_positions, int64_t* nearest_index,
+                         void* support, int64_t num_lag_nodes, double dx,
+                         double eul_grid_coord_shift, void* stream);
+
+/* cosine / peskin interpolation weights — :387-518. kernel_type 0 = cosine, 1 = peskin.
+ * support is divided by dx in place (cosine, :404) or replaced by |d|/dx (peskin, :432). */
+int sopht_lag_interp_weights_3d(int dtype, int kernel_type, void* weights, void* support,
+                                int64_t num_lag_nodes, double dx, void* stream);
+
+/* eulerian_to_lagrangian_grid_interpolation_kernel_3d — :180-299. n_components 1 or 3. */
+int sopht_eul_to_lag_interp_3d(int dtype, int n_components, void* lag_field,
+                               const void* const eul_field[3], const int64_t* eul_strides,
+                               int64_t nz, int64_t ny, int64_t nx, const void* weights,
+                               const int64_t* nearest_index, int64_t num_lag_nodes, double dx,
+                               void* stream);
+
+typedef struct sopht_spread3d sopht_spread3d;
+/* Workspace for deterministic spreading of up to max_lag_nodes nodes. */
+int sopht_spread3d_create(int64_t max_lag_nodes, sopht_spread3d** ws);
+int sopht_spread3d_destroy(sopht_spread3d* ws);
+/* lagrangian_to_eulerian_grid_interpolation_kernel_3d — :302-384. F[block(n)] += F_n w_n,
+ * summed per cell in increasing node order (the reference's serial order) without float
+ * atomics. Returns SOPHT_ERR_OUT_OF_DOMAIN (after the launch, on the next call that
+ * synchronises) if a support block leaves the grid; such contributions are dropped. */
+int sopht_lag_to_eul_spread_3d(int dtype, int n_components, sopht_spread3d* ws,
+                               void* const eul_field[3], const int64_t* eul_strides, int64_t nz,
+                               int64_t ny, int64_t nx, const void* lag_field,
+                               const void* weights, const int64_t* nearest_index,
+                               int64_t num_lag_nodes, void* stream);
+/* Reads back (synchronising `stream`) and clears the out-of-domain flag of ws. */
+int sopht_spread3d_check(sopht_spread3d* ws, void* stream);
+
+/* VirtualBoundaryForcing.compute_interaction_force_on_lag_grid —
+ * sopht/numeric/immersed_boundary_ops/VirtualBoundaryForcing.py:187-230 fused:
+ * support, weights, interpolation of the velocity, velocity mismatch (flow - body) and
+ * penalty force k*dX + c*dV in one pass over the nodes. Lagrangian arrays are (3, n)
+ * real except positions/body velocity (float64). */
+int sopht_vbf_lag_force_3d(int dtype, int kernel_type, const void* const velocity[3],
+                           const int64_t* velocity_strides, int64_t nz, int64_t ny, int64_t nx,
+                           const double* lag_positions, const double* lag_body_velocity,
+                           int64_t num_lag_nodes, double dx, double eul_grid_coord_shift,
+                           double stiffness, double damping, int64_t* nearest_index,
+                           void* support, void* weights, void* lag_flow_velocity,
+                           void* velocity_mismatch, const void* position_mismatch,
+                           void* lag_forcing, void* stream);
+
+/* VirtualBoundaryForcing.time_step — :149-164,276-283: dX += dt * dV. */
+int sopht_vbf_time_step_3d(int dtype, void* position_mismatch, const void* velocity_mismatch,
+                           int64_t num_lag_nodes, double dt, void* stream);
+
+/* ---------------------------------------------------------------- the fused flow step */
+
+typedef struct sopht_ns3d sopht_ns3d;
+
+/* UnboundedNavierStokesFlowSimulator3D — sopht/simulator/flow/navier_stokes_flow_simulators.py:226-447
+ * (greens-function Poisson solver, filter off). Owns the Poisson plan and step scratch. */
+int sopht_ns3d_create(int dtype, int64_t nz, int64_t ny, int64_t nx, double x_range,
+                      double kinematic_viscosity, double flow_density, int penalty_zone_width,
+                      int with_forcing, int with_free_stream_flow, void* stream,
+                      sopht_ns3d** plan);
+int sopht_ns3d_destroy(sopht_ns3d* plan);
+
+/* _navier_stokes_with_forcing_time_step / _navier_stokes_time_step — :449-498.
+ * Fields are contiguous (3, nz, ny, nx). forcing may be NULL when with_forcing is 0 and is
+ * zeroed after use. buffer is the simulator's buffer_vector_field (scratch; its first
+ * component holds the last diffusion flux as in the reference only in spirit: contents are
+ * unspecified after the call). free_stream: 3 doubles (host). */
+int sopht_ns3d_step(sopht_ns3d* plan, void* vorticity, void* velocity, void* stream_function,
+                    void* forcing, void* buffer, double dt, const double* free_stream,
+                    void* stream);
+
+/* The velocity recovery half of the step (:475-485): penalise vorticity (in place),
+ * Poisson solve into stream_function, curl into velocity with border reset, + free stream. */
+int sopht_ns3d_recover_velocity(sopht_ns3d* plan, void* vorticity, void* velocity,
+                                void* stream_function, const double* free_stream, void* stream);
+
+/* compute_advection_diffusion_stable_timestep's reduction —
+ * sopht/simulator/flow/passive_transport_flow_simulators.py:139-155:
+ * max over cells of |u_x|+|u_y|+|u_z| into *out (device scalar of dtype). */
+int sopht_ns3d_max_abs_velocity_sum(sopht_ns3d* plan, const void* velocity, void* out,
+                                    void* stream);
+
+/* Per-kernel timing of sopht_ns3d_step with HIP events on the launch stream (no
+ * counterpart in the reference, which has no profiling hooks). enable_timing creates the
+ * events; stage_times harvests (synchronising on the last recorded step), writes the
+ * accumulated milliseconds of up to n stages into ms, the number of timed steps into
+ * *steps, optionally resets, and returns the stage count. stage_name names stage i. */
+int sopht_ns3d_enable_timing(sopht_ns3d* plan, int enable);
+int sopht_ns3d_stage_times(sopht_ns3d* plan, double* ms, int n, int64_t* steps, int reset);
+const char* sopht_ns3d_stage_name(int stage);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SOPHT_AMD_H */

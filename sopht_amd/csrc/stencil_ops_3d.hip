@@ -1,0 +1,554 @@
+// Standalone drop-in grid stencils (the gen_*_pyst_kernel_3d family) for gfx950.
+//
+// These keep the reference's per-op semantics exactly (which cells each op writes, border
+// resets, strided views) and run one thread per written cell with x-fastest lanes, so
+// every neighbour access along x is coalesced and y/z neighbours come from L2. The flow
+// step itself does not use them: it runs the fused sweeps in flow_step_3d.hip.
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+
+#include "common.h"
+
+namespace sopht {
+
+static thread_local char g_last_error[512] = "";
+
+void set_last_error(const char* msg, const char* file, int line) {
+  snprintf(g_last_error, sizeof(g_last_error), "%s (%s:%d)", msg, file, line);
+}
+
+namespace {
+
+// Visit every cell of the box [z0, z0+bz) x [y0, y0+by) x [x0, x0+bx).
+template <typename Fn>
+__global__ void __launch_bounds__(256) box_kernel(int64_t z0, int64_t y0, int64_t x0, int64_t bz,
+                                                  int64_t by, int64_t bx, Fn fn) {
+  const int64_t total = bz * by * bx;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < total;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t x = i % bx;
+    const int64_t t = i / bx;
+    const int64_t y = t % by;
+    const int64_t z = t / by;
+    fn(z0 + z, y0 + y, x0 + x);
+  }
+}
+
+template <typename Fn>
+int launch_box(int64_t z0, int64_t y0, int64_t x0, int64_t bz, int64_t by, int64_t bx, Fn fn,
+               hipStream_t s) {
+  const int64_t total = bz * by * bx;
+  if (total <= 0) return SOPHT_OK;
+  int blocks = ceil_div(total, 256);
+  if (blocks > 65536) blocks = 65536;
+  box_kernel<<<blocks, 256, 0, s>>>(z0, y0, x0, bz, by, bx, fn);
+  SOPHT_CHECK_LAUNCH();
+  return SOPHT_OK;
+}
+
+// The six width-w boundary slabs of an (nz, ny, nx) box, written in the reference's order
+// (z front/back, y front/back, x front/back; elementwise_ops_3d.py:228-233). Overlaps are
+// written more than once with the same value.
+template <typename Fn>
+int launch_boundary_slabs(int64_t nz, int64_t ny, int64_t nx, int64_t w, Fn fn, hipStream_t s) {
+  int rc;
+  const int64_t wz = w < nz ? w : nz, wy = w < ny ? w : ny, wx = w < nx ? w : nx;
+  if ((rc = launch_box(0, 0, 0, wz, ny, nx, fn, s))) return rc;
+  if ((rc = launch_box(nz - wz, 0, 0, wz, ny, nx, fn, s))) return rc;
+  if ((rc = launch_box(0, 0, 0, nz, wy, nx, fn, s))) return rc;
+  if ((rc = launch_box(0, ny - wy, 0, nz, wy, nx, fn, s))) return rc;
+  if ((rc = launch_box(0, 0, 0, nz, ny, wx, fn, s))) return rc;
+  return launch_box(0, 0, nx - wx, nz, ny, wx, fn, s);
+}
+
+inline hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
+
+// ------------------------------------------------------------------ curl-type operators
+template <typename T>
+int curl_impl(void* const out[3], const int64_t* os, const void* const in[3], const int64_t* is,
+              int64_t nz, int64_t ny, int64_t nx, T p, int reset, bool accumulate,
+              hipStream_t s) {
+  View3<T> o0 = make_view<T>(out[0], os), o1 = make_view<T>(out[1], os),
+           o2 = make_view<T>(out[2], os);
+  View3<T> fx = make_view<T>(in[0], is), fy = make_view<T>(in[1], is), fz = make_view<T>(in[2], is);
+  int rc = SOPHT_OK;
+  if (nz > 2 && ny > 2 && nx > 2) {
+    rc = launch_box(
+        1, 1, 1, nz - 2, ny - 2, nx - 2,
+        [=] __device__(int64_t z, int64_t y, int64_t x) {
+          // curl_x = df_z/dy - df_y/dz, curl_y = df_x/dz - df_z/dx, curl_z = df_y/dx - df_x/dy
+          const T cx = p * (fz(z, y + 1, x) - fz(z, y - 1, x) - fy(z + 1, y, x) + fy(z - 1, y, x));
+          const T cy = p * (fx(z + 1, y, x) - fx(z - 1, y, x) - fz(z, y, x + 1) + fz(z, y, x - 1));
+          const T cz = p * (fy(z, y, x + 1) - fy(z, y, x - 1) - fx(z, y + 1, x) + fx(z, y - 1, x));
+          if (accumulate) {
+            o0(z, y, x) = o0(z, y, x) + cx;
+            o1(z, y, x) = o1(z, y, x) + cy;
+            o2(z, y, x) = o2(z, y, x) + cz;
+          } else {
+            o0(z, y, x) = cx;
+            o1(z, y, x) = cy;
+            o2(z, y, x) = cz;
+          }
+        },
+        s);
+    if (rc) return rc;
+  }
+  if (reset) {
+    rc = launch_boundary_slabs(
+        nz, ny, nx, 1,
+        [=] __device__(int64_t z, int64_t y, int64_t x) {
+          o0(z, y, x) = T(0);
+          o1(z, y, x) = T(0);
+          o2(z, y, x) = T(0);
+        },
+        s);
+  }
+  return rc;
+}
+
+template <typename T>
+int update_penalised_impl(void* const w[3], const int64_t* ws, const void* const up[3],
+                          const int64_t* ups, const void* const u[3], const int64_t* us,
+                          int64_t nz, int64_t ny, int64_t nx, T p, hipStream_t s) {
+  View3<T> w0 = make_view<T>(w[0], ws), w1 = make_view<T>(w[1], ws), w2 = make_view<T>(w[2], ws);
+  View3<T> px = make_view<T>(up[0], ups), py = make_view<T>(up[1], ups), pz = make_view<T>(up[2], ups);
+  View3<T> vx = make_view<T>(u[0], us), vy = make_view<T>(u[1], us), vz = make_view<T>(u[2], us);
+  if (nz <= 2 || ny <= 2 || nx <= 2) return SOPHT_OK;
+  return launch_box(
+      1, 1, 1, nz - 2, ny - 2, nx - 2,
+      [=] __device__(int64_t z, int64_t y, int64_t x) {
+        // update_vorticity_from_velocity_forcing_3d.py:175-184 (and the y, z analogues)
+        w0(z, y, x) = w0(z, y, x) + p * (pz(z, y + 1, x) - vz(z, y + 1, x) - pz(z, y - 1, x) +
+                                         vz(z, y - 1, x) - py(z + 1, y, x) + vy(z + 1, y, x) +
+                                         py(z - 1, y, x) - vy(z - 1, y, x));
+        w1(z, y, x) = w1(z, y, x) + p * (px(z + 1, y, x) - vx(z + 1, y, x) - px(z - 1, y, x) +
+                                         vx(z - 1, y, x) - pz(z, y, x + 1) + vz(z, y, x + 1) +
+                                         pz(z, y, x - 1) - vz(z, y, x - 1));
+        w2(z, y, x) = w2(z, y, x) + p * (py(z, y, x + 1) - vy(z, y, x + 1) - py(z, y, x - 1) +
+                                         vy(z, y, x - 1) - px(z, y + 1, x) + vx(z, y + 1, x) +
+                                         px(z, y - 1, x) - vx(z, y - 1, x));
+      },
+      s);
+}
+
+template <typename T>
+int cross_impl(void* const r[3], const int64_t* rs, const void* const a[3], const int64_t* as,
+               const void* const b[3], const int64_t* bs, int64_t nz, int64_t ny, int64_t nx,
+               hipStream_t s) {
+  View3<T> r0 = make_view<T>(r[0], rs), r1 = make_view<T>(r[1], rs), r2 = make_view<T>(r[2], rs);
+  View3<T> a0 = make_view<T>(a[0], as), a1 = make_view<T>(a[1], as), a2 = make_view<T>(a[2], as);
+  View3<T> b0 = make_view<T>(b[0], bs), b1 = make_view<T>(b[1], bs), b2 = make_view<T>(b[2], bs);
+  return launch_box(
+      0, 0, 0, nz, ny, nx,
+      [=] __device__(int64_t z, int64_t y, int64_t x) {
+        const T ax = a0(z, y, x), ay = a1(z, y, x), az = a2(z, y, x);
+        const T bx = b0(z, y, x), by = b1(z, y, x), bz = b2(z, y, x);
+        r0(z, y, x) = ay * bz - by * az;
+        r1(z, y, x) = az * bx - bz * ax;
+        r2(z, y, x) = ax * by - bx * ay;
+      },
+      s);
+}
+
+template <typename T>
+int diffusion_flux_impl(void* flux, const int64_t* fs, const void* field, const int64_t* is,
+                        int64_t nz, int64_t ny, int64_t nx, T p, int reset, hipStream_t s) {
+  View3<T> o = make_view<T>(flux, fs), f = make_view<T>(field, is);
+  int rc = SOPHT_OK;
+  if (nz > 2 && ny > 2 && nx > 2) {
+    rc = launch_box(
+        1, 1, 1, nz - 2, ny - 2, nx - 2,
+        [=] __device__(int64_t z, int64_t y, int64_t x) {
+          o(z, y, x) = p * (f(z + 1, y, x) + f(z - 1, y, x) + f(z, y + 1, x) + f(z, y - 1, x) +
+                            f(z, y, x + 1) + f(z, y, x - 1) - T(6) * f(z, y, x));
+        },
+        s);
+    if (rc) return rc;
+  }
+  if (reset)
+    rc = launch_boundary_slabs(
+        nz, ny, nx, 1, [=] __device__(int64_t z, int64_t y, int64_t x) { o(z, y, x) = T(0); }, s);
+  return rc;
+}
+
+template <typename T>
+int divergence_impl(void* div, const int64_t* ds, const void* const in[3], const int64_t* is,
+                    int64_t nz, int64_t ny, int64_t nx, T inv_dx, int reset, hipStream_t s) {
+  View3<T> o = make_view<T>(div, ds);
+  View3<T> fx = make_view<T>(in[0], is), fy = make_view<T>(in[1], is), fz = make_view<T>(in[2], is);
+  int rc = SOPHT_OK;
+  if (nz > 2 && ny > 2 && nx > 2) {
+    rc = launch_box(
+        1, 1, 1, nz - 2, ny - 2, nx - 2,
+        [=] __device__(int64_t z, int64_t y, int64_t x) {
+          o(z, y, x) = T(0.5) * inv_dx *
+                       (fx(z, y, x + 1) - fx(z, y, x - 1) + fy(z, y + 1, x) - fy(z, y - 1, x) +
+                        fz(z + 1, y, x) - fz(z - 1, y, x));
+        },
+        s);
+    if (rc) return rc;
+  }
+  if (reset)
+    rc = launch_boundary_slabs(
+        nz, ny, nx, 1, [=] __device__(int64_t z, int64_t y, int64_t x) { o(z, y, x) = T(0); }, s);
+  return rc;
+}
+
+// Penalisation, race-free in place: every border cell t takes the ORIGINAL value of its
+// clamped source c(t) (clamp of each index to [w-1, n-w]) times the three slab factors
+// (penalise_field_boundary_3d.py:384-403 collapsed to one pass). Targets are grouped by
+// source: one thread per source cell on the surface of the core box [w-1, n-w]^3 reads
+// its value once and writes its whole group (itself last), so no thread reads a cell
+// another thread writes.
+template <typename T>
+struct SineTables {
+  T f[3][8];  // front factors per axis (x, y, z), index i < w
+  T b[3][8];  // back factors per axis, index n-w+i
+};
+
+template <typename T>
+int penalise_impl(void* field, const int64_t* fs, int64_t nz, int64_t ny, int64_t nx, int w,
+                  const double* tables, hipStream_t s) {
+  if (w == 0) return SOPHT_OK;
+  SOPHT_REQUIRE(w > 0 && w <= 8, "penalty width must be in [0, 8]");
+  SOPHT_REQUIRE(2 * w <= nz && 2 * w <= ny && 2 * w <= nx, "grid too small for penalty width");
+  SineTables<T> st;
+  for (int a = 0; a < 3; ++a)
+    for (int i = 0; i < w; ++i) {
+      st.f[a][i] = T(tables[a * 2 * w + i]);
+      st.b[a][i] = T(tables[a * 2 * w + w + i]);
+    }
+  View3<T> f = make_view<T>(field, fs);
+  const int64_t cz0 = w - 1, cz1 = nz - w, cy0 = w - 1, cy1 = ny - w, cx0 = w - 1, cx1 = nx - w;
+  const int64_t bz = cz1 - cz0 + 1, by = cy1 - cy0 + 1, bx = cx1 - cx0 + 1;
+  return launch_box(
+      cz0, cy0, cx0, bz, by, bx,
+      [=] __device__(int64_t z, int64_t y, int64_t x) {
+        const bool onz = (z == cz0) || (z == cz1), ony = (y == cy0) || (y == cy1),
+                   onx = (x == cx0) || (x == cx1);
+        if (!(onz || ony || onx)) return;
+        const T v = f(z, y, x);
+        // target ranges and factors per axis: a source on the low face fans out to [0, w),
+        // on the high face to [n-w, n), else only itself (factor 1).
+        auto range = [&](int64_t c, int64_t c0, int64_t c1, int64_t n, int64_t& lo,
+                         int64_t& hi) {
+          if (c == c0) { lo = 0; hi = w; }
+          else if (c == c1) { lo = n - w; hi = n; }
+          else { lo = c; hi = c + 1; }
+          if (c0 == c1) { lo = 0; hi = n; }  // degenerate core (n == 2w - 1 is impossible here)
+        };
+        auto factor = [&](int a, int64_t i, int64_t n) -> T {
+          if (i < w) return st.f[a][i];
+          if (i >= n - w) return st.b[a][i - (n - w)];
+          return T(1);
+        };
+        int64_t zl, zh, yl, yh, xl, xh;
+        range(z, cz0, cz1, nz, zl, zh);
+        range(y, cy0, cy1, ny, yl, yh);
+        range(x, cx0, cx1, nx, xl, xh);
+        for (int64_t tz = zl; tz < zh; ++tz) {
+          const T sz = factor(2, tz, nz);
+          for (int64_t ty = yl; ty < yh; ++ty) {
+            const T sy = factor(1, ty, ny);
+            for (int64_t tx = xl; tx < xh; ++tx) {
+              if (tz == z && ty == y && tx == x) continue;
+              f(tz, ty, tx) = ((v * factor(0, tx, nx)) * sy) * sz;
+            }
+          }
+        }
+        f(z, y, x) = ((v * factor(0, x, nx)) * factor(1, y, ny)) * factor(2, z, nz);
+      },
+      s);
+}
+
+// ENO3 conservative advection flux (advection_flux_3d.py:12-233): the six sub-stencils
+// accumulate in the reference order on [2:-2]^3; one thread computes all six for its cell.
+template <typename T>
+int eno3_impl(void* flux, const int64_t* fs, const void* field, const int64_t* is,
+              const void* const vel[3], const int64_t* vs, int64_t nz, int64_t ny, int64_t nx,
+              T inv_dx, hipStream_t s) {
+  if (nz <= 4 || ny <= 4 || nx <= 4) return SOPHT_OK;
+  View3<T> o = make_view<T>(flux, fs), f = make_view<T>(field, is);
+  View3<T> u0 = make_view<T>(vel[0], vs), u1 = make_view<T>(vel[1], vs), u2 = make_view<T>(vel[2], vs);
+  return launch_box(
+      2, 2, 2, nz - 4, ny - 4, nx - 4,
+      [=] __device__(int64_t z, int64_t y, int64_t x) {
+        const T c13 = T(1.0 / 3.0), c56 = T(5.0 / 6.0), c16 = T(1.0 / 6.0);
+        T acc = o(z, y, x);
+        // axis a: 0 = x (u0), 1 = y (u1), 2 = z (u2)
+        for (int a = 0; a < 3; ++a) {
+          const View3<T>& u = a == 0 ? u0 : (a == 1 ? u1 : u2);
+          auto F = [&](int k) -> T {
+            return a == 0 ? f(z, y, x + k) : (a == 1 ? f(z, y + k, x) : f(z + k, y, x));
+          };
+          auto U = [&](int k) -> T {
+            return a == 0 ? u(z, y, x + k) : (a == 1 ? u(z, y + k, x) : u(z + k, y, x));
+          };
+          T front = (U(0) > -U(1)) ? (c13 * F(1) * U(1) + c56 * F(0) * U(0) - c16 * F(-1) * U(-1))
+                                   : (c13 * F(0) * U(0) + c56 * F(1) * U(1) - c16 * F(2) * U(2));
+          acc = acc + inv_dx * front;
+          T back = (U(0) > -U(-1)) ? (c13 * F(0) * U(0) + c56 * F(-1) * U(-1) - c16 * F(-2) * U(-2))
+                                   : (c13 * F(-1) * U(-1) + c56 * F(0) * U(0) - c16 * F(1) * U(1));
+          acc = acc - inv_dx * back;
+        }
+        o(z, y, x) = acc;
+      },
+      s);
+}
+
+template <typename T>
+int stretching_impl(void* const flux[3], const int64_t* fs, const void* const w[3],
+                    const int64_t* ws, const void* const u[3], const int64_t* us, int64_t nz,
+                    int64_t ny, int64_t nx, T p, hipStream_t s) {
+  View3<T> o0 = make_view<T>(flux[0], fs), o1 = make_view<T>(flux[1], fs), o2 = make_view<T>(flux[2], fs);
+  View3<T> wx = make_view<T>(w[0], ws), wy = make_view<T>(w[1], ws), wz = make_view<T>(w[2], ws);
+  View3<T> u0 = make_view<T>(u[0], us), u1 = make_view<T>(u[1], us), u2 = make_view<T>(u[2], us);
+  int rc = SOPHT_OK;
+  if (nz > 2 && ny > 2 && nx > 2) {
+    rc = launch_box(
+        1, 1, 1, nz - 2, ny - 2, nx - 2,
+        [=] __device__(int64_t z, int64_t y, int64_t x) {
+          const T a = wx(z, y, x), b = wy(z, y, x), c = wz(z, y, x);
+          auto g = [&](const View3<T>& v) -> T {
+            return p * (a * (v(z, y, x + 1) - v(z, y, x - 1)) + b * (v(z, y + 1, x) - v(z, y - 1, x)) +
+                        c * (v(z + 1, y, x) - v(z - 1, y, x)));
+          };
+          o0(z, y, x) = g(u0);
+          o1(z, y, x) = g(u1);
+          o2(z, y, x) = g(u2);
+        },
+        s);
+    if (rc) return rc;
+  }
+  return launch_boundary_slabs(
+      nz, ny, nx, 1,
+      [=] __device__(int64_t z, int64_t y, int64_t x) {
+        o0(z, y, x) = T(0);
+        o1(z, y, x) = T(0);
+        o2(z, y, x) = T(0);
+      },
+      s);
+}
+
+}  // namespace
+}  // namespace sopht
+
+using namespace sopht;
+
+#define DISPATCH(dtype, CALL)                                          \
+  do {                                                                 \
+    if ((dtype) == SOPHT_F32) {                                        \
+      using T = float;                                                 \
+      return CALL;                                                     \
+    } else if ((dtype) == SOPHT_F64) {                                 \
+      using T = double;                                                \
+      return CALL;                                                     \
+    }                                                                  \
+    set_last_error("dtype must be SOPHT_F32 or SOPHT_F64", __FILE__, __LINE__); \
+    return SOPHT_ERR_INVALID;                                          \
+  } while (0)
+
+#define REQUIRE_GRID(nz, ny, nx) \
+  SOPHT_REQUIRE((nz) > 0 && (ny) > 0 && (nx) > 0, "grid sizes must be positive")
+
+extern "C" {
+
+const char* sopht_last_error(void) { return g_last_error; }
+int sopht_version(void) { return 1; }
+
+int sopht_curl_3d(int dtype, void* const curl[3], const int64_t* cs, const void* const field[3],
+                  const int64_t* fs, int64_t nz, int64_t ny, int64_t nx, double prefactor,
+                  int reset, void* stream) {
+  REQUIRE_GRID(nz, ny, nx);
+  DISPATCH(dtype, curl_impl<T>(curl, cs, field, fs, nz, ny, nx, T(prefactor), reset, false,
+                               as_stream(stream)));
+}
+
+int sopht_update_vorticity_from_velocity_forcing_3d(int dtype, void* const w[3],
+                                                    const int64_t* ws, const void* const f[3],
+                                                    const int64_t* fs, int64_t nz, int64_t ny,
+                                                    int64_t nx, double prefactor, void* stream) {
+  REQUIRE_GRID(nz, ny, nx);
+  DISPATCH(dtype, curl_impl<T>(w, ws, f, fs, nz, ny, nx, T(prefactor), 0, true, as_stream(stream)));
+}
+
+int sopht_update_vorticity_from_penalised_velocity_3d(int dtype, void* const w[3],
+                                                      const int64_t* ws,
+                                                      const void* const up[3],
+                                                      const int64_t* ups, const void* const u[3],
+                                                      const int64_t* us, int64_t nz, int64_t ny,
+                                                      int64_t nx, double prefactor,
+                                                      void* stream) {
+  REQUIRE_GRID(nz, ny, nx);
+  DISPATCH(dtype, update_penalised_impl<T>(w, ws, up, ups, u, us, nz, ny, nx, T(prefactor),
+                                           as_stream(stream)));
+}
+
+int sopht_elementwise_cross_product_3d(int dtype, void* const r[3], const int64_t* rs,
+                                       const void* const a[3], const int64_t* as,
+                                       const void* const b[3], const int64_t* bs, int64_t nz,
+                                       int64_t ny, int64_t nx, void* stream) {
+  REQUIRE_GRID(nz, ny, nx);
+  DISPATCH(dtype, cross_impl<T>(r, rs, a, as, b, bs, nz, ny, nx, as_stream(stream)));
+}
+
+int sopht_diffusion_flux_3d(int dtype, void* flux, const int64_t* fs, const void* field,
+                            const int64_t* is, int64_t nz, int64_t ny, int64_t nx,
+                            double prefactor, int reset, void* stream) {
+  REQUIRE_GRID(nz, ny, nx);
+  DISPATCH(dtype, diffusion_flux_impl<T>(flux, fs, field, is, nz, ny, nx, T(prefactor), reset,
+                                         as_stream(stream)));
+}
+
+int sopht_elementwise_sum_3d(int dtype, void* sum, const int64_t* ss, const void* f1,
+                             const int64_t* s1, const void* f2, const int64_t* s2, int64_t nz,
+                             int64_t ny, int64_t nx, void* stream);
+
+int sopht_diffusion_timestep_euler_forward_3d(int dtype, void* field, const int64_t* fs,
+                                              void* flux, const int64_t* xs, int64_t nz,
+                                              int64_t ny, int64_t nx, double p, void* stream) {
+  int rc = sopht_diffusion_flux_3d(dtype, flux, xs, field, fs, nz, ny, nx, p, 1, stream);
+  if (rc) return rc;
+  return sopht_elementwise_sum_3d(dtype, field, fs, field, fs, flux, xs, nz, ny, nx, stream);
+}
+
+int sopht_penalise_field_boundary_3d(int dtype, void* field, const int64_t* fs, int64_t nz,
+                                     int64_t ny, int64_t nx, int width, const double* tables,
+                                     void* stream) {
+  REQUIRE_GRID(nz, ny, nx);
+  SOPHT_REQUIRE(width >= 0, "Invalid width for boundary zone, must be a non-negative integer");
+  DISPATCH(dtype, penalise_impl<T>(field, fs, nz, ny, nx, width, tables, as_stream(stream)));
+}
+
+int sopht_divergence_3d(int dtype, void* div, const int64_t* ds, const void* const field[3],
+                        const int64_t* fs, int64_t nz, int64_t ny, int64_t nx, double inv_dx,
+                        int reset, void* stream) {
+  REQUIRE_GRID(nz, ny, nx);
+  DISPATCH(dtype, divergence_impl<T>(div, ds, field, fs, nz, ny, nx, T(inv_dx), reset,
+                                     as_stream(stream)));
+}
+
+int sopht_set_fixed_val_3d(int dtype, void* field, const int64_t* st, int64_t nz, int64_t ny,
+                           int64_t nx, double value, void* stream) {
+  REQUIRE_GRID(nz, ny, nx);
+  DISPATCH(dtype, ([&] {
+             View3<T> f = make_view<T>(field, st);
+             const T v = T(value);
+             return launch_box(0, 0, 0, nz, ny, nx,
+                               [=] __device__(int64_t z, int64_t y, int64_t x) { f(z, y, x) = v; },
+                               as_stream(stream));
+           }()));
+}
+
+int sopht_add_fixed_val_3d(int dtype, void* sum, const int64_t* ss, const void* field,
+                           const int64_t* fs, int64_t nz, int64_t ny, int64_t nx, double value,
+                           void* stream) {
+  REQUIRE_GRID(nz, ny, nx);
+  DISPATCH(dtype, ([&] {
+             View3<T> o = make_view<T>(sum, ss), f = make_view<T>(field, fs);
+             const T v = T(value);
+             return launch_box(
+                 0, 0, 0, nz, ny, nx,
+                 [=] __device__(int64_t z, int64_t y, int64_t x) { o(z, y, x) = f(z, y, x) + v; },
+                 as_stream(stream));
+           }()));
+}
+
+int sopht_elementwise_saxpby_3d(int dtype, void* sum, const int64_t* ss, const void* f1,
+                                const int64_t* s1, const void* f2, const int64_t* s2, int64_t nz,
+                                int64_t ny, int64_t nx, double a, double b, void* stream) {
+  REQUIRE_GRID(nz, ny, nx);
+  DISPATCH(dtype, ([&] {
+             View3<T> o = make_view<T>(sum, ss), x1 = make_view<T>(f1, s1), x2 = make_view<T>(f2, s2);
+             const T ta = T(a), tb = T(b);
+             return launch_box(
+                 0, 0, 0, nz, ny, nx,
+                 [=] __device__(int64_t z, int64_t y, int64_t x) {
+                   o(z, y, x) = ta * x1(z, y, x) + tb * x2(z, y, x);
+                 },
+                 as_stream(stream));
+           }()));
+}
+
+int sopht_elementwise_sum_3d(int dtype, void* sum, const int64_t* ss, const void* f1,
+                             const int64_t* s1, const void* f2, const int64_t* s2, int64_t nz,
+                             int64_t ny, int64_t nx, void* stream) {
+  REQUIRE_GRID(nz, ny, nx);
+  DISPATCH(dtype, ([&] {
+             View3<T> o = make_view<T>(sum, ss), x1 = make_view<T>(f1, s1), x2 = make_view<T>(f2, s2);
+             return launch_box(
+                 0, 0, 0, nz, ny, nx,
+                 [=] __device__(int64_t z, int64_t y, int64_t x) {
+                   o(z, y, x) = x1(z, y, x) + x2(z, y, x);
+                 },
+                 as_stream(stream));
+           }()));
+}
+
+int sopht_elementwise_copy_3d(int dtype, void* field, const int64_t* st, const void* rhs,
+                              const int64_t* rs, int64_t nz, int64_t ny, int64_t nx,
+                              void* stream) {
+  REQUIRE_GRID(nz, ny, nx);
+  DISPATCH(dtype, ([&] {
+             View3<T> o = make_view<T>(field, st), r = make_view<T>(rhs, rs);
+             return launch_box(
+                 0, 0, 0, nz, ny, nx,
+                 [=] __device__(int64_t z, int64_t y, int64_t x) { o(z, y, x) = r(z, y, x); },
+                 as_stream(stream));
+           }()));
+}
+
+int sopht_set_fixed_val_at_boundaries_3d(int dtype, void* field, const int64_t* st, int64_t nz,
+                                         int64_t ny, int64_t nx, int width, double value,
+                                         void* stream) {
+  REQUIRE_GRID(nz, ny, nx);
+  SOPHT_REQUIRE(width > 0, "Invalid width for boundary zone, must be a positive integer");
+  DISPATCH(dtype, ([&] {
+             View3<T> f = make_view<T>(field, st);
+             const T v = T(value);
+             return launch_boundary_slabs(
+                 nz, ny, nx, width,
+                 [=] __device__(int64_t z, int64_t y, int64_t x) { f(z, y, x) = v; },
+                 as_stream(stream));
+           }()));
+}
+
+// Complex fields are interleaved (re, im); strides are in complex elements.
+int sopht_elementwise_complex_product_3d(int dtype, void* prod, const int64_t* ps,
+                                         const void* f1, const int64_t* s1, const void* f2,
+                                         const int64_t* s2, int64_t nz, int64_t ny, int64_t nx,
+                                         void* stream) {
+  REQUIRE_GRID(nz, ny, nx);
+  DISPATCH(dtype, ([&] {
+             View3<Cplx<T>> o = make_view<Cplx<T>>(prod, ps), a = make_view<Cplx<T>>(f1, s1),
+                            b = make_view<Cplx<T>>(f2, s2);
+             return launch_box(
+                 0, 0, 0, nz, ny, nx,
+                 [=] __device__(int64_t z, int64_t y, int64_t x) {
+                   const Cplx<T> u = a(z, y, x), v = b(z, y, x);
+                   o(z, y, x) = Cplx<T>{u.re * v.re - u.im * v.im, u.re * v.im + u.im * v.re};
+                 },
+                 as_stream(stream));
+           }()));
+}
+
+int sopht_advection_flux_conservative_eno3_3d(int dtype, void* flux, const int64_t* fs,
+                                              const void* field, const int64_t* is,
+                                              const void* const vel[3], const int64_t* vs,
+                                              int64_t nz, int64_t ny, int64_t nx, double inv_dx,
+                                              void* stream) {
+  REQUIRE_GRID(nz, ny, nx);
+  DISPATCH(dtype, eno3_impl<T>(flux, fs, field, is, vel, vs, nz, ny, nx, T(inv_dx),
+                               as_stream(stream)));
+}
+
+int sopht_vorticity_stretching_flux_3d(int dtype, void* const flux[3], const int64_t* fs,
+                                       const void* const w[3], const int64_t* ws,
+                                       const void* const u[3], const int64_t* us, int64_t nz,
+                                       int64_t ny, int64_t nx, double p, void* stream) {
+  REQUIRE_GRID(nz, ny, nx);
+  DISPATCH(dtype, stretching_impl<T>(flux, fs, w, ws, u, us, nz, ny, nx, T(p), as_stream(stream)));
+}
+
+}  // extern "C"

@@ -1,0 +1,33 @@
+"""Simulator layer: the 3D flow simulator and the immersed-body glue."""
+
+from .flow import (
+    FlowSimulator,
+    UnboundedNavierStokesFlowSimulator3D,
+    compute_advection_diffusion_stable_timestep,
+    create_unbounded_flow_simulator_3d,
+)
+from .immersed_body import (
+    FibonacciEllipsoidForcingGrid,
+    FibonacciSphereForcingGrid,
+    ImmersedBodyFlowInteraction,
+    ImmersedBodyForcingGrid,
+    RigidBody,
+    RigidBodyFlowInteraction,
+    SphereForcingGrid,
+    ThreeDimensionalRigidBodyForcingGrid,
+)
+
+__all__ = [
+    "FibonacciEllipsoidForcingGrid",
+    "FibonacciSphereForcingGrid",
+    "FlowSimulator",
+    "ImmersedBodyFlowInteraction",
+    "ImmersedBodyForcingGrid",
+    "RigidBody",
+    "RigidBodyFlowInteraction",
+    "SphereForcingGrid",
+    "ThreeDimensionalRigidBodyForcingGrid",
+    "UnboundedNavierStokesFlowSimulator3D",
+    "compute_advection_diffusion_stable_timestep",
+    "create_unbounded_flow_simulator_3d",
+]

@@ -1,0 +1,100 @@
+"""GPU parity of the free-space Poisson solve (pruned FFT passes) against the oracle and
+against the brute-force convolution; sizes include the reference test (n = 16, x_range 2),
+non-cubic grids, radix-3 lengths and strided operands. FP64 agreement is required to
+relative L2/Linf <= 1e-10; FP32 to the reference's 1e3*eps scaled by the solution size."""
+
+import numpy as np
+import pytest
+
+from helpers import PREC, rel_l2, rel_linf, to_dev, to_host
+from oracle import poisson as P
+
+pytestmark = pytest.mark.gpu
+
+
+def solver_cls():
+    from sopht_amd.numeric.eulerian_grid_ops import UnboundedPoissonSolverPYFFTW3D
+
+    return UnboundedPoissonSolverPYFFTW3D
+
+
+@pytest.mark.parametrize("precision", ["single", "double"])
+@pytest.mark.parametrize("shape,x_range", [((16, 16, 16), 2.0), ((8, 12, 16), 1.0), ((24, 24, 24), 1.0), ((6, 4, 8), 0.7)])
+def test_vector_solve_matches_oracle(dev, precision, shape, x_range, rng):
+    real_t = PREC[precision]
+    rhs = rng.standard_normal((3, *shape)).astype(real_t)
+    ref = np.zeros_like(rhs, dtype=np.float64)
+    P.UnboundedPoissonSolver3D(*shape, x_range=x_range, real_t=np.float64).vector_field_solve(ref, rhs.astype(np.float64))
+    s = solver_cls()(*shape, x_range=x_range, real_t=real_t)
+    out = to_dev(np.zeros_like(rhs), dev)
+    s.vector_field_solve(solution_vector_field=out, rhs_vector_field=to_dev(rhs, dev))
+    got = to_host(out)
+    bound = 1e-10 if real_t == np.float64 else 2e-5
+    assert rel_l2(got, ref) <= bound, rel_l2(got, ref)
+    assert rel_linf(got, ref) <= 5 * bound, rel_linf(got, ref)
+
+
+@pytest.mark.parametrize("precision", ["single", "double"])
+def test_scalar_solve_reference_case(dev, precision, rng):
+    """test_unbounded_poisson_solver_3d.py:101-125 (n = 16, x_range = 2)."""
+    real_t = PREC[precision]
+    rhs = rng.standard_normal((16, 16, 16)).astype(real_t)
+    ref = np.zeros_like(rhs)
+    P.UnboundedPoissonSolver3D(16, 16, 16, x_range=real_t(2.0), real_t=real_t).solve(ref, rhs)
+    s = solver_cls()(16, 16, 16, x_range=real_t(2.0), real_t=real_t)
+    out = to_dev(np.zeros_like(rhs), dev)
+    s.solve(solution_field=out, rhs_field=to_dev(rhs, dev))
+    np.testing.assert_allclose(to_host(out), ref, atol=real_t(1e3) * np.finfo(real_t).eps)
+
+
+def test_brute_force_fp64(dev, rng):
+    rhs = rng.standard_normal((6, 6, 6))
+    s = solver_cls()(6, 6, 6, x_range=1.3, real_t=np.float64)
+    out = to_dev(np.zeros_like(rhs), dev)
+    s.solve(out, to_dev(rhs, dev))
+    ref = P.brute_force_free_space_solve(rhs, s.dx)
+    assert rel_linf(to_host(out), ref) < 1e-12
+
+
+def test_rhs_untouched_and_strided(dev, rng):
+    rhs = rng.standard_normal((3, 16, 16, 16))
+    d_rhs = to_dev(rhs, dev)
+    big = to_dev(np.zeros((3, 18, 16, 20)), dev)
+    view = big[:, 1:17, :, 2:18]
+    s = solver_cls()(16, 16, 16, real_t=np.float64)
+    s.vector_field_solve(view, d_rhs)
+    np.testing.assert_array_equal(to_host(d_rhs), rhs)
+    ref = np.zeros_like(rhs)
+    P.UnboundedPoissonSolver3D(16, 16, 16).vector_field_solve(ref, rhs)
+    assert rel_l2(to_host(view), ref) < 1e-12
+    b = to_host(big)
+    assert np.all(b[:, 0] == 0) and np.all(b[:, :, :, :2] == 0)
+
+
+@pytest.mark.parametrize("n", [64, 96])
+def test_fp64_larger_grids(dev, n, rng):
+    rhs = rng.standard_normal((3, n, n, n))
+    s = solver_cls()(n, n, n, real_t=np.float64)
+    out = to_dev(np.zeros_like(rhs), dev)
+    s.vector_field_solve(out, to_dev(rhs, dev))
+    ref = np.zeros_like(rhs)
+    P.UnboundedPoissonSolver3D(n, n, n).vector_field_solve(ref, rhs)
+    assert rel_l2(to_host(out), ref) < 1e-12
+    assert rel_linf(to_host(out), ref) < 1e-11
+
+
+def test_linearity_at_256(dev):
+    """Size-independent property at the benchmark size: solve(a x + b y) = a solve(x) + b solve(y)."""
+    import torch
+
+    n = 256
+    g = torch.Generator(device=dev).manual_seed(5)
+    x = torch.randn((3, n, n, n), device=dev, dtype=torch.float32, generator=g)
+    y = torch.randn((3, n, n, n), device=dev, dtype=torch.float32, generator=g)
+    s = solver_cls()(n, n, n, real_t=np.float32)
+    px, py, pz = (torch.empty_like(x) for _ in range(3))
+    s.vector_field_solve(px, x)
+    s.vector_field_solve(py, y)
+    s.vector_field_solve(pz, 2.0 * x - 0.5 * y)
+    err = torch.linalg.vector_norm(pz - (2.0 * px - 0.5 * py)) / torch.linalg.vector_norm(pz)
+    assert err.item() < 1e-5

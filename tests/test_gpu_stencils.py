@@ -1,0 +1,199 @@
+"""GPU parity of the drop-in stencil factories against the oracle (whole arrays, so the
+write regions and border resets are checked too). Sizes follow the reference tests
+(n = 16) plus a non-cubic grid and strided views."""
+
+import numpy as np
+import pytest
+
+from helpers import PREC, to_dev, to_host, tol
+from oracle import stencils as S
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(16, 16, 16), (12, 10, 14)]
+
+
+def spne():
+    from sopht_amd.numeric import eulerian_grid_ops
+
+    return eulerian_grid_ops
+
+
+@pytest.fixture(params=["single", "double"])
+def real_t(request):
+    return PREC[request.param]
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("reset", [True, False])
+def test_curl(dev, real_t, shape, reset, rng):
+    f = rng.standard_normal((3, *shape)).astype(real_t)
+    out0 = (np.ones if reset else np.zeros)((3, *shape), dtype=real_t)
+    ref = out0.copy()
+    S.curl(ref, f, real_t(0.1), reset_ghost_zone=reset)
+    out = to_dev(out0, dev)
+    spne().gen_curl_pyst_kernel_3d(real_t, reset_ghost_zone=reset)(curl=out, field=to_dev(f, dev), prefactor=real_t(0.1))
+    np.testing.assert_allclose(to_host(out), ref, atol=tol(real_t), rtol=0)
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_update_vorticity(dev, real_t, shape, rng):
+    w = rng.standard_normal((3, *shape)).astype(real_t)
+    f = rng.standard_normal((3, *shape)).astype(real_t)
+    u = rng.standard_normal((3, *shape)).astype(real_t)
+    ref = w.copy()
+    S.update_vorticity_from_velocity_forcing(ref, f, real_t(0.2))
+    wd = to_dev(w, dev)
+    spne().gen_update_vorticity_from_velocity_forcing_pyst_kernel_3d(real_t)(
+        vorticity_field=wd, velocity_forcing_field=to_dev(f, dev), prefactor=real_t(0.2))
+    np.testing.assert_allclose(to_host(wd), ref, atol=tol(real_t), rtol=0)
+    ref2 = w.copy()
+    S.update_vorticity_from_penalised_velocity(ref2, f, u, real_t(0.2))
+    wd = to_dev(w, dev)
+    spne().gen_update_vorticity_from_penalised_velocity_pyst_kernel_3d(real_t)(
+        vorticity_field=wd, penalised_velocity_field=to_dev(f, dev), velocity_field=to_dev(u, dev),
+        prefactor=real_t(0.2))
+    np.testing.assert_allclose(to_host(wd), ref2, atol=tol(real_t), rtol=0)
+
+
+def test_cross_product(dev, real_t, rng):
+    a = rng.standard_normal((3, 16, 16, 16)).astype(real_t)
+    b = rng.standard_normal((3, 16, 16, 16)).astype(real_t)
+    ref = np.zeros_like(a)
+    S.elementwise_cross_product(ref, a, b)
+    r = to_dev(np.zeros_like(a), dev)
+    spne().gen_elementwise_cross_product_pyst_kernel_3d(real_t)(result_field=r, field_1=to_dev(a, dev), field_2=to_dev(b, dev))
+    np.testing.assert_allclose(to_host(r), ref, atol=tol(real_t), rtol=0)
+
+
+@pytest.mark.parametrize("field_type", ["scalar", "vector"])
+@pytest.mark.parametrize("reset", [True, False])
+def test_diffusion_flux_and_timestep(dev, real_t, field_type, reset, rng):
+    shape = (3, 16, 16, 16) if field_type == "vector" else (16, 16, 16)
+    f = rng.standard_normal(shape).astype(real_t)
+    flux0 = np.ones(shape, dtype=real_t)
+    ref = flux0.copy()
+    comps = [(ref, f)] if field_type == "scalar" else [(ref[i], f[i]) for i in range(3)]
+    for o, i in comps:
+        S.diffusion_flux(o, i, real_t(0.1), reset_ghost_zone=reset)
+    out = to_dev(flux0, dev)
+    spne().gen_diffusion_flux_pyst_kernel_3d(real_t, field_type=field_type, reset_ghost_zone=reset)(
+        out, to_dev(f, dev), real_t(0.1))
+    np.testing.assert_allclose(to_host(out), ref, atol=tol(real_t), rtol=0)
+    # Euler step shares one scalar flux buffer
+    reff = f.copy()
+    buf = np.zeros((16, 16, 16), dtype=real_t)
+    S.diffusion_timestep_euler_forward(reff, buf, real_t(0.1))
+    fd = to_dev(f, dev)
+    bd = to_dev(np.zeros((16, 16, 16), dtype=real_t), dev)
+    spne().gen_diffusion_timestep_euler_forward_pyst_kernel_3d(real_t, field_type=field_type)(fd, bd, real_t(0.1))
+    np.testing.assert_allclose(to_host(fd), reff, atol=tol(real_t), rtol=0)
+    np.testing.assert_allclose(to_host(bd), buf, atol=tol(real_t), rtol=0)
+
+
+@pytest.mark.parametrize("width", [0, 1, 2, 4])
+@pytest.mark.parametrize("field_type", ["scalar", "vector"])
+def test_penalise_field_boundary(dev, real_t, width, field_type, rng):
+    n = 16
+    dx = real_t(0.1)
+    c = np.linspace(real_t(dx / 2), 1 - real_t(dx / 2), n).astype(real_t)
+    zg, yg, xg = np.meshgrid(c, c, c, indexing="ij")
+    shape = (3, n, n, n) if field_type == "vector" else (n, n, n)
+    f = rng.standard_normal(shape).astype(real_t)
+    ref = f.copy()
+    if field_type == "vector":
+        S.penalise_vector_field_boundary(ref, width, dx, xg, yg, zg)
+    else:
+        S.penalise_field_boundary(ref, width, dx, xg, yg, zg)
+    fd = to_dev(f, dev)
+    k = spne().gen_penalise_field_boundary_pyst_kernel_3d(width, dx, xg, yg, zg, real_t, field_type=field_type)
+    k(fd)
+    np.testing.assert_allclose(to_host(fd), ref, atol=tol(real_t), rtol=0)
+
+
+def test_divergence(dev, real_t, rng):
+    f = rng.standard_normal((3, 16, 16, 16)).astype(real_t)
+    ref = np.ones((16, 16, 16), dtype=real_t)
+    S.divergence(ref, f, real_t(16.0))
+    d = to_dev(np.ones((16, 16, 16), dtype=real_t), dev)
+    spne().gen_divergence_pyst_kernel_3d(real_t)(divergence=d, field=to_dev(f, dev), inv_dx=real_t(16.0))
+    np.testing.assert_allclose(to_host(d), ref, atol=16 * tol(real_t), rtol=0)
+
+
+def test_elementwise_family(dev, real_t, rng):
+    k = spne()
+    a = rng.standard_normal((3, 8, 9, 10)).astype(real_t)
+    b = rng.standard_normal((3, 8, 9, 10)).astype(real_t)
+    ad, bd = to_dev(a, dev), to_dev(b, dev)
+    s = to_dev(np.zeros_like(a), dev)
+    k.gen_elementwise_sum_pyst_kernel_3d(real_t, field_type="vector")(s, ad, bd)
+    np.testing.assert_array_equal(to_host(s), a + b)
+    k.gen_elementwise_saxpby_pyst_kernel_3d(real_t, field_type="vector")(s, ad, bd, 0.75, 0.25)
+    np.testing.assert_allclose(to_host(s), real_t(0.75) * a + real_t(0.25) * b, atol=tol(real_t))
+    k.gen_set_fixed_val_pyst_kernel_3d(real_t, field_type="vector")(s, (1.0, 2.0, 3.0))
+    assert np.all(to_host(s)[2] == 3.0)
+    k.gen_add_fixed_val_pyst_kernel_3d(real_t, field_type="vector")(s, ad, (1.0, 2.0, 3.0))
+    np.testing.assert_array_equal(to_host(s), a + np.array([1, 2, 3], dtype=real_t).reshape(3, 1, 1, 1))
+    k.gen_elementwise_copy_pyst_kernel_3d(real_t)(s[1], bd[0])
+    np.testing.assert_array_equal(to_host(s)[1], b[0])
+    one = to_dev(np.ones((8, 9, 10), dtype=real_t), dev)
+    k.gen_set_fixed_val_at_boundaries_pyst_kernel_3d(real_t, width=2)(one, 0.0)
+    assert to_host(one).sum() == 4 * 5 * 6
+    ca = (a[0] + 1j * a[1]).astype(np.complex64 if real_t == np.float32 else np.complex128)
+    cb = (b[0] + 1j * b[1]).astype(ca.dtype)
+    cp = to_dev(np.zeros_like(ca), dev)
+    k.gen_elementwise_complex_product_pyst_kernel_3d(real_t)(cp, to_dev(ca, dev), to_dev(cb, dev))
+    np.testing.assert_allclose(to_host(cp), ca * cb, atol=tol(real_t))
+
+
+def test_strided_views(dev, real_t, rng):
+    """Slabs and padded sub-blocks are legal operands, as in the reference (strided views)."""
+    big = rng.standard_normal((3, 12, 14, 16)).astype(real_t)
+    sub = big[:, 1:-1, 2:, ::2]
+    ref = np.zeros((3, *sub.shape[1:]), dtype=real_t)
+    S.curl(ref, sub, real_t(0.3))
+    bd = to_dev(big, dev)
+    out = to_dev(np.zeros((3, 20, 12, 8), dtype=real_t), dev)[:, 5:15]
+    spne().gen_curl_pyst_kernel_3d(real_t)(curl=out, field=bd[:, 1:-1, 2:, ::2], prefactor=real_t(0.3))
+    np.testing.assert_allclose(to_host(out), ref, atol=tol(real_t), rtol=0)
+
+
+def test_eno3_flux_and_timestep(dev, real_t, rng):
+    n = 16
+    f = rng.standard_normal((n, n, n)).astype(real_t)
+    u = rng.standard_normal((3, n, n, n)).astype(real_t)
+    ref = rng.standard_normal((n, n, n)).astype(real_t)
+    flux0 = ref.copy()
+    S.advection_flux_conservative_eno3(ref, f, u, real_t(0.1))
+    fl = to_dev(flux0, dev)
+    spne().gen_advection_flux_conservative_eno3_pyst_kernel_3d(real_t)(fl, to_dev(f, dev), to_dev(u, dev), real_t(0.1))
+    np.testing.assert_allclose(to_host(fl), ref, atol=tol(real_t), rtol=0)
+    v = rng.standard_normal((3, n, n, n)).astype(real_t)
+    refv = v.copy()
+    S.advection_timestep_euler_forward_conservative_eno3(refv, np.zeros((n, n, n), dtype=real_t), u, real_t(0.05))
+    vd = to_dev(v, dev)
+    spne().gen_advection_timestep_euler_forward_conservative_eno3_pyst_kernel_3d(real_t, field_type="vector")(
+        vd, to_dev(np.zeros((n, n, n), dtype=real_t), dev), to_dev(u, dev), real_t(0.05))
+    np.testing.assert_allclose(to_host(vd), refv, atol=tol(real_t), rtol=0)
+
+
+def test_vorticity_stretching(dev, real_t, rng):
+    n = 16
+    w = rng.standard_normal((3, n, n, n)).astype(real_t)
+    u = rng.standard_normal((3, n, n, n)).astype(real_t)
+    ref = np.zeros_like(w)
+    S.vorticity_stretching_flux(ref, w, u, real_t(0.1))
+    out = to_dev(np.ones_like(w), dev)
+    spne().gen_vorticity_stretching_flux_pyst_kernel_3d(real_t)(out, to_dev(w, dev), to_dev(u, dev), real_t(0.1))
+    np.testing.assert_allclose(to_host(out), ref, atol=tol(real_t), rtol=0)
+    r1 = w.copy()
+    S.vorticity_stretching_timestep_euler_forward(r1, u, np.zeros_like(w), real_t(0.1))
+    wd = to_dev(w, dev)
+    spne().gen_vorticity_stretching_timestep_euler_forward_pyst_kernel_3d(real_t)(wd, to_dev(u, dev), to_dev(np.zeros_like(w), dev), real_t(0.1))
+    np.testing.assert_allclose(to_host(wd), r1, atol=tol(real_t), rtol=0)
+    r3 = w.copy()
+    S.vorticity_stretching_timestep_ssprk3(r3, u, np.zeros_like(w), real_t(0.1), np.zeros_like(w))
+    wd = to_dev(w, dev)
+    mid = to_dev(np.zeros_like(w), dev)
+    spne().gen_vorticity_stretching_timestep_ssprk3_pyst_kernel_3d(real_t, mid)(wd, to_dev(u, dev), to_dev(np.zeros_like(w), dev), real_t(0.1))
+    np.testing.assert_allclose(to_host(wd), r3, atol=10 * tol(real_t), rtol=0)
