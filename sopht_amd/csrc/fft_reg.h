@@ -1,0 +1,280 @@
+// Register-resident mixed-radix Stockham FFT for gfx950 (see LinePlan below): a team of
+// T threads per line, E elements per thread, butterflies in registers and LDS only for the
+// exchanges between stages. Zero inputs and unused outputs known at compile time are
+// pruned by constant propagation (the register DFTs are fully unrolled).
+//
+// Register DFTs are recursive decimation-in-time codelets; their twiddles are constants
+// computed at compile time (exact octant reduction plus Taylor series in double). The
+// runtime stage twiddles come from a table tw[m] = exp(-2 pi i m / L) built in FP64 on the
+// host.
+#pragma once
+
+#include <type_traits>
+
+#include "common.h"
+
+namespace sopht {
+namespace rfft {
+
+// ------------------------------------------------------------------ compile-time loops
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
+// ------------------------------------------------------------------ constexpr twiddles
+struct CD {
+  double re, im;
+};
+
+constexpr double taylor_sin(double x) {  // |x| <= pi/4
+  double x2 = x * x, term = x, sum = x;
+  for (int k = 1; k < 14; ++k) {
+    term *= -x2 / double((2 * k) * (2 * k + 1));
+    sum += term;
+  }
+  return sum;
+}
+
+constexpr double taylor_cos(double x) {  // |x| <= pi/4
+  double x2 = x * x, term = 1.0, sum = 1.0;
+  for (int k = 1; k < 14; ++k) {
+    term *= -x2 / double((2 * k - 1) * (2 * k));
+    sum += term;
+  }
+  return sum;
+}
+
+// exp(-2 pi i m / n), reduced exactly to an angle phi <= pi/4 plus a quarter-turn
+constexpr CD twiddle(long m, long n) {
+  m %= n;
+  if (m < 0) m += n;
+  const long q = (4 * m) / n, r = (4 * m) % n;
+  const double half_pi = 1.57079632679489661923132169163975144;
+  double c = 0, s = 0;  // cos, sin of the in-quadrant angle
+  if (2 * r <= n) {
+    const double phi = half_pi * (double(r) / double(n));
+    c = taylor_cos(phi);
+    s = taylor_sin(phi);
+  } else {
+    const double phi = half_pi * (double(n - r) / double(n));
+    c = taylor_sin(phi);
+    s = taylor_cos(phi);
+  }
+  double ct = c, st = s;
+  switch (q & 3) {
+    case 1: ct = -s; st = c; break;
+    case 2: ct = -c; st = -s; break;
+    case 3: ct = s; st = -c; break;
+    default: break;
+  }
+  return CD{ct, -st};
+}
+
+// x * w_N^E (forward) or x * conj(w_N^E) (inverse); exact special cases avoid work
+template <typename T, int N, int E, bool INV>
+__device__ __forceinline__ Cplx<T> twmul(Cplx<T> x) {
+  constexpr int e = ((E % N) + N) % N;
+  if constexpr (e == 0) {
+    return x;
+  } else if constexpr (4 * e == N) {  // w = -i
+    return INV ? Cplx<T>{-x.im, x.re} : Cplx<T>{x.im, -x.re};
+  } else if constexpr (2 * e == N) {  // w = -1
+    return Cplx<T>{-x.re, -x.im};
+  } else if constexpr (4 * e == 3 * N) {  // w = +i
+    return INV ? Cplx<T>{x.im, -x.re} : Cplx<T>{-x.im, x.re};
+  } else {
+    constexpr CD w = twiddle(e, N);
+    const T wr = T(w.re), wi = T(INV ? -w.im : w.im);
+    return Cplx<T>{x.re * wr - x.im * wi, x.re * wi + x.im * wr};
+  }
+}
+
+// ------------------------------------------------------------------ register codelets
+template <typename T, bool INV>
+__device__ __forceinline__ void dft2(Cplx<T>* v) {
+  const Cplx<T> a = v[0], b = v[1];
+  v[0] = cadd(a, b);
+  v[1] = csub(a, b);
+}
+
+template <typename T, bool INV>
+__device__ __forceinline__ void dft3(Cplx<T>* v) {
+  const T h = T(0.86602540378443864676372317075294);  // sqrt(3)/2
+  const Cplx<T> t1 = cadd(v[1], v[2]);
+  const Cplx<T> t2 = {v[0].re - T(0.5) * t1.re, v[0].im - T(0.5) * t1.im};
+  const Cplx<T> d = csub(v[1], v[2]);
+  const Cplx<T> t3 = mul_mi<T, INV>(Cplx<T>{h * d.re, h * d.im});
+  v[0] = cadd(v[0], t1);
+  v[1] = cadd(t2, t3);
+  v[2] = csub(t2, t3);
+}
+
+template <typename T, bool INV>
+__device__ __forceinline__ void dft4(Cplx<T>* v) {
+  const Cplx<T> t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+  const Cplx<T> t2 = cadd(v[1], v[3]), t3 = mul_mi<T, INV>(csub(v[1], v[3]));
+  v[0] = cadd(t0, t2);
+  v[2] = csub(t0, t2);
+  v[1] = cadd(t1, t3);
+  v[3] = csub(t1, t3);
+}
+
+// DFT of length N in registers (unnormalised; INV uses the conjugate roots)
+template <typename T, int N, bool INV>
+__device__ __forceinline__ void rdft(Cplx<T>* v) {
+  if constexpr (N == 1) {
+  } else if constexpr (N == 2) {
+    dft2<T, INV>(v);
+  } else if constexpr (N == 3) {
+    dft3<T, INV>(v);
+  } else if constexpr (N == 4) {
+    dft4<T, INV>(v);
+  } else {
+    // decimation in time: N = R * M, X[k + M q] = sum_r w_N^{r k} w_R^{r q} S_r[k]
+    constexpr int R = (N % 4 == 0) ? 4 : ((N % 2 == 0) ? 2 : 3);
+    constexpr int M = N / R;
+    Cplx<T> s[R][M];
+    static_for<0, R>([&](auto r) {
+      static_for<0, M>([&](auto m) { s[r][m] = v[decltype(r)::value + R * decltype(m)::value]; });
+    });
+    static_for<0, R>([&](auto r) { rdft<T, M, INV>(s[decltype(r)::value]); });
+    static_for<0, M>([&](auto k) {
+      constexpr int kk = decltype(k)::value;
+      Cplx<T> t[R];
+      static_for<0, R>([&](auto r) {
+        constexpr int rr = decltype(r)::value;
+        t[rr] = twmul<T, N, rr * kk, INV>(s[rr][kk]);
+      });
+      if constexpr (R == 2) dft2<T, INV>(t);
+      else if constexpr (R == 3) dft3<T, INV>(t);
+      else dft4<T, INV>(t);
+      static_for<0, R>([&](auto q) { v[kk + M * decltype(q)::value] = t[decltype(q)::value]; });
+    });
+  }
+}
+
+// ------------------------------------------------------------------ line plans
+// A line of length L is transformed by a team of T = L / E threads; thread t holds the E
+// elements x[t + T e] (e < E). Stage s of the Stockham sequence has radix R_s, with R_s | E:
+// each thread computes E / R_s butterflies j = t + p T, reading j + r L/R_s and writing
+// (j / Ns) Ns R_s + j % Ns + r Ns (Ns = product of the earlier radices), with twiddles
+// w_{Ns R_s}^{r (j % Ns)}. The first stage's inputs and the last stage's outputs are both
+// x[t + T e] held by thread t, so a forward transform, a pointwise multiply and an
+// inverse transform need no reordering, and the zero-padded half of a line (t + T e >= L/2,
+// i.e. e >= E/2) is known at compile time. Between stages the elements are exchanged
+// through LDS (one write, one barrier, one read, one barrier).
+template <int... Rs>
+struct Radices {};
+
+template <int L>
+struct LinePlan;
+#define SOPHT_LINE_PLAN(L_, E_, ...)                                        \
+  template <>                                                               \
+  struct LinePlan<L_> {                                                     \
+    static constexpr int E = E_, T = L_ / E_;                               \
+    using R = Radices<__VA_ARGS__>;                                         \
+  };
+SOPHT_LINE_PLAN(8, 8, 8)
+SOPHT_LINE_PLAN(12, 12, 12)
+SOPHT_LINE_PLAN(16, 8, 8, 2)
+SOPHT_LINE_PLAN(24, 12, 2, 12)
+SOPHT_LINE_PLAN(32, 8, 8, 4)
+SOPHT_LINE_PLAN(48, 12, 4, 12)
+SOPHT_LINE_PLAN(64, 8, 8, 8)
+SOPHT_LINE_PLAN(96, 12, 4, 4, 6)
+SOPHT_LINE_PLAN(128, 8, 8, 4, 4)
+SOPHT_LINE_PLAN(192, 12, 4, 4, 12)
+SOPHT_LINE_PLAN(256, 8, 8, 8, 4)
+SOPHT_LINE_PLAN(384, 24, 8, 4, 12)
+SOPHT_LINE_PLAN(512, 8, 8, 8, 8)
+SOPHT_LINE_PLAN(768, 24, 8, 8, 12)
+SOPHT_LINE_PLAN(1024, 8, 8, 8, 4, 4)
+#undef SOPHT_LINE_PLAN
+
+// LDS exchange layouts: map element index e (0..L-1) of the caller's line to an LDS index.
+// Column tiles (lanes run over WC neighbouring lines, w fastest):
+template <int WC>
+struct ColSlot {
+  int w;
+  __device__ __forceinline__ int operator()(int e) const { return e * WC + w; }
+};
+// Row tiles (lanes run over the T threads of one line): one pad entry every 32 elements
+// breaks the power-of-two strides of the butterfly writes across LDS banks.
+template <int L>
+struct RowSlot {
+  int line;
+  static constexpr int kStride = L + L / 32 + 1;
+  __device__ __forceinline__ int operator()(int e) const { return line * kStride + e + (e >> 5); }
+};
+
+template <typename T, int L, int R, int NS, bool INV, typename Slot>
+__device__ __forceinline__ void stockham_stage(Cplx<T>* v, Cplx<T>* xb, const Cplx<T>* tw,
+                                               int t, Slot slot) {
+  constexpr int E = LinePlan<L>::E, TT = LinePlan<L>::T;
+  constexpr int PER = E / R;
+  static_assert(E % R == 0, "radix must divide the elements per thread");
+  // butterfly p takes v[p + PER r] (= x[j + r L/R], j = t + p T)
+  static_for<0, PER>([&](auto p_) {
+    constexpr int p = decltype(p_)::value;
+    Cplx<T> u[R];
+    static_for<0, R>([&](auto r_) { u[decltype(r_)::value] = v[p + PER * decltype(r_)::value]; });
+    if constexpr (NS > 1) {
+      const int k = (t + p * TT) % NS;
+      static_for<1, R>([&](auto r_) {
+        constexpr int r = decltype(r_)::value;
+        const Cplx<T> w = tw[r * k * (L / (NS * R))];
+        u[r] = INV ? cmulc(u[r], w) : cmul(u[r], w);
+      });
+    }
+    rdft<T, R, INV>(u);
+    static_for<0, R>([&](auto r_) { v[p + PER * decltype(r_)::value] = u[decltype(r_)::value]; });
+  });
+  if constexpr (NS * R < L) {
+    // exchange: outputs of butterfly j go to (j/NS) NS R + j%NS + r NS; the next stage reads
+    // x[t + T e] again
+    static_for<0, PER>([&](auto p_) {
+      constexpr int p = decltype(p_)::value;
+      const int j = t + p * TT;
+      const int base = (j / NS) * NS * R + j % NS;
+      static_for<0, R>([&](auto r_) {
+        constexpr int r = decltype(r_)::value;
+        xb[slot(base + r * NS)] = v[p + PER * r];
+      });
+    });
+    __syncthreads();
+    static_for<0, E>([&](auto e_) {
+      constexpr int e = decltype(e_)::value;
+      v[e] = xb[slot(t + TT * e)];
+    });
+    __syncthreads();
+  }
+}
+
+template <typename T, int L, bool INV, int NS, typename Slot, int R0, int... Rest>
+__device__ __forceinline__ void stockham_run(Cplx<T>* v, Cplx<T>* xb, const Cplx<T>* tw, int t,
+                                             Slot slot) {
+  stockham_stage<T, L, R0, NS, INV>(v, xb, tw, t, slot);
+  if constexpr (sizeof...(Rest) > 0) stockham_run<T, L, INV, NS * R0, Slot, Rest...>(v, xb, tw, t, slot);
+}
+
+template <typename T, int L, bool INV, typename Slot, int... Rs>
+__device__ __forceinline__ void stockham_plan(Cplx<T>* v, Cplx<T>* xb, const Cplx<T>* tw, int t,
+                                              Slot slot, Radices<Rs...>) {
+  stockham_run<T, L, INV, 1, Slot, Rs...>(v, xb, tw, t, slot);
+}
+
+// In-place transform of a line held as v[e] = x[t + T e] by thread t of its team; on return
+// v[e] = X[t + T e] (forward: exp(-), inverse: exp(+), unnormalised). Every thread of the
+// workgroup calls this together; the exchange area xb is free again on return.
+template <typename T, int L, bool INV, typename Slot>
+__device__ __forceinline__ void team_fft(Cplx<T>* v, Cplx<T>* xb, const Cplx<T>* tw, int t,
+                                         Slot slot) {
+  stockham_plan<T, L, INV>(v, xb, tw, t, slot, typename LinePlan<L>::R{});
+}
+
+}  // namespace rfft
+}  // namespace sopht

@@ -22,8 +22,9 @@
 struct sopht_spread3d {
   int64_t capacity;   // max nodes
   int64_t padded;     // power of two >= capacity
-  uint64_t* keys;     // (key << 32 | node), padded
-  int* flag;          // out-of-domain flag
+  uint64_t* keys;     // (key << 32 | node), sorted, padded
+  uint64_t* prev;     // unsorted keys of the previous call, padded
+  int* flag;          // [0] out-of-domain, [1] keys changed since the last sort
 };
 
 namespace sopht {
@@ -187,30 +188,55 @@ __global__ void k_interp(int ncomp, T* lag, Eul3<T> u, int64_t nz, int64_t ny, i
   for (int c = 0; c < ncomp; ++c) lag[c * n + i] = out[c];
 }
 
-// Fused virtual-boundary force on the Lagrangian grid.
+// Fused virtual-boundary force on the Lagrangian grid: one wavefront per node, lane =
+// support slot (kz, ky, kx). Each lane computes its weight, reads u at its cell, and the
+// 64 products are summed with a butterfly reduction; lane 0 forms U, dV and F.
 template <typename T>
-__global__ void k_vbf_lag_force(int kernel_type, Eul3<T> u, int64_t nz, int64_t ny, int64_t nx,
-                                const double* X, const double* V, int64_t n, double dx,
-                                double shift, double k, double c, int64_t* idx, T* support, T* weights,
-                                T* lag_u, T* dV, const T* dX, T* F, int* flag) {
-  const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-  if (i >= n) return;
-  int64_t base[3];
-  T wloc[64];
-  node_support_weights<T>(kernel_type, X, n, i, dx, shift, idx, support, weights, base, wloc);
-  T U[3] = {T(0), T(0), T(0)};
+__global__ void __launch_bounds__(256) k_vbf_lag_force(int kernel_type, Eul3<T> u, int64_t nz,
+                                                       int64_t ny, int64_t nx, const double* X,
+                                                       const double* V, int64_t n, double dx,
+                                                       double shift, double k, double c,
+                                                       int64_t* idx, T* support, T* weights,
+                                                       T* lag_u, T* dV, const T* dX, T* F) {
+  const int lane = threadIdx.x & 63;
+  const int64_t i = int64_t(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (i >= n) return;  // whole wavefronts leave together; no barriers follow
+  const int off[3] = {lane & 3, (lane >> 2) & 3, lane >> 4};  // kx, ky, kz
   const T tdx = T(dx);
-  if (block_inside<T>(base, nz, ny, nx)) {
-    gather_node<T>(u, 3, base, wloc, 1, tdx * tdx * tdx, U);
-  } else if (flag) {
-    atomicOr(flag, 1);
-  }
+  int64_t base[3];
+  T s[3];
   for (int a = 0; a < 3; ++a) {
-    lag_u[a * n + i] = U[a];
-    const T dv = T(U[a] - V[a * n + i]);
-    dV[a * n + i] = dv;
-    // F = k dX + c dV evaluated in FP64 (the reference's rescaled coefficients are float64)
-    F[a * n + i] = T(k * double(dX[a * n + i]) + c * double(dv));
+    const double xa = X[a * n + i];
+    const int64_t ia = int64_t(np_floor_divide(xa - shift, dx));
+    base[a] = ia - 1;
+    if (lane == 0) idx[a * n + i] = ia;
+    // reference: support stored in real_t, then divided by dx (cosine) or |.|/dx (peskin)
+    const T dr = T(double(ia + off[a] - 1) * dx + shift - xa);
+    s[a] = kernel_type == 0 ? dr / tdx : fabs(dr) / tdx;
+    support[(int64_t(a) * 64 + lane) * n + i] = s[a];
+  }
+  const T pre = kernel_type == 0 ? T((T(0.25) / tdx) * (T(0.25) / tdx) * (T(0.25) / tdx))
+                                 : T((T(0.125) / tdx) * (T(0.125) / tdx) * (T(0.125) / tdx));
+  T w = pre;
+  for (int a = 0; a < 3; ++a) w = w * (kernel_type == 0 ? cosine_phi<T>(s[a]) : peskin_phi<T>(s[a]));
+  weights[int64_t(lane) * n + i] = w;
+  T p[3] = {T(0), T(0), T(0)};
+  if (block_inside<T>(base, nz, ny, nx)) {
+    const int64_t cell = (base[2] + off[2]) * u.sz + (base[1] + off[1]) * u.sy + (base[0] + off[0]) * u.sx;
+    for (int a = 0; a < 3; ++a) p[a] = u.p[a][cell] * w;
+  }
+  for (int m = 32; m >= 1; m >>= 1)
+    for (int a = 0; a < 3; ++a) p[a] += __shfl_xor(p[a], m, 64);
+  if (lane == 0) {
+    const T dx3 = tdx * tdx * tdx;
+    for (int a = 0; a < 3; ++a) {
+      const T U = p[a] * dx3;
+      lag_u[a * n + i] = U;
+      const T dv = T(U - V[a * n + i]);
+      dV[a * n + i] = dv;
+      // F = k dX + c dV evaluated in FP64 (the reference's rescaled coefficients are float64)
+      F[a * n + i] = T(k * double(dX[a * n + i]) + c * double(dv));
+    }
   }
 }
 
@@ -222,7 +248,7 @@ __global__ void k_vbf_time_step(T* dX, const T* dV, int64_t m, T dt) {
 
 // -------------------------------------------------------------------- spreading
 __global__ void k_spread_keys(const int64_t* idx, int64_t n, int64_t padded, int64_t nz,
-                              int64_t ny, int64_t nx, uint64_t* keys, int* flag) {
+                              int64_t ny, int64_t nx, uint64_t* prev, int* flag) {
   const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (i >= padded) return;
   uint64_t key = ~uint64_t(0);
@@ -235,13 +261,20 @@ __global__ void k_spread_keys(const int64_t* idx, int64_t n, int64_t padded, int
       atomicOr(flag, 1);
     }
   }
-  keys[i] = key;
+  // the sorted order is reused while every node keeps its support block
+  if (prev[i] != key) {
+    atomicOr(flag + 1, 1);
+    prev[i] = key;
+  }
 }
 
-// Bitonic sort of `padded` (power of two) keys within one workgroup in LDS.
-__global__ void __launch_bounds__(1024) k_bitonic_lds(uint64_t* keys, int padded) {
+// Stable sort of the packed keys (bitonic on unique 64-bit keys) in one workgroup, only
+// when some node changed its support block since the previous call.
+__global__ void __launch_bounds__(1024) k_bitonic_lds(const uint64_t* prev, uint64_t* keys,
+                                                      int padded, int* flag) {
   extern __shared__ uint64_t sk[];
-  for (int i = threadIdx.x; i < padded; i += blockDim.x) sk[i] = keys[i];
+  if (atomicAdd(flag + 1, 0) == 0) return;  // uniform for the workgroup
+  for (int i = threadIdx.x; i < padded; i += blockDim.x) sk[i] = prev[i];
   __syncthreads();
   for (int k = 2; k <= padded; k <<= 1) {
     for (int j = k >> 1; j > 0; j >>= 1) {
@@ -260,6 +293,14 @@ __global__ void __launch_bounds__(1024) k_bitonic_lds(uint64_t* keys, int padded
     }
   }
   for (int i = threadIdx.x; i < padded; i += blockDim.x) keys[i] = sk[i];
+  __syncthreads();
+  if (threadIdx.x == 0) atomicExch(flag + 1, 0);
+}
+
+__global__ void k_copy_keys(const uint64_t* src, uint64_t* dst, int64_t m, int* flag) {
+  const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (i < m) dst[i] = src[i];
+  if (i == 0) atomicExch(flag + 1, 0);
 }
 
 // One compare-exchange step of a global-memory bitonic sort (for large node counts).
@@ -295,12 +336,20 @@ struct EulW3 {
   int64_t sz, sy, sx;
 };
 
-template <typename T>
-__global__ void k_spread_gather(int ncomp, EulW3<T> F, int64_t nz, int64_t ny, int64_t nx,
-                                const T* lag, const T* weights, const uint64_t* keys, int64_t n) {
+template <typename T, bool LDS_KEYS>
+__global__ void __launch_bounds__(512) k_spread_gather(int ncomp, EulW3<T> F, int64_t nz, int64_t ny,
+                                                       int64_t nx, const T* lag, const T* weights,
+                                                       const uint64_t* gkeys, int64_t n) {
   // every multiply and add below is rounded separately, exactly as the reference's
   // `F[block] += F_n * w_n` (no fused multiply-add contraction)
 #pragma clang fp contract(off)
+  extern __shared__ uint64_t skeys[];
+  const uint64_t* keys = gkeys;
+  if constexpr (LDS_KEYS) {
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) skeys[i] = gkeys[i];
+    __syncthreads();
+    keys = skeys;
+  }
   const int64_t gid = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   const int64_t p = gid >> 6;
   const int slot = int(gid & 63);
@@ -442,9 +491,12 @@ int sopht_spread3d_create(int64_t max_nodes, sopht_spread3d** out) {
   int64_t p = 1;
   while (p < max_nodes) p <<= 1;
   w->padded = p;
+  const int init_flags[2] = {0, 1};  // no out-of-domain node yet; first call sorts
   if (hipMalloc(&w->keys, sizeof(uint64_t) * p) != hipSuccess ||
-      hipMalloc(&w->flag, sizeof(int)) != hipSuccess ||
-      hipMemset(w->flag, 0, sizeof(int)) != hipSuccess) {
+      hipMalloc(&w->prev, sizeof(uint64_t) * p) != hipSuccess ||
+      hipMalloc(&w->flag, 2 * sizeof(int)) != hipSuccess ||
+      hipMemset(w->prev, 0xff, sizeof(uint64_t) * p) != hipSuccess ||
+      hipMemcpy(w->flag, init_flags, sizeof(init_flags), hipMemcpyHostToDevice) != hipSuccess) {
     sopht_spread3d_destroy(w);
     set_last_error("hipMalloc of the spread workspace failed", __FILE__, __LINE__);
     return SOPHT_ERR_HIP;
@@ -456,6 +508,7 @@ int sopht_spread3d_create(int64_t max_nodes, sopht_spread3d** out) {
 int sopht_spread3d_destroy(sopht_spread3d* w) {
   if (!w) return SOPHT_OK;
   if (w->keys) (void)hipFree(w->keys);
+  if (w->prev) (void)hipFree(w->prev);
   if (w->flag) (void)hipFree(w->flag);
   delete w;
   return SOPHT_OK;
@@ -473,25 +526,37 @@ int sopht_lag_to_eul_spread_3d(int dtype, int ncomp, sopht_spread3d* ws, void* c
   hipStream_t s = static_cast<hipStream_t>(stream);
   int64_t padded = 1;
   while (padded < n) padded <<= 1;
-  k_spread_keys<<<blocks_for(padded, 256), 256, 0, s>>>(idx, n, padded, nz, ny, nx, ws->keys, ws->flag);
+  // keys of this call into ws->prev (and the "changed" flag) over the workspace's padded
+  // length, so a later call with fewer nodes still sees its stale tail as changed
+  k_spread_keys<<<blocks_for(ws->padded, 256), 256, 0, s>>>(idx, n, ws->padded, nz, ny, nx,
+                                                            ws->prev, ws->flag);
   SOPHT_CHECK_LAUNCH();
   if (padded <= 8192) {
-    k_bitonic_lds<<<1, 1024, padded * sizeof(uint64_t), s>>>(ws->keys, int(padded));
+    k_bitonic_lds<<<1, 1024, padded * sizeof(uint64_t), s>>>(ws->prev, ws->keys, int(padded), ws->flag);
     SOPHT_CHECK_LAUNCH();
   } else {
+    k_copy_keys<<<blocks_for(padded, 256), 256, 0, s>>>(ws->prev, ws->keys, padded, ws->flag);
+    SOPHT_CHECK_LAUNCH();
     for (int64_t k = 2; k <= padded; k <<= 1)
       for (int64_t j = k >> 1; j > 0; j >>= 1) {
         k_bitonic_step<<<blocks_for(padded, 256), 256, 0, s>>>(ws->keys, padded, j, k);
         SOPHT_CHECK_LAUNCH();
       }
   }
+  // sorted keys in LDS for the gather's binary searches when they fit (128 KiB)
+  const bool lds_keys = n <= 16384;
   IB_DISPATCH(dtype, ([&] {
                 EulW3<T> F;
                 for (int c = 0; c < 3; ++c) F.p[c] = c < ncomp ? static_cast<T*>(eul[c]) : nullptr;
                 F.sz = es[0]; F.sy = es[1]; F.sx = es[2];
-                k_spread_gather<T><<<blocks_for(n * 64, 256), 256, 0, s>>>(
-                    ncomp, F, nz, ny, nx, static_cast<const T*>(lag), static_cast<const T*>(weights),
-                    ws->keys, n);
+                if (lds_keys)
+                  k_spread_gather<T, true><<<blocks_for(n * 64, 512), 512, n * sizeof(uint64_t), s>>>(
+                      ncomp, F, nz, ny, nx, static_cast<const T*>(lag), static_cast<const T*>(weights),
+                      ws->keys, n);
+                else
+                  k_spread_gather<T, false><<<blocks_for(n * 64, 512), 512, 0, s>>>(
+                      ncomp, F, nz, ny, nx, static_cast<const T*>(lag), static_cast<const T*>(weights),
+                      ws->keys, n);
               }()));
   SOPHT_CHECK_LAUNCH();
   return SOPHT_OK;
@@ -523,10 +588,10 @@ int sopht_vbf_lag_force_3d(int dtype, int kernel_type, const void* const vel[3],
                 Eul3<T> u;
                 for (int a = 0; a < 3; ++a) u.p[a] = static_cast<const T*>(vel[a]);
                 u.sz = vs[0]; u.sy = vs[1]; u.sx = vs[2];
-                k_vbf_lag_force<T><<<blocks_for(n, 64), 64, 0, s>>>(
+                k_vbf_lag_force<T><<<blocks_for(n, 4), 256, 0, s>>>(
                     kernel_type, u, nz, ny, nx, X, V, n, dx, shift, k, c, idx,
                     static_cast<T*>(support), static_cast<T*>(weights), static_cast<T*>(lag_u),
-                    static_cast<T*>(dV), static_cast<const T*>(dX), static_cast<T*>(F), nullptr);
+                    static_cast<T*>(dV), static_cast<const T*>(dX), static_cast<T*>(F));
               }()));
   SOPHT_CHECK_LAUNCH();
   return SOPHT_OK;

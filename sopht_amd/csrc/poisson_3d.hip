@@ -21,7 +21,7 @@
 #include <new>
 
 #include "common.h"
-#include "fft.h"
+#include "fft_reg.h"
 
 struct sopht_poisson3d {
   int dtype;
@@ -36,41 +36,51 @@ struct sopht_poisson3d {
 namespace sopht {
 namespace {
 
-using fft::fft_lds;
-using fft::team_size;
+using rfft::ColSlot;
+using rfft::LinePlan;
+using rfft::RowSlot;
+using rfft::team_fft;
+using rfft::static_for;
 
-// Dynamic LDS per workgroup (lines + twiddle table) stays within 64 KiB.
+// Dynamic LDS per workgroup stays within 64 KiB, so two 1024-thread workgroups fit a CU.
 constexpr int kLdsBudget = 64 * 1024;
 
+// Threads per workgroup: 1024 (a 128-VGPR budget) for 8-element FP32 lines; 512 (256
+// VGPRs) for the 12/24-element plans and FP64, which need more registers.
+template <typename T, int L>
+constexpr int max_threads() {
+  return (LinePlan<L>::E > 8 || sizeof(T) == 8) ? 512 : 1024;
+}
+
+// Column tiles (y and z passes): WC neighbouring kx columns per workgroup, T threads per
+// column.
+template <typename T, int L>
+constexpr int cols_per_block() {
+  constexpr int sc = int(sizeof(Cplx<T>));
+  int wc = max_threads<T, L>() / LinePlan<L>::T;
+  const int cap = kLdsBudget / (L * sc);
+  if (wc > cap) wc = cap;
+  return wc < 1 ? 1 : wc;
+}
+
+// Row tiles (x passes): LPB row pairs per workgroup.
 template <typename T, int L>
 constexpr int rows_per_block() {
-  // lines per workgroup for the x-direction kernels
   constexpr int sc = int(sizeof(Cplx<T>));
-  constexpr int tl = team_size<L>();
-  int lpb = 256 / tl;
-  if (lpb < 1) lpb = 1;
-  const int cap = (kLdsBudget - L * sc) / (L * sc);
+  int lpb = max_threads<T, L>() / LinePlan<L>::T;
+  const int cap = kLdsBudget / (RowSlot<L>::kStride * sc);
   if (lpb > cap) lpb = cap;
   return lpb < 1 ? 1 : lpb;
 }
 
 template <typename T, int L>
-constexpr int cols_per_block() {
-  // columns per workgroup for the y/z kernels: at least a 128-byte row segment and four
-  // wavefronts when the LDS budget allows
-  constexpr int sc = int(sizeof(Cplx<T>));
-  constexpr int tl = team_size<L>();
-  int wc = 128 / sc;
-  while (wc * tl < 256) wc *= 2;
-  const int cap = (kLdsBudget - L * sc) / (L * sc);
-  if (wc > cap) wc = cap;
-  while (wc * tl > 1024) --wc;
-  return wc < 1 ? 1 : wc;
+constexpr size_t row_lds_bytes() {
+  return size_t(rows_per_block<T, L>()) * RowSlot<L>::kStride * sizeof(Cplx<T>);
 }
 
 template <typename T, int L>
-__device__ __forceinline__ void load_twiddles(Cplx<T>* tw_lds, const Cplx<T>* tw) {
-  for (int i = threadIdx.x; i < L; i += blockDim.x) tw_lds[i] = tw[i];
+constexpr size_t col_lds_bytes() {
+  return size_t(L) * cols_per_block<T, L>() * sizeof(Cplx<T>);
 }
 
 struct SlabGeom {
@@ -79,6 +89,11 @@ struct SlabGeom {
   // H index of (c, z, ky, kx)
   __device__ __forceinline__ int64_t h(int c, int64_t z, int64_t ky, int64_t kx) const {
     return ((int64_t(c) * nz + z) * (2 * ny) + ky) * (nx + 1) + kx;
+  }
+  __device__ __forceinline__ int64_t row_h(int64_t r) const {  // H row of input row r
+    const int c = int(r / (nz * ny));
+    const int64_t rem = r % (nz * ny);
+    return h(c, rem / ny, rem % ny, 0);
   }
 };
 
@@ -91,70 +106,62 @@ struct CompPtrsW {
   int64_t sz, sy, sx;
 };
 
-// K1: x real-to-complex of zero-padded rows, two rows per complex FFT of length L = 2nx.
+// K1: x real-to-complex of zero-padded rows, two rows (ra, rb) per complex FFT of length
+// L = 2nx; the pair is separated with A = (Z[k] + conj Z[L-k])/2, B = (Z[k] - conj Z[L-k])/(2i).
 template <typename T, int L>
-__global__ void __launch_bounds__(1024) k1_xr2c(SlabGeom g, CompPtrs in, Cplx<T>* H,
+__global__ void __launch_bounds__((max_threads<T, L>())) k1_xr2c(SlabGeom g, CompPtrs in, Cplx<T>* H,
                                                 const Cplx<T>* tw) {
-  constexpr int TL = team_size<L>();
+  constexpr int E = LinePlan<L>::E, TT = LinePlan<L>::T;
   constexpr int LPB = rows_per_block<T, L>();
   extern __shared__ unsigned char smem_raw[];
-  Cplx<T>* tw_s = reinterpret_cast<Cplx<T>*>(smem_raw);
-  Cplx<T>* buf = tw_s + L;
-  load_twiddles<T, L>(tw_s, tw);
-  const int t = threadIdx.x % TL;
-  const int line = threadIdx.x / TL;
+  Cplx<T>* xb = reinterpret_cast<Cplx<T>*>(smem_raw);
+  const int t = threadIdx.x % TT;
+  const int line = threadIdx.x / TT;
   const int64_t rows = int64_t(g.ncomp) * g.nz * g.ny;
-  const int64_t q = int64_t(blockIdx.x) * LPB + line;  // pair index
+  const int64_t q = int64_t(blockIdx.x) * LPB + line;
   const int64_t ra = 2 * q, rb = 2 * q + 1;
-  Cplx<T>* ln = buf + line * L;
-  const int64_t nx = g.nx;
   auto row_ptr = [&](int64_t r) -> const T* {
     const int c = int(r / (g.nz * g.ny));
     const int64_t rem = r % (g.nz * g.ny);
-    const int64_t z = rem / g.ny, y = rem % g.ny;
-    return static_cast<const T*>(in.p[c]) + z * in.sz + y * in.sy;
+    return static_cast<const T*>(in.p[c]) + (rem / g.ny) * in.sz + (rem % g.ny) * in.sy;
   };
   const T* pa = ra < rows ? row_ptr(ra) : nullptr;
   const T* pb = rb < rows ? row_ptr(rb) : nullptr;
-  for (int x = t; x < L; x += TL) {
-    T a = T(0), b = T(0);
-    if (x < nx) {
-      if (pa) a = pa[x * in.sx];
-      if (pb) b = pb[x * in.sx];
+  Cplx<T> v[E];
+  static_for<0, E>([&](auto e_) {
+    constexpr int e = decltype(e_)::value;
+    if constexpr (e < E / 2) {  // x = t + T e < nx
+      const int64_t x = t + TT * e;
+      v[e] = Cplx<T>{pa ? pa[x * in.sx] : T(0), pb ? pb[x * in.sx] : T(0)};
+    } else {
+      v[e] = Cplx<T>{T(0), T(0)};
     }
-    ln[x] = Cplx<T>{a, b};
-  }
+  });
+  RowSlot<L> slot{line};
+  team_fft<T, L, false>(v, xb, tw, t, slot);
+  static_for<0, E>([&](auto e_) {
+    constexpr int e = decltype(e_)::value;
+    xb[slot(t + TT * e)] = v[e];
+  });
   __syncthreads();
-  fft_lds<T, L, false>(ln, 1, t, tw_s);
-  // separate the two real rows: A = (Z[k] + conj Z[L-k])/2, B = (Z[k] - conj Z[L-k])/(2i)
-  for (int k = t; k <= nx; k += TL) {
-    const Cplx<T> zk = ln[k];
-    const Cplx<T> zm = ln[(L - k) % L];
-    const Cplx<T> A = {T(0.5) * (zk.re + zm.re), T(0.5) * (zk.im - zm.im)};
-    const Cplx<T> B = {T(0.5) * (zk.im + zm.im), T(0.5) * (zm.re - zk.re)};
-    if (ra < rows) {
-      const int c = int(ra / (g.nz * g.ny));
-      const int64_t rem = ra % (g.nz * g.ny);
-      H[g.h(c, rem / g.ny, rem % g.ny, k)] = A;
-    }
-    if (rb < rows) {
-      const int c = int(rb / (g.nz * g.ny));
-      const int64_t rem = rb % (g.nz * g.ny);
-      H[g.h(c, rem / g.ny, rem % g.ny, k)] = B;
-    }
+  Cplx<T>* ha = ra < rows ? H + g.row_h(ra) : nullptr;
+  Cplx<T>* hb = rb < rows ? H + g.row_h(rb) : nullptr;
+  for (int k = t; k <= g.nx; k += TT) {
+    const Cplx<T> zk = xb[slot(k)];
+    const Cplx<T> zm = xb[slot((L - k) % L)];
+    if (ha) ha[k] = Cplx<T>{T(0.5) * (zk.re + zm.re), T(0.5) * (zk.im - zm.im)};
+    if (hb) hb[k] = Cplx<T>{T(0.5) * (zk.im + zm.im), T(0.5) * (zm.re - zk.re)};
   }
 }
 
-// K2 / K4: FFT along y of columns (c, z, kx-tile). Forward: inputs y < ny, zero padding,
-// all 2ny outputs stored. Inverse: all 2ny inputs, outputs y < ny stored.
+// K2 / K4: FFT along y of columns (c, z, kx). Forward: inputs y < ny (zero padding), all
+// 2ny outputs stored. Inverse: all 2ny inputs, outputs y < ny stored.
 template <typename T, int L, bool INV>
-__global__ void __launch_bounds__(1024) k_ycols(SlabGeom g, Cplx<T>* H, const Cplx<T>* tw) {
-  constexpr int TL = team_size<L>();
+__global__ void __launch_bounds__((max_threads<T, L>())) k_ycols(SlabGeom g, Cplx<T>* H, const Cplx<T>* tw) {
+  constexpr int E = LinePlan<L>::E, TT = LinePlan<L>::T;
   constexpr int WC = cols_per_block<T, L>();
   extern __shared__ unsigned char smem_raw[];
-  Cplx<T>* tw_s = reinterpret_cast<Cplx<T>*>(smem_raw);
-  Cplx<T>* buf = tw_s + L;
-  load_twiddles<T, L>(tw_s, tw);
+  Cplx<T>* xb = reinterpret_cast<Cplx<T>*>(smem_raw);
   const int w = threadIdx.x % WC;
   const int t = threadIdx.x / WC;
   const int64_t ncols = g.nx + 1;
@@ -165,29 +172,31 @@ __global__ void __launch_bounds__(1024) k_ycols(SlabGeom g, Cplx<T>* H, const Cp
   const int64_t z = cz % g.nz;
   const int64_t kx = tile * WC + w;
   const bool live = kx < ncols;
-  const int64_t nin = INV ? L : g.ny;
-  const int64_t nout = INV ? g.ny : L;
-  for (int e = t; e < L; e += TL) {
-    Cplx<T> v = {T(0), T(0)};
-    if (live && e < nin) v = H[g.h(c, z, e, kx)];
-    buf[e * WC + w] = v;
-  }
-  __syncthreads();
-  fft_lds<T, L, INV>(buf + w, WC, t, tw_s);
+  Cplx<T>* col = H + g.h(c, z, 0, live ? kx : 0);
+  const int64_t ys = ncols;  // stride between consecutive y (or ky)
+  Cplx<T> v[E];
+  static_for<0, E>([&](auto e_) {
+    constexpr int e = decltype(e_)::value;
+    if constexpr (INV || e < E / 2) v[e] = col[(t + TT * e) * ys];
+    else v[e] = Cplx<T>{T(0), T(0)};
+  });
+  team_fft<T, L, INV>(v, xb, tw, t, ColSlot<WC>{w});
   if (live)
-    for (int e = t; e < nout; e += TL) H[g.h(c, z, e, kx)] = buf[e * WC + w];
+    static_for<0, (INV ? E / 2 : E)>([&](auto e_) {
+      constexpr int e = decltype(e_)::value;
+      col[(t + TT * e) * ys] = v[e];
+    });
 }
 
-// K3: z pencils (c, ky, kx-tile): zero-padded forward FFT, times G^, inverse FFT, crop.
+// K3: z pencils (c, ky, kx): zero-padded forward FFT, times G^ (real, even: read from the
+// octant), inverse FFT, crop to z < nz.
 template <typename T, int L>
-__global__ void __launch_bounds__(1024) k3_zconv(SlabGeom g, Cplx<T>* H, const T* ghat,
+__global__ void __launch_bounds__((max_threads<T, L>())) k3_zconv(SlabGeom g, Cplx<T>* H, const T* ghat,
                                                  const Cplx<T>* tw) {
-  constexpr int TL = team_size<L>();
+  constexpr int E = LinePlan<L>::E, TT = LinePlan<L>::T;
   constexpr int WC = cols_per_block<T, L>();
   extern __shared__ unsigned char smem_raw[];
-  Cplx<T>* tw_s = reinterpret_cast<Cplx<T>*>(smem_raw);
-  Cplx<T>* buf = tw_s + L;
-  load_twiddles<T, L>(tw_s, tw);
+  Cplx<T>* xb = reinterpret_cast<Cplx<T>*>(smem_raw);
   const int w = threadIdx.x % WC;
   const int t = threadIdx.x / WC;
   const int64_t ncols = g.nx + 1;
@@ -198,82 +207,93 @@ __global__ void __launch_bounds__(1024) k3_zconv(SlabGeom g, Cplx<T>* H, const T
   const int64_t ky = cky % (2 * g.ny);
   const int64_t kx = tile * WC + w;
   const bool live = kx < ncols;
-  const int64_t nz = g.nz;
-  // H stride between consecutive z for fixed (c, ky, kx)
-  for (int e = t; e < L; e += TL) {
-    Cplx<T> v = {T(0), T(0)};
-    if (live && e < nz) v = H[g.h(c, e, ky, kx)];
-    buf[e * WC + w] = v;
+  const int64_t kxs = live ? kx : 0;
+  Cplx<T>* col = H + g.h(c, 0, ky, kxs);
+  const int64_t zs = 2 * g.ny * ncols;  // stride between consecutive z
+  Cplx<T> v[E];
+  static_for<0, E>([&](auto e_) {
+    constexpr int e = decltype(e_)::value;
+    if constexpr (e < E / 2) v[e] = col[(t + TT * e) * zs];
+    else v[e] = Cplx<T>{T(0), T(0)};
+  });
+  // G^ at this thread's frequencies kz = t + T e, loaded ahead of the transform
+  T gv[E];
+  {
+    const int64_t kyo = ky <= g.ny ? ky : 2 * g.ny - ky;
+    const T* gcol = ghat + kyo * ncols + kxs;
+    const int64_t gz = (g.ny + 1) * ncols;
+    static_for<0, E>([&](auto e_) {
+      constexpr int e = decltype(e_)::value;
+      const int kz = t + TT * e;
+      gv[e] = gcol[(kz <= L / 2 ? kz : L - kz) * gz];
+    });
   }
-  __syncthreads();
-  fft_lds<T, L, false>(buf + w, WC, t, tw_s);
-  const int64_t kyo = ky <= g.ny ? ky : 2 * g.ny - ky;
-  if (live) {
-    const T* gcol = ghat + kyo * (g.nx + 1) + kx;
-    const int64_t gz = (g.ny + 1) * (g.nx + 1);
-    for (int e = t; e < L; e += TL) {
-      const int64_t kzo = e <= nz ? e : L - e;
-      const T s = gcol[kzo * gz];
-      Cplx<T> v = buf[e * WC + w];
-      buf[e * WC + w] = Cplx<T>{v.re * s, v.im * s};
-    }
-  }
-  __syncthreads();
-  fft_lds<T, L, true>(buf + w, WC, t, tw_s);
+  ColSlot<WC> slot{w};
+  team_fft<T, L, false>(v, xb, tw, t, slot);
+  static_for<0, E>([&](auto e_) {
+    constexpr int e = decltype(e_)::value;
+    v[e] = Cplx<T>{v[e].re * gv[e], v[e].im * gv[e]};
+  });
+  team_fft<T, L, true>(v, xb, tw, t, slot);
   if (live)
-    for (int e = t; e < nz; e += TL) H[g.h(c, e, ky, kx)] = buf[e * WC + w];
+    static_for<0, E / 2>([&](auto e_) {
+      constexpr int e = decltype(e_)::value;
+      col[(t + TT * e) * zs] = v[e];
+    });
 }
 
-// K5: x complex-to-real of two rows per complex inverse FFT, crop to x < nx.
+// K5: x complex-to-real of two rows per complex inverse FFT, crop to x < nx. The Hermitian
+// half spectra A, B (kx <= nx) of rows ra, rb become Z[k] = A[k] + i B[k],
+// Z[L-k] = conj(A[k]) + i conj(B[k]); imaginary parts at kx = 0 and nx are dropped
+// (irfft semantics).
 template <typename T, int L>
-__global__ void __launch_bounds__(1024) k5_xc2r(SlabGeom g, const Cplx<T>* H, CompPtrsW out,
+__global__ void __launch_bounds__((max_threads<T, L>())) k5_xc2r(SlabGeom g, const Cplx<T>* H, CompPtrsW out,
                                                 const Cplx<T>* tw) {
-  constexpr int TL = team_size<L>();
+  constexpr int E = LinePlan<L>::E, TT = LinePlan<L>::T;
   constexpr int LPB = rows_per_block<T, L>();
   extern __shared__ unsigned char smem_raw[];
-  Cplx<T>* tw_s = reinterpret_cast<Cplx<T>*>(smem_raw);
-  Cplx<T>* buf = tw_s + L;
-  load_twiddles<T, L>(tw_s, tw);
-  const int t = threadIdx.x % TL;
-  const int line = threadIdx.x / TL;
+  Cplx<T>* xb = reinterpret_cast<Cplx<T>*>(smem_raw);
+  const int t = threadIdx.x % TT;
+  const int line = threadIdx.x / TT;
   const int64_t rows = int64_t(g.ncomp) * g.nz * g.ny;
   const int64_t q = int64_t(blockIdx.x) * LPB + line;
   const int64_t ra = 2 * q, rb = 2 * q + 1;
-  Cplx<T>* ln = buf + line * L;
   const int64_t nx = g.nx;
-  auto hrow = [&](int64_t r) -> int64_t {
-    const int c = int(r / (g.nz * g.ny));
-    const int64_t rem = r % (g.nz * g.ny);
-    return g.h(c, rem / g.ny, rem % g.ny, 0);
-  };
-  const int64_t ha = ra < rows ? hrow(ra) : -1;
-  const int64_t hb = rb < rows ? hrow(rb) : -1;
-  for (int k = t; k <= nx; k += TL) {
-    Cplx<T> A = {T(0), T(0)}, B = {T(0), T(0)};
-    if (ha >= 0) A = H[ha + k];
-    if (hb >= 0) B = H[hb + k];
-    if (k == 0 || k == nx) {  // DC and Nyquist of a real row are real (irfft semantics)
-      A.im = T(0);
-      B.im = T(0);
+  const Cplx<T>* ha = ra < rows ? H + g.row_h(ra) : nullptr;
+  const Cplx<T>* hb = rb < rows ? H + g.row_h(rb) : nullptr;
+  RowSlot<L> slot{line};
+  for (int k = t; k <= nx; k += TT) {
+    Cplx<T> a = {T(0), T(0)}, bb = {T(0), T(0)};
+    if (ha) a = ha[k];
+    if (hb) bb = hb[k];
+    if (k == 0 || k == nx) {
+      a.im = T(0);
+      bb.im = T(0);
     }
-    ln[k] = Cplx<T>{A.re - B.im, A.im + B.re};
-    if (k > 0 && k < nx) ln[L - k] = Cplx<T>{A.re + B.im, B.re - A.im};
+    xb[slot(k)] = Cplx<T>{a.re - bb.im, a.im + bb.re};
+    if (k > 0 && k < nx) xb[slot(L - k)] = Cplx<T>{a.re + bb.im, bb.re - a.im};
   }
   __syncthreads();
-  fft_lds<T, L, true>(ln, 1, t, tw_s);
+  Cplx<T> v[E];
+  static_for<0, E>([&](auto e_) {
+    constexpr int e = decltype(e_)::value;
+    v[e] = xb[slot(t + TT * e)];
+  });
+  __syncthreads();  // exchange area reuse
+  team_fft<T, L, true>(v, xb, tw, t, slot);
   auto row_ptr = [&](int64_t r) -> T* {
     const int c = int(r / (g.nz * g.ny));
     const int64_t rem = r % (g.nz * g.ny);
-    const int64_t z = rem / g.ny, y = rem % g.ny;
-    return static_cast<T*>(out.p[c]) + z * out.sz + y * out.sy;
+    return static_cast<T*>(out.p[c]) + (rem / g.ny) * out.sz + (rem % g.ny) * out.sy;
   };
   T* pa = ra < rows ? row_ptr(ra) : nullptr;
   T* pb = rb < rows ? row_ptr(rb) : nullptr;
-  for (int x = t; x < nx; x += TL) {
-    const Cplx<T> v = ln[x];
-    if (pa) pa[x * out.sx] = v.re;
-    if (pb) pb[x * out.sx] = v.im;
-  }
+  static_for<0, E / 2>([&](auto e_) {
+    constexpr int e = decltype(e_)::value;
+    const int64_t x = t + TT * e;
+    if (pa) pa[x * out.sx] = v[e].re;
+    if (pb) pb[x * out.sx] = v[e].im;
+  });
 }
 
 // ------------------------------------------------------------------ Green's function octant
@@ -333,9 +353,8 @@ int launch_k1(int64_t L, SlabGeom g, CompPtrs in, Cplx<T>* H, const Cplx<T>* tw,
 #define X(n)                                                                          \
   if (L == n) {                                                                       \
     constexpr int lpb = rows_per_block<T, n>();                                       \
-    const int threads = lpb * team_size<n>();                                         \
-    const size_t shm = size_t(n) * sizeof(Cplx<T>) * (lpb + 1);                        \
-    k1_xr2c<T, n><<<ceil_div(pairs, lpb), threads, shm, s>>>(g, in, H, tw);           \
+    const int threads = lpb * LinePlan<n>::T;                                         \
+    k1_xr2c<T, n><<<ceil_div(pairs, lpb), threads, row_lds_bytes<T, n>(), s>>>(g, in, H, tw); \
     SOPHT_CHECK_LAUNCH();                                                             \
     return SOPHT_OK;                                                                  \
   }
@@ -349,10 +368,9 @@ int launch_ycols(int64_t L, SlabGeom g, Cplx<T>* H, const Cplx<T>* tw, hipStream
 #define X(n)                                                                          \
   if (L == n) {                                                                       \
     constexpr int wc = cols_per_block<T, n>();                                        \
-    const int threads = wc * team_size<n>();                                          \
-    const size_t shm = size_t(n) * sizeof(Cplx<T>) * (wc + 1);                         \
+    const int threads = wc * LinePlan<n>::T;                                          \
     const int64_t tiles = (g.nx + 1 + wc - 1) / wc;                                   \
-    k_ycols<T, n, INV><<<tiles * g.ncomp * g.nz, threads, shm, s>>>(g, H, tw);        \
+    k_ycols<T, n, INV><<<tiles * g.ncomp * g.nz, threads, col_lds_bytes<T, n>(), s>>>(g, H, tw); \
     SOPHT_CHECK_LAUNCH();                                                             \
     return SOPHT_OK;                                                                  \
   }
@@ -367,10 +385,9 @@ int launch_k3(int64_t L, SlabGeom g, Cplx<T>* H, const T* ghat, const Cplx<T>* t
 #define X(n)                                                                          \
   if (L == n) {                                                                       \
     constexpr int wc = cols_per_block<T, n>();                                        \
-    const int threads = wc * team_size<n>();                                          \
-    const size_t shm = size_t(n) * sizeof(Cplx<T>) * (wc + 1);                         \
+    const int threads = wc * LinePlan<n>::T;                                          \
     const int64_t tiles = (g.nx + 1 + wc - 1) / wc;                                   \
-    k3_zconv<T, n><<<tiles * g.ncomp * 2 * g.ny, threads, shm, s>>>(g, H, ghat, tw);  \
+    k3_zconv<T, n><<<tiles * g.ncomp * 2 * g.ny, threads, col_lds_bytes<T, n>(), s>>>(g, H, ghat, tw); \
     SOPHT_CHECK_LAUNCH();                                                             \
     return SOPHT_OK;                                                                  \
   }
@@ -387,9 +404,8 @@ int launch_k5(int64_t L, SlabGeom g, const Cplx<T>* H, CompPtrsW out, const Cplx
 #define X(n)                                                                          \
   if (L == n) {                                                                       \
     constexpr int lpb = rows_per_block<T, n>();                                       \
-    const int threads = lpb * team_size<n>();                                         \
-    const size_t shm = size_t(n) * sizeof(Cplx<T>) * (lpb + 1);                        \
-    k5_xc2r<T, n><<<ceil_div(pairs, lpb), threads, shm, s>>>(g, H, out, tw);          \
+    const int threads = lpb * LinePlan<n>::T;                                         \
+    k5_xc2r<T, n><<<ceil_div(pairs, lpb), threads, row_lds_bytes<T, n>(), s>>>(g, H, out, tw); \
     SOPHT_CHECK_LAUNCH();                                                             \
     return SOPHT_OK;                                                                  \
   }

@@ -32,6 +32,42 @@ def as_device_f64(a, device=None):
     return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64)).to(device, non_blocking=False)
 
 
+class _LagrangianUpload:
+    """Device copy of a host ``(3, n)`` float64 array, refreshed only when its values change.
+
+    A rigid body at rest presents the same positions every step; re-sending them would
+    cost a synchronous host-to-device copy per call. New values go through a pinned
+    staging buffer with an asynchronous copy on the current stream; the staging buffer is
+    rewritten only after the previous copy from it has completed."""
+
+    def __init__(self):
+        self.host = None
+        self.dev = None
+        self.pinned = None
+        self.event = None
+
+    def get(self, a):
+        import torch
+
+        if isinstance(a, torch.Tensor):
+            return as_device_f64(a)
+        a = np.asarray(a, dtype=np.float64)
+        if self.host is not None and self.host.shape == a.shape and np.array_equal(self.host, a):
+            return self.dev
+        if self.dev is None or tuple(self.dev.shape) != a.shape:
+            self.dev = torch.empty(a.shape, dtype=torch.float64, device=_device())
+            self.pinned = torch.empty(a.shape, dtype=torch.float64, pin_memory=True)
+            self.event = None
+        if self.event is not None:
+            self.event.synchronize()
+        self.pinned.numpy()[...] = a
+        self.dev.copy_(self.pinned, non_blocking=True)
+        self.event = torch.cuda.Event()
+        self.event.record()
+        self.host = a.copy()
+        return self.dev
+
+
 class VirtualBoundaryForcing:
     """Goldstein (1993) virtual boundary feedback between body and flow."""
 
@@ -92,6 +128,8 @@ class VirtualBoundaryForcing:
         )
         self._ktype = self.eul_lag_grid_communicator._ktype
         self._tag = L.dtype_tag(real_t)
+        self._pos_upload = _LagrangianUpload()
+        self._vel_upload = _LagrangianUpload()
         self.enable_eul_grid_forcing_reset = enable_eul_grid_forcing_reset
         if enable_eul_grid_forcing_reset:
             self.compute_interaction_forcing = (
@@ -140,8 +178,8 @@ class VirtualBoundaryForcing:
         self, eul_grid_velocity_field, lag_grid_position_field, lag_grid_velocity_field
     ):
         L.require_device_tensor(eul_grid_velocity_field, self.real_t, "eul_grid_velocity_field", 4)
-        X = as_device_f64(lag_grid_position_field)
-        V = as_device_f64(lag_grid_velocity_field)
+        X = self._pos_upload.get(lag_grid_position_field)
+        V = self._vel_upload.get(lag_grid_velocity_field)
         u = eul_grid_velocity_field
         nz, ny, nx = (int(s) for s in u.shape[1:])
         L.call(
