@@ -46,8 +46,7 @@ def stage_words(nz, ny, nx):
     c = (nx + 1) / nx  # complex columns kx = 0..nx per real row of nx
     ghat = (nz + 1) * (ny + 1) * (nx + 1) / (nz * ny * nx)
     return {
-        "vorticity_forcing_cross": 9 + 6,      # read w, u, F; write w, B
-        "vorticity_advect": 6 + 3,             # read w, B; write w2 (into psi)
+        "vorticity_forcing_advect": 9 + 3,     # read w, u, F; write w2 (into psi)
         "vorticity_diffuse_penalise": 3 + 6,   # read w2; write w, F = 0
         "poisson_x_r2c": 3 * (1 + 2 * c),      # real rows in, half spectra out
         "poisson_y_fwd": 3 * (2 * c + 4 * c),  # y < ny in, all 2ny out
@@ -122,6 +121,41 @@ def cpu_baseline(n, real_t, steps, n_lag=3000):
     }
 
 
+def copy_probe(n, reps):
+    """A plain device copy of known size (one (n, n, n) fp32 field per launch, read once
+    and written once) through the library's elementwise copy kernel: the calibration run
+    for the HBM byte counters (FETCH_SIZE, WRITE_SIZE) and the achievable copy bandwidth."""
+    import torch
+
+    from sopht_amd import _lib as L
+
+    torch.cuda.set_device(0)
+    src = torch.rand((n, n, n), device="cuda", dtype=torch.float32)
+    dst = torch.empty_like(src)
+    st = L.strides3(src)
+    args = (L.dtype_tag(np.float32), L.ptr(dst), st, L.ptr(src), st, n, n, n, L.stream_ptr())
+    for _ in range(3):
+        L.call("sopht_elementwise_copy_3d", *args)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    torch.cuda.synchronize()
+    ev[0].record()
+    for _ in range(reps):
+        L.call("sopht_elementwise_copy_3d", *args)
+    ev[1].record()
+    torch.cuda.synchronize()
+    if not torch.equal(src, dst):
+        print("copy probe mismatch", file=sys.stderr)
+        return 1
+    ms = ev[0].elapsed_time(ev[1]) / reps
+    nbytes = src.numel() * 4
+    print(json.dumps({
+        "probe": "sopht_elementwise_copy_3d", "grid": [n, n, n], "bytes_read": nbytes,
+        "bytes_written": nbytes, "launches": reps + 3, "ms_per_launch": round(ms, 4),
+        "copy_gbs": round(2 * nbytes / (ms * 1e-3) / 1e9, 1),
+    }))
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -130,6 +164,8 @@ def main():
     ap.add_argument("--grid", type=int, default=256)
     ap.add_argument("--precision", choices=["single", "double"], default="single")
     ap.add_argument("--cpu-steps", type=int, default=2, help="steps of the CPU baseline sample (0 = skip)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="REPS",
+                    help="only time REPS launches of a plain (grid^3 fp32) copy kernel and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -138,6 +174,8 @@ def main():
             file=sys.stderr,
         )
         return 2
+    if args.copy_probe > 0:
+        return copy_probe(args.grid, args.copy_probe)
 
     import torch
 

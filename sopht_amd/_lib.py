@@ -1,7 +1,8 @@
 """ctypes binding of ``libsopht_amd.so`` (the C ABI in ``include/sopht_amd.h``).
 
-The library is loaded from this package directory only; there is no fallback. A missing
-library raises ``ImportError`` at first use, naming the build command. Device buffers are
+The library is loaded from this package directory (or from the path in the environment
+variable ``SOPHT_AMD_LIB``); there is no fallback. A missing library raises ``ImportError``
+at first use, naming the build command. Device buffers are
 passed as raw pointers (``tensor.data_ptr()``) of torch tensors on the current HIP device;
 every launch goes on torch's current stream.
 """
@@ -14,7 +15,7 @@ from pathlib import Path
 
 import numpy as np
 
-LIB_PATH = Path(__file__).resolve().parent / "libsopht_amd.so"
+LIB_PATH = Path(os.environ.get("SOPHT_AMD_LIB") or Path(__file__).resolve().parent / "libsopht_amd.so")
 
 SOPHT_F32 = 0
 SOPHT_F64 = 1

@@ -166,7 +166,7 @@ __device__ __forceinline__ void rdft(Cplx<T>* v) {
 // x[t + T e] held by thread t, so a forward transform, a pointwise multiply and an
 // inverse transform need no reordering, and the zero-padded half of a line (t + T e >= L/2,
 // i.e. e >= E/2) is known at compile time. Between stages the elements are exchanged
-// through LDS (one write, one barrier, one read, one barrier).
+// through LDS (one write, then one read after a wavefront-scope sync). T divides 64.
 template <int... Rs>
 struct Radices {};
 
@@ -192,28 +192,56 @@ SOPHT_LINE_PLAN(256, 8, 8, 8, 4)
 SOPHT_LINE_PLAN(384, 24, 8, 4, 12)
 SOPHT_LINE_PLAN(512, 8, 8, 8, 8)
 SOPHT_LINE_PLAN(768, 24, 8, 8, 12)
-SOPHT_LINE_PLAN(1024, 8, 8, 8, 4, 4)
+SOPHT_LINE_PLAN(1024, 16, 16, 8, 8)
 #undef SOPHT_LINE_PLAN
 
-// LDS exchange layouts: map element index e (0..L-1) of the caller's line to an LDS index.
-// Column tiles (lanes run over WC neighbouring lines, w fastest):
-template <int WC>
-struct ColSlot {
-  int w;
-  __device__ __forceinline__ int operator()(int e) const { return e * WC + w; }
-};
-// Row tiles (lanes run over the T threads of one line): one pad entry every 32 elements
-// breaks the power-of-two strides of the butterfly writes across LDS banks.
-template <int L>
-struct RowSlot {
-  int line;
-  static constexpr int kStride = L + L / 32 + 1;
-  __device__ __forceinline__ int operator()(int e) const { return line * kStride + e + (e >> 5); }
+constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v / 2); }
+
+// LDS layout of the exchange after a stage of radix R and write stride NS: element p of the
+// team's line lives at p ^ (((p >> A) & (2^MB - 1)) << B). The stage writes
+// (j / NS) NS R + j % NS + r NS (stride R or NS between lanes) and the next stage reads
+// t + T e (unit stride); the XOR moves the lane bits that would collide onto the bank bits the
+// other pattern leaves free, so on gfx950 (64-bit accesses served 32 lanes at a time by 64
+// four-byte banks) both patterns are conflict-free for every plan from 128 points on
+// (checked exhaustively on the host). The map is an involution on [0, L). Shorter lines keep
+// the identity.
+template <int L, int NS, int R>
+struct Swizzle {
+  static constexpr int T = LinePlan<L>::T;
+  static constexpr bool kOn = L >= 128;
+  static constexpr int A = T >= 32 ? 5 : 4;
+  static constexpr int MB = NS == 1 ? ilog2(R) : (ilog2(R) < 3 ? ilog2(R) : 3);
+  static constexpr int B = NS == 1 ? 0 : 2;
+  static_assert(!kOn || (1 << ilog2(R)) == R, "swizzled exchanges need a power-of-two radix");
+  static_assert(!kOn || (B + MB <= A && L % (1 << (A + MB)) == 0), "swizzle must stay inside the line");
+  static __device__ __forceinline__ int map(int p) {
+    if constexpr (kOn) return p ^ (((p >> A) & ((1 << MB) - 1)) << B);
+    else return p;
+  }
 };
 
-template <typename T, int L, int R, int NS, bool INV, typename Slot>
+// LDS elements reserved per line. Teams of 16 threads share a 32-lane half-wave, so their
+// regions start 16 slots apart (mod 32). Column tiles stage WC = 8 columns with lanes
+// (column, row); a stride of 4 (mod 32) between column regions keeps those accesses
+// conflict-free too.
+template <int L, bool COLUMN>
+constexpr int region_stride() {
+  if (LinePlan<L>::T < 32) return L + 16;
+  return COLUMN ? L + 4 : L;
+}
+
+// All lanes of a team sit in one wavefront (T divides 64), so the exchanges need only
+// wavefront-scope ordering: LDS operations of one wavefront execute in order, and the
+// fences keep the compiler from moving accesses across the exchange point.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <typename T, int L, int R, int NS, bool INV>
 __device__ __forceinline__ void stockham_stage(Cplx<T>* v, Cplx<T>* xb, const Cplx<T>* tw,
-                                               int t, Slot slot) {
+                                               int t) {
   constexpr int E = LinePlan<L>::E, TT = LinePlan<L>::T;
   constexpr int PER = E / R;
   static_assert(E % R == 0, "radix must divide the elements per thread");
@@ -242,38 +270,37 @@ __device__ __forceinline__ void stockham_stage(Cplx<T>* v, Cplx<T>* xb, const Cp
       const int base = (j / NS) * NS * R + j % NS;
       static_for<0, R>([&](auto r_) {
         constexpr int r = decltype(r_)::value;
-        xb[slot(base + r * NS)] = v[p + PER * r];
+        xb[Swizzle<L, NS, R>::map(base + r * NS)] = v[p + PER * r];
       });
     });
-    __syncthreads();
+    wave_sync();
     static_for<0, E>([&](auto e_) {
       constexpr int e = decltype(e_)::value;
-      v[e] = xb[slot(t + TT * e)];
+      v[e] = xb[Swizzle<L, NS, R>::map(t + TT * e)];
     });
-    __syncthreads();
+    wave_sync();
   }
 }
 
-template <typename T, int L, bool INV, int NS, typename Slot, int R0, int... Rest>
-__device__ __forceinline__ void stockham_run(Cplx<T>* v, Cplx<T>* xb, const Cplx<T>* tw, int t,
-                                             Slot slot) {
-  stockham_stage<T, L, R0, NS, INV>(v, xb, tw, t, slot);
-  if constexpr (sizeof...(Rest) > 0) stockham_run<T, L, INV, NS * R0, Slot, Rest...>(v, xb, tw, t, slot);
+template <typename T, int L, bool INV, int NS, int R0, int... Rest>
+__device__ __forceinline__ void stockham_run(Cplx<T>* v, Cplx<T>* xb, const Cplx<T>* tw, int t) {
+  stockham_stage<T, L, R0, NS, INV>(v, xb, tw, t);
+  if constexpr (sizeof...(Rest) > 0) stockham_run<T, L, INV, NS * R0, Rest...>(v, xb, tw, t);
 }
 
-template <typename T, int L, bool INV, typename Slot, int... Rs>
+template <typename T, int L, bool INV, int... Rs>
 __device__ __forceinline__ void stockham_plan(Cplx<T>* v, Cplx<T>* xb, const Cplx<T>* tw, int t,
-                                              Slot slot, Radices<Rs...>) {
-  stockham_run<T, L, INV, 1, Slot, Rs...>(v, xb, tw, t, slot);
+                                              Radices<Rs...>) {
+  stockham_run<T, L, INV, 1, Rs...>(v, xb, tw, t);
 }
 
 // In-place transform of a line held as v[e] = x[t + T e] by thread t of its team; on return
-// v[e] = X[t + T e] (forward: exp(-), inverse: exp(+), unnormalised). Every thread of the
-// workgroup calls this together; the exchange area xb is free again on return.
-template <typename T, int L, bool INV, typename Slot>
-__device__ __forceinline__ void team_fft(Cplx<T>* v, Cplx<T>* xb, const Cplx<T>* tw, int t,
-                                         Slot slot) {
-  stockham_plan<T, L, INV>(v, xb, tw, t, slot, typename LinePlan<L>::R{});
+// v[e] = X[t + T e] (forward: exp(-), inverse: exp(+), unnormalised). `region` (L elements
+// of LDS) belongs to the team alone; its contents are undefined on return.
+template <typename T, int L, bool INV>
+__device__ __forceinline__ void team_fft(Cplx<T>* v, Cplx<T>* region, const Cplx<T>* tw, int t) {
+  static_assert(64 % LinePlan<L>::T == 0, "a team must sit inside one wavefront");
+  stockham_plan<T, L, INV>(v, region, tw, t, typename LinePlan<L>::R{});
 }
 
 }  // namespace rfft

@@ -1,10 +1,10 @@
 // The 3D unbounded Navier-Stokes step with forcing (navier_stokes_flow_simulators.py:449-498)
 // as fused sweeps on gfx950. Fields are contiguous (3, nz, ny, nx), x fastest.
 //
-//   SA  : omega1 = omega + pF curl_h(F)  (interior; border omega unchanged), written back to
-//         omega in place (pointwise in omega), and B = u x omega1 into the buffer (all cells).
-//   SB  : omega2 = omega1 + pB curl_h(B) (interior) into the stream-function field, which the
-//         Poisson solve overwrites later in the step.
+//   S1  : omega1 = omega + pF curl_h(F) (interior; border omega unchanged), B = u x omega1,
+//         omega2 = omega1 + pB curl_h(B) (interior) into the stream-function field, which the
+//         Poisson solve overwrites later in the step; omega1 and B live only in LDS and
+//         registers.
 //   SC  : omega3 = omega2 + nu dt/dx^2 lap_h(omega2) (interior; flux border 0), penalised
 //         (value at the clamped index times the per-axis sine factors) into omega; F = 0.
 //   Poisson (poisson_3d.hip) omega -> psi.
@@ -30,8 +30,7 @@ struct sopht_ns3d {
 
 // Stage order of one step, for sopht_ns3d_stage_times.
 static const char* const kStageNames[] = {
-    "vorticity_forcing_cross",  // SA
-    "vorticity_advect",         // SB
+    "vorticity_forcing_advect",    // S1
     "vorticity_diffuse_penalise",  // SC
     "poisson_x_r2c",            // K1
     "poisson_y_fwd",            // K2
@@ -40,7 +39,7 @@ static const char* const kStageNames[] = {
     "poisson_x_c2r",            // K5
     "velocity_curl",            // S2
 };
-static constexpr int kNumStages = 9;
+static constexpr int kNumStages = 8;
 
 namespace sopht {
 namespace {
@@ -72,52 +71,152 @@ __device__ __forceinline__ void curl_at(const T* f0, const T* f1, const T* f2, c
   c[2] = f1[i + 1] - f1[i - 1] - f0[i + sy] + f0[i - sy];
 }
 
+// z planes per S1 work item (two extra planes of B and three of F are recomputed per item)
+constexpr int kS1Chunk = 16;
+
+// S1: omega2 = omega1 + pB curl_h(u x omega1), omega1 = omega +
+// pF curl_h(F), with neither omega1 nor B stored in HBM. A workgroup owns a TX x TY tile of
+// (x, y) columns over a chunk of ZC planes and marches up z. Iteration i
+//   (1) stages F plane i+2 (tile plus a 2-cell ring) in a 4-plane LDS ring,
+//   (2) produces omega1 and B on plane i over the tile plus its 1-cell edge ring (B goes to a
+//       3-plane LDS ring; the tile's own omega1, B(i-1), B(i-2) stay in registers),
+//   (3) after the one barrier, consumes plane i-1: its curl_h(B) takes the in-plane
+//       neighbours from LDS and B(i), B(i-2) from registers.
+// Ring depths make one barrier per plane enough: the slot written at iteration i was last
+// read before the previous iteration's barrier.
 template <typename T>
-__global__ void __launch_bounds__(BX * BY) k_sa(Grid g, T* w, const T* u, const T* F, T* B, T pF,
-                                                int with_forcing) {
-  const int64_t x = blockIdx.x * int64_t(BX) + threadIdx.x;
-  const int64_t y = blockIdx.y * int64_t(BY) + threadIdx.y;
-  const int64_t z = blockIdx.z;
-  if (x >= g.nx || y >= g.ny) return;
-  const int64_t n = g.nz * g.ny * g.nx;
-  const int64_t i = g.at(z, y, x);
-  T w0 = w[i], w1 = w[n + i], w2 = w[2 * n + i];
-  if (with_forcing && g.interior(z, y, x)) {
-    T c[3];
-    curl_at<T>(F, F + n, F + 2 * n, g, z, y, x, c);
-    w0 = w0 + pF * c[0];
-    w1 = w1 + pF * c[1];
-    w2 = w2 + pF * c[2];
-    w[i] = w0;
-    w[n + i] = w1;
-    w[2 * n + i] = w2;
-  }
-  const T u0 = u[i], u1 = u[n + i], u2 = u[2 * n + i];
-  // elementwise_ops_3d.py:427-447: B_i = u_j w_k - w_j u_k
-  B[i] = u1 * w2 - w1 * u2;
-  B[n + i] = u2 * w0 - w2 * u0;
-  B[2 * n + i] = u0 * w1 - w0 * u1;
-}
+struct S1Tile {
+  static constexpr int TX = 64, TY = sizeof(T) == 4 ? 8 : 4, NT = TX * TY;
+  static constexpr int FX = TX + 4, FY = TY + 4, FP = FX * FY;  // F staging box (2-cell ring)
+  static constexpr int GX = TX + 2, GY = TY + 2, GP = GX * GY;  // B box (1-cell ring)
+  static constexpr int NEDGE = 2 * (TX + TY);                    // edge-ring items (no corners)
+};
 
 template <typename T>
-__global__ void __launch_bounds__(BX * BY) k_sb(Grid g, const T* w, const T* B, T* w2out, T pB) {
-  const int64_t x = blockIdx.x * int64_t(BX) + threadIdx.x;
-  const int64_t y = blockIdx.y * int64_t(BY) + threadIdx.y;
-  const int64_t z = blockIdx.z;
-  if (x >= g.nx || y >= g.ny) return;
-  const int64_t n = g.nz * g.ny * g.nx;
-  const int64_t i = g.at(z, y, x);
-  T a0 = w[i], a1 = w[n + i], a2 = w[2 * n + i];
-  if (g.interior(z, y, x)) {
-    T c[3];
-    curl_at<T>(B, B + n, B + 2 * n, g, z, y, x, c);
-    a0 = a0 + pB * c[0];
-    a1 = a1 + pB * c[1];
-    a2 = a2 + pB * c[2];
+__global__ void __launch_bounds__(S1Tile<T>::NT) k_s1(Grid g, const T* w, const T* u, const T* F,
+                                                      T* w2out, T pF, T pB, int with_forcing,
+                                                      int zchunk, int tiles_x, int tiles_y,
+                                                      int xcd_remap) {
+  using S = S1Tile<T>;
+  constexpr int TX = S::TX, TY = S::TY, NT = S::NT, FX = S::FX, FP = S::FP, GX = S::GX, GP = S::GP;
+  __shared__ T fs[4][3][FP];
+  __shared__ T bs[3][3][GP];
+  // Work item of this workgroup. Workgroups are dealt to the 8 XCDs in turn; with the remap,
+  // XCD k takes a contiguous run of items (neighbouring y tiles share halo rows in its L2).
+  int64_t item = blockIdx.x;
+  if (xcd_remap) item = (item % 8) * (gridDim.x / 8) + item / 8;
+  const int64_t tx_i = item % tiles_x;
+  const int64_t ty_i = (item / tiles_x) % tiles_y;
+  const int64_t zc_i = item / (int64_t(tiles_x) * tiles_y);
+  const int tid = threadIdx.x;
+  const int tx = tid % TX, ty = tid / TX;
+  const int64_t x0 = tx_i * TX, y0 = ty_i * TY;
+  const int64_t z0 = zc_i * zchunk;
+  const int64_t z1 = z0 + zchunk < g.nz ? z0 + zchunk : g.nz;
+  const int64_t n = g.nz * g.ny * g.nx, sz = g.ny * g.nx, sy = g.nx;
+  const bool tv = x0 + tx < g.nx && y0 + ty < g.ny;
+  // edge-ring item: x = -1 / TX columns, then y = -1 / TY rows
+  int ex = 0, ey = 0;
+  if (tid < 2 * TY) {
+    ex = (tid & 1) ? TX : -1;
+    ey = tid >> 1;
+  } else if (tid < S::NEDGE) {
+    ex = (tid - 2 * TY) >> 1;
+    ey = ((tid - 2 * TY) & 1) ? TY : -1;
   }
-  w2out[i] = a0;
-  w2out[n + i] = a1;
-  w2out[2 * n + i] = a2;
+  const bool ev = tid < S::NEDGE && x0 + ex >= 0 && x0 + ex < g.nx && y0 + ey >= 0 && y0 + ey < g.ny;
+
+  auto stage_f = [&](int64_t q) {  // F plane q -> ring slot q & 3 (zeros outside the grid)
+    T(*dst)[FP] = fs[q & 3];
+    const bool qin = q >= 0 && q < g.nz;
+    for (int k = tid; k < FP; k += NT) {
+      const int64_t x = x0 + (k % FX) - 2, y = y0 + (k / FX) - 2;
+      const bool in = qin && x >= 0 && x < g.nx && y >= 0 && y < g.ny;
+      const int64_t i = in ? q * sz + y * sy + x : 0;
+      dst[0][k] = in ? F[i] : T(0);
+      dst[1][k] = in ? F[n + i] : T(0);
+      dst[2][k] = in ? F[2 * n + i] : T(0);
+    }
+  };
+  // omega1 and B = u x omega1 at (x0 + xr, y0 + yr, q)
+  auto produce = [&](int xr, int yr, int64_t q, T b[3], T a[3]) {
+    const int64_t x = x0 + xr, y = y0 + yr;
+    const int64_t i = q * sz + y * sy + x;
+    a[0] = w[i];
+    a[1] = w[n + i];
+    a[2] = w[2 * n + i];
+    if (with_forcing && g.interior(q, y, x)) {
+      const int c = (yr + 2) * FX + xr + 2;
+      const T(*Fc)[FP] = fs[q & 3];
+      const T(*Fm)[FP] = fs[(q - 1) & 3];
+      const T(*Fp)[FP] = fs[(q + 1) & 3];
+      const T c0 = Fc[2][c + FX] - Fc[2][c - FX] - Fp[1][c] + Fm[1][c];
+      const T c1 = Fp[0][c] - Fm[0][c] - Fc[2][c + 1] + Fc[2][c - 1];
+      const T c2 = Fc[1][c + 1] - Fc[1][c - 1] - Fc[0][c + FX] + Fc[0][c - FX];
+      a[0] = a[0] + pF * c0;
+      a[1] = a[1] + pF * c1;
+      a[2] = a[2] + pF * c2;
+    }
+    const T u0 = u[i], u1 = u[n + i], u2 = u[2 * n + i];
+    // elementwise_ops_3d.py:427-447: B_i = u_j w_k - w_j u_k
+    b[0] = u1 * a[2] - a[1] * u2;
+    b[1] = u2 * a[0] - a[2] * u0;
+    b[2] = u0 * a[1] - a[0] * u1;
+  };
+
+  if (with_forcing) {
+    stage_f(z0 - 2);
+    stage_f(z0 - 1);
+    stage_f(z0);
+  }
+  __syncthreads();
+  T bm2[3] = {T(0), T(0), T(0)}, bm1[3] = {T(0), T(0), T(0)}, a1prev[3] = {T(0), T(0), T(0)};
+  const int gt = (ty + 1) * GX + tx + 1;
+  for (int64_t i = z0 - 1; i <= z1; ++i) {
+    if (with_forcing) stage_f(i + 2);
+    T b0[3] = {T(0), T(0), T(0)}, a1[3] = {T(0), T(0), T(0)};
+    if (i >= 0 && i < g.nz) {
+      T(*Bq)[GP] = bs[i % 3];
+      if (tv) {
+        produce(tx, ty, i, b0, a1);
+        Bq[0][gt] = b0[0];
+        Bq[1][gt] = b0[1];
+        Bq[2][gt] = b0[2];
+      }
+      if (ev) {
+        T eb[3], ea[3];
+        produce(ex, ey, i, eb, ea);
+        const int ge = (ey + 1) * GX + ex + 1;
+        Bq[0][ge] = eb[0];
+        Bq[1][ge] = eb[1];
+        Bq[2][ge] = eb[2];
+      }
+    }
+    __syncthreads();
+    const int64_t r = i - 1;  // consumed plane: B(r-1) = bm2, B(r+1) = b0, omega1(r) = a1prev
+    if (r >= z0 && tv) {
+      const int64_t x = x0 + tx, y = y0 + ty;
+      T o0 = a1prev[0], o1 = a1prev[1], o2 = a1prev[2];
+      if (g.interior(r, y, x)) {
+        const T(*Bc)[GP] = bs[r % 3];
+        const T c0 = Bc[2][gt + GX] - Bc[2][gt - GX] - b0[1] + bm2[1];
+        const T c1 = b0[0] - bm2[0] - Bc[2][gt + 1] + Bc[2][gt - 1];
+        const T c2 = Bc[1][gt + 1] - Bc[1][gt - 1] - Bc[0][gt + GX] + Bc[0][gt - GX];
+        o0 = o0 + pB * c0;
+        o1 = o1 + pB * c1;
+        o2 = o2 + pB * c2;
+      }
+      const int64_t j = r * sz + y * sy + x;
+      w2out[j] = o0;
+      w2out[n + j] = o1;
+      w2out[2 * n + j] = o2;
+    }
+    for (int c = 0; c < 3; ++c) {
+      bm2[c] = bm1[c];
+      bm1[c] = b0[c];
+      a1prev[c] = a1[c];
+    }
+  }
 }
 
 struct Pen {
@@ -249,7 +348,7 @@ int step_impl(sopht_ns3d* P, void* vort, void* vel, void* psi, void* forcing, vo
   T* u = static_cast<T*>(vel);
   T* ps = static_cast<T*>(psi);
   T* F = static_cast<T*>(forcing);
-  T* B = static_cast<T*>(buffer);
+  (void)buffer;  // u x omega1 never leaves the S1 sweep
   const T dx = T(P->dx);
   const int wf = P->with_forcing && F != nullptr;
   // prefactors as the reference forms them: real_t(dt / (2 * dx * rho)), real_t(dt / (2 dx)),
@@ -260,27 +359,31 @@ int step_impl(sopht_ns3d* P, void* vort, void* vel, void* psi, void* forcing, vo
   const T pC = T(0.5 / double(dx));
   StageTimer& tm = P->timer;
   tm.begin(s);
-  k_sa<T><<<grid, block, 0, s>>>(g, w, u, F, B, pF, wf);
-  SOPHT_CHECK_LAUNCH();
+  {
+    using ST = S1Tile<T>;
+    const int zchunk = kS1Chunk;
+    const int tiles_x = int(ceil_div(g.nx, ST::TX)), tiles_y = int(ceil_div(g.ny, ST::TY));
+    const int64_t items = int64_t(tiles_x) * tiles_y * ceil_div(g.nz, zchunk);
+    k_s1<T><<<unsigned(items), ST::NT, 0, s>>>(g, w, u, F, ps, pF, pB, wf, zchunk, tiles_x, tiles_y,
+                                              items % 8 == 0 ? 1 : 0);
+    SOPHT_CHECK_LAUNCH();
+  }
   tm.mark(0, s);
-  k_sb<T><<<grid, block, 0, s>>>(g, w, B, ps, pB);
-  SOPHT_CHECK_LAUNCH();
-  tm.mark(1, s);
   k_sc<T><<<grid, block, 0, s>>>(g, ps, w, F, pD, make_pen(P), wf);
   SOPHT_CHECK_LAUNCH();
-  tm.mark(2, s);
+  tm.mark(1, s);
   const int64_t n = g.nz * g.ny * g.nx;
   void* const sol[3] = {ps, ps + n, ps + 2 * n};
   const void* const rhs[3] = {w, w + n, w + 2 * n};
   const int64_t st[3] = {g.ny * g.nx, g.nx, 1};
-  int rc = poisson_vector_solve_timed(P->poisson, sol, st, rhs, st, s, &tm, 3);
+  int rc = poisson_vector_solve_timed(P->poisson, sol, st, rhs, st, s, &tm, 2);
   if (rc) return rc;
   const T U0 = P->with_free_stream ? T(U[0]) : T(0);
   const T U1 = P->with_free_stream ? T(U[1]) : T(0);
   const T U2 = P->with_free_stream ? T(U[2]) : T(0);
   k_s2<T><<<grid, block, 0, s>>>(g, ps, u, pC, U0, U1, U2);
   SOPHT_CHECK_LAUNCH();
-  tm.mark(8, s);
+  tm.mark(7, s);
   tm.end();
   return SOPHT_OK;
 }

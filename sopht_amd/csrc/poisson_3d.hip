@@ -30,65 +30,69 @@ struct sopht_poisson3d {
   void* slab;       // complex slab, 3 components
   void* ghat;       // real octant
   void* tw[3];      // twiddles for lengths 2nx, 2ny, 2nz (x, y, z)
+  int64_t gz;       // G^ line stride (elements): nz + 1 rounded up to a 128-byte line
   int64_t slab_bytes, ghat_bytes;
 };
 
 namespace sopht {
 namespace {
 
-using rfft::ColSlot;
 using rfft::LinePlan;
-using rfft::RowSlot;
-using rfft::team_fft;
+using rfft::region_stride;
 using rfft::static_for;
+using rfft::team_fft;
 
-// Dynamic LDS per workgroup stays within 64 KiB, so two 1024-thread workgroups fit a CU.
-constexpr int kLdsBudget = 64 * 1024;
+// Slab rows (kx = 0..nx) are padded to a whole number of 128-byte lines, so every column
+// tile of the y and z passes reads and writes whole cache lines.
+template <typename T>
+constexpr int kLineCplx = 128 / int(sizeof(Cplx<T>));
+#ifndef SOPHT_COL_HALF
+#define SOPHT_COL_HALF 1
+#endif
+#ifndef SOPHT_ROW_PAD
+#define SOPHT_ROW_PAD 1
+#endif
+// Column tiles are half a 128-byte line when SOPHT_COL_HALF is set; the two halves of a line
+// then go to workgroups b and b + 8, which the dispatcher sends to the same XCD one after the
+// other, so the second half is read from that XCD's L2.
+constexpr int kColHalf = SOPHT_COL_HALF;
+constexpr int64_t kRowPad = SOPHT_ROW_PAD;
 
-// Threads per workgroup: 1024 (a 128-VGPR budget) for 8-element FP32 lines; 512 (256
-// VGPRs) for the 12/24-element plans and FP64, which need more registers.
-template <typename T, int L>
-constexpr int max_threads() {
-  return (LinePlan<L>::E > 8 || sizeof(T) == 8) ? 512 : 1024;
+// (row, first kx) of the column tile of workgroup b; false for the padding workgroups.
+__device__ __forceinline__ bool col_tile(int64_t b, int64_t ncols, int64_t nrows, int wc,
+                                         int64_t& row, int64_t& kx0) {
+  if (kColHalf) {
+    const int64_t lines = (ncols + 2 * wc - 1) / (2 * wc);
+    const int64_t line = (b / 16) * 8 + b % 8;
+    const int64_t half = (b / 8) % 2;
+    row = line / lines;
+    kx0 = ((line % lines) * 2 + half) * wc;
+    return row < nrows;
+  }
+  const int64_t tiles = (ncols + wc - 1) / wc;
+  row = b / tiles;
+  kx0 = (b % tiles) * wc;
+  return row < nrows;
 }
 
-// Column tiles (y and z passes): WC neighbouring kx columns per workgroup, T threads per
-// column.
-template <typename T, int L>
-constexpr int cols_per_block() {
-  constexpr int sc = int(sizeof(Cplx<T>));
-  int wc = max_threads<T, L>() / LinePlan<L>::T;
-  const int cap = kLdsBudget / (L * sc);
-  if (wc > cap) wc = cap;
-  return wc < 1 ? 1 : wc;
+inline int64_t col_tile_blocks(int64_t ncols, int64_t nrows, int wc) {
+  if (kColHalf) {
+    const int64_t lines = (ncols + 2 * wc - 1) / (2 * wc);
+    return 16 * ((lines * nrows + 7) / 8);
+  }
+  return (ncols + wc - 1) / wc * nrows;
 }
 
-// Row tiles (x passes): LPB row pairs per workgroup.
-template <typename T, int L>
-constexpr int rows_per_block() {
-  constexpr int sc = int(sizeof(Cplx<T>));
-  int lpb = max_threads<T, L>() / LinePlan<L>::T;
-  const int cap = kLdsBudget / (RowSlot<L>::kStride * sc);
-  if (lpb > cap) lpb = cap;
-  return lpb < 1 ? 1 : lpb;
-}
-
-template <typename T, int L>
-constexpr size_t row_lds_bytes() {
-  return size_t(rows_per_block<T, L>()) * RowSlot<L>::kStride * sizeof(Cplx<T>);
-}
-
-template <typename T, int L>
-constexpr size_t col_lds_bytes() {
-  return size_t(L) * cols_per_block<T, L>() * sizeof(Cplx<T>);
-}
+inline int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 
 struct SlabGeom {
   int64_t nz, ny, nx;  // physical grid
+  int64_t rs;          // row stride (complex elements), >= nx + 1
+  int64_t gz;          // G^ line stride (elements), >= nz + 1
   int ncomp;
   // H index of (c, z, ky, kx)
   __device__ __forceinline__ int64_t h(int c, int64_t z, int64_t ky, int64_t kx) const {
-    return ((int64_t(c) * nz + z) * (2 * ny) + ky) * (nx + 1) + kx;
+    return ((int64_t(c) * nz + z) * (2 * ny) + ky) * rs + kx;
   }
   __device__ __forceinline__ int64_t row_h(int64_t r) const {  // H row of input row r
     const int c = int(r / (nz * ny));
@@ -106,19 +110,41 @@ struct CompPtrsW {
   int64_t sz, sy, sx;
 };
 
+// Row passes (x): one team of T threads per row pair, RowTile::NTEAM teams per workgroup.
+template <typename T, int L>
+struct RowTile {
+  static constexpr int E = LinePlan<L>::E, TT = LinePlan<L>::T;
+  static constexpr int NTEAM = TT >= 256 ? 1 : 256 / TT;
+  static constexpr int NT = NTEAM * TT;
+  static constexpr int S = region_stride<L, false>();
+  static constexpr size_t lds = size_t(NTEAM) * S * sizeof(Cplx<T>);
+};
+
+// Column passes (y, z): WC neighbouring kx columns per workgroup (one 128-byte line of a
+// slab row), one team per column. Loads and stores run with kx fastest (lanes (w, row));
+// the transform runs one team per column; LDS region w (S elements) holds column w.
+template <typename T, int L>
+struct ColTile {
+  static constexpr int E = LinePlan<L>::E, TT = LinePlan<L>::T;
+  static constexpr int WC = kColHalf ? kLineCplx<T> / 2 : kLineCplx<T>;
+  static constexpr int NT = WC * TT;
+  static constexpr int S = region_stride<L, true>();
+  static constexpr size_t lds = size_t(WC) * S * sizeof(Cplx<T>);
+};
+
 // K1: x real-to-complex of zero-padded rows, two rows (ra, rb) per complex FFT of length
 // L = 2nx; the pair is separated with A = (Z[k] + conj Z[L-k])/2, B = (Z[k] - conj Z[L-k])/(2i).
 template <typename T, int L>
-__global__ void __launch_bounds__((max_threads<T, L>())) k1_xr2c(SlabGeom g, CompPtrs in, Cplx<T>* H,
-                                                const Cplx<T>* tw) {
-  constexpr int E = LinePlan<L>::E, TT = LinePlan<L>::T;
-  constexpr int LPB = rows_per_block<T, L>();
+__global__ void __launch_bounds__((RowTile<T, L>::NT)) k1_xr2c(SlabGeom g, CompPtrs in, Cplx<T>* H,
+                                                             const Cplx<T>* tw) {
+  using RT = RowTile<T, L>;
+  constexpr int E = RT::E, TT = RT::TT;
   extern __shared__ unsigned char smem_raw[];
   Cplx<T>* xb = reinterpret_cast<Cplx<T>*>(smem_raw);
   const int t = threadIdx.x % TT;
-  const int line = threadIdx.x / TT;
+  const int team = threadIdx.x / TT;
   const int64_t rows = int64_t(g.ncomp) * g.nz * g.ny;
-  const int64_t q = int64_t(blockIdx.x) * LPB + line;
+  const int64_t q = int64_t(blockIdx.x) * RT::NTEAM + team;
   const int64_t ra = 2 * q, rb = 2 * q + 1;
   auto row_ptr = [&](int64_t r) -> const T* {
     const int c = int(r / (g.nz * g.ny));
@@ -137,18 +163,18 @@ __global__ void __launch_bounds__((max_threads<T, L>())) k1_xr2c(SlabGeom g, Com
       v[e] = Cplx<T>{T(0), T(0)};
     }
   });
-  RowSlot<L> slot{line};
-  team_fft<T, L, false>(v, xb, tw, t, slot);
+  Cplx<T>* const line = xb + team * RT::S;
+  team_fft<T, L, false>(v, line, tw, t);
   static_for<0, E>([&](auto e_) {
     constexpr int e = decltype(e_)::value;
-    xb[slot(t + TT * e)] = v[e];
+    line[t + TT * e] = v[e];
   });
-  __syncthreads();
+  rfft::wave_sync();
   Cplx<T>* ha = ra < rows ? H + g.row_h(ra) : nullptr;
   Cplx<T>* hb = rb < rows ? H + g.row_h(rb) : nullptr;
   for (int k = t; k <= g.nx; k += TT) {
-    const Cplx<T> zk = xb[slot(k)];
-    const Cplx<T> zm = xb[slot((L - k) % L)];
+    const Cplx<T> zk = line[k];
+    const Cplx<T> zm = line[(L - k) % L];
     if (ha) ha[k] = Cplx<T>{T(0.5) * (zk.re + zm.re), T(0.5) * (zk.im - zm.im)};
     if (hb) hb[k] = Cplx<T>{T(0.5) * (zk.im + zm.im), T(0.5) * (zm.re - zk.re)};
   }
@@ -157,89 +183,140 @@ __global__ void __launch_bounds__((max_threads<T, L>())) k1_xr2c(SlabGeom g, Com
 // K2 / K4: FFT along y of columns (c, z, kx). Forward: inputs y < ny (zero padding), all
 // 2ny outputs stored. Inverse: all 2ny inputs, outputs y < ny stored.
 template <typename T, int L, bool INV>
-__global__ void __launch_bounds__((max_threads<T, L>())) k_ycols(SlabGeom g, Cplx<T>* H, const Cplx<T>* tw) {
-  constexpr int E = LinePlan<L>::E, TT = LinePlan<L>::T;
-  constexpr int WC = cols_per_block<T, L>();
+__global__ void __launch_bounds__((ColTile<T, L>::NT)) k_ycols(SlabGeom g, Cplx<T>* H, const Cplx<T>* tw) {
+  using CT = ColTile<T, L>;
+  constexpr int E = CT::E, TT = CT::TT, WC = CT::WC, S = CT::S;
+  constexpr int KIN = INV ? E : E / 2, KOUT = INV ? E / 2 : E;  // rows = TT * K
   extern __shared__ unsigned char smem_raw[];
   Cplx<T>* xb = reinterpret_cast<Cplx<T>*>(smem_raw);
-  const int w = threadIdx.x % WC;
-  const int t = threadIdx.x / WC;
+  const int tid = threadIdx.x;
+  const int lw = tid % WC, lr = tid / WC;  // load / store role
+  const int team = tid / TT, t = tid % TT;  // transform role
   const int64_t ncols = g.nx + 1;
-  const int64_t tiles = (ncols + WC - 1) / WC;
-  const int64_t tile = blockIdx.x % tiles;
-  const int64_t cz = blockIdx.x / tiles;  // c * nz + z
+  int64_t cz, kx0;  // cz = c * nz + z
+  if (!col_tile(blockIdx.x, ncols, int64_t(g.ncomp) * g.nz, WC, cz, kx0)) return;
   const int c = int(cz / g.nz);
   const int64_t z = cz % g.nz;
-  const int64_t kx = tile * WC + w;
+  const int64_t kx = kx0 + lw;
   const bool live = kx < ncols;
+  // a wavefront whose teams all have kx >= ncols skips its transforms
+  const bool wave_live = kx0 + (tid / 64) * (64 / TT) < ncols;
   Cplx<T>* col = H + g.h(c, z, 0, live ? kx : 0);
-  const int64_t ys = ncols;  // stride between consecutive y (or ky)
+  const int64_t ys = g.rs;  // stride between consecutive y (or ky)
+  static_for<0, KIN>([&](auto k_) {
+    constexpr int k = decltype(k_)::value;
+    const int y = lr + TT * k;
+    xb[lw * S + y] = live ? col[y * ys] : Cplx<T>{T(0), T(0)};
+  });
+  __syncthreads();
+  Cplx<T>* const line = xb + team * S;
   Cplx<T> v[E];
   static_for<0, E>([&](auto e_) {
     constexpr int e = decltype(e_)::value;
-    if constexpr (INV || e < E / 2) v[e] = col[(t + TT * e) * ys];
+    if constexpr (e < KIN) v[e] = line[t + TT * e];
     else v[e] = Cplx<T>{T(0), T(0)};
   });
-  team_fft<T, L, INV>(v, xb, tw, t, ColSlot<WC>{w});
+  rfft::wave_sync();
+  if (wave_live) team_fft<T, L, INV>(v, line, tw, t);
+  static_for<0, KOUT>([&](auto e_) {
+    constexpr int e = decltype(e_)::value;
+    line[t + TT * e] = v[e];
+  });
+  __syncthreads();
   if (live)
-    static_for<0, (INV ? E / 2 : E)>([&](auto e_) {
-      constexpr int e = decltype(e_)::value;
-      col[(t + TT * e) * ys] = v[e];
+    static_for<0, KOUT>([&](auto k_) {
+      constexpr int k = decltype(k_)::value;
+      const int y = lr + TT * k;
+      col[y * ys] = xb[lw * S + y];
     });
 }
 
-// K3: z pencils (c, ky, kx): zero-padded forward FFT, times G^ (real, even: read from the
-// octant), inverse FFT, crop to z < nz.
+// K3: z pencils (c, ky, kx): zero-padded forward FFT, times G^ (real, even), inverse FFT,
+// crop to z < nz. A workgroup owns one kx tile of one |ky| = kyo and runs the pencils of
+// every component at ky = kyo and ky = 2ny - kyo (the same G^ line) one after another: G^
+// stays in registers, and the next pencil tile's loads are in flight during the transforms.
+// G^ is stored [kyo][kx][kz], kz fastest, so a team reads its line contiguously.
 template <typename T, int L>
-__global__ void __launch_bounds__((max_threads<T, L>())) k3_zconv(SlabGeom g, Cplx<T>* H, const T* ghat,
-                                                 const Cplx<T>* tw) {
-  constexpr int E = LinePlan<L>::E, TT = LinePlan<L>::T;
-  constexpr int WC = cols_per_block<T, L>();
+__global__ void __launch_bounds__((ColTile<T, L>::NT)) k3_zconv(SlabGeom g, Cplx<T>* H, const T* ghat,
+                                                              const Cplx<T>* tw) {
+  using CT = ColTile<T, L>;
+  constexpr int E = CT::E, TT = CT::TT, WC = CT::WC, S = CT::S, K = E / 2;
   extern __shared__ unsigned char smem_raw[];
   Cplx<T>* xb = reinterpret_cast<Cplx<T>*>(smem_raw);
-  const int w = threadIdx.x % WC;
-  const int t = threadIdx.x / WC;
+  const int tid = threadIdx.x;
+  const int lw = tid % WC, lr = tid / WC;
+  const int team = tid / TT, t = tid % TT;
   const int64_t ncols = g.nx + 1;
-  const int64_t tiles = (ncols + WC - 1) / WC;
-  const int64_t tile = blockIdx.x % tiles;
-  const int64_t cky = blockIdx.x / tiles;  // c * 2ny + ky
-  const int c = int(cky / (2 * g.ny));
-  const int64_t ky = cky % (2 * g.ny);
-  const int64_t kx = tile * WC + w;
+  int64_t kyo, kx0;  // kyo = 0..ny
+  if (!col_tile(blockIdx.x, ncols, g.ny + 1, WC, kyo, kx0)) return;
+  const int njobs = (kyo == 0 || kyo == g.ny) ? g.ncomp : 2 * g.ncomp;
+  const int64_t kx = kx0 + lw;
   const bool live = kx < ncols;
-  const int64_t kxs = live ? kx : 0;
-  Cplx<T>* col = H + g.h(c, 0, ky, kxs);
-  const int64_t zs = 2 * g.ny * ncols;  // stride between consecutive z
-  Cplx<T> v[E];
-  static_for<0, E>([&](auto e_) {
-    constexpr int e = decltype(e_)::value;
-    if constexpr (e < E / 2) v[e] = col[(t + TT * e) * zs];
-    else v[e] = Cplx<T>{T(0), T(0)};
-  });
-  // G^ at this thread's frequencies kz = t + T e, loaded ahead of the transform
+  const bool wave_live = kx0 + (tid / 64) * (64 / TT) < ncols;
   T gv[E];
   {
-    const int64_t kyo = ky <= g.ny ? ky : 2 * g.ny - ky;
-    const T* gcol = ghat + kyo * ncols + kxs;
-    const int64_t gz = (g.ny + 1) * ncols;
+    const int64_t kxt = kx0 + team;
+    const T* gq = ghat + (kyo * ncols + (kxt < ncols ? kxt : 0)) * g.gz;
     static_for<0, E>([&](auto e_) {
       constexpr int e = decltype(e_)::value;
       const int kz = t + TT * e;
-      gv[e] = gcol[(kz <= L / 2 ? kz : L - kz) * gz];
+      gv[e] = gq[kz <= L / 2 ? kz : L - kz];
     });
   }
-  ColSlot<WC> slot{w};
-  team_fft<T, L, false>(v, xb, tw, t, slot);
-  static_for<0, E>([&](auto e_) {
-    constexpr int e = decltype(e_)::value;
-    v[e] = Cplx<T>{v[e].re * gv[e], v[e].im * gv[e]};
-  });
-  team_fft<T, L, true>(v, xb, tw, t, slot);
-  if (live)
-    static_for<0, E / 2>([&](auto e_) {
-      constexpr int e = decltype(e_)::value;
-      col[(t + TT * e) * zs] = v[e];
+  const int64_t zs = 2 * g.ny * g.rs;  // stride between consecutive z
+  auto column = [&](int j) -> Cplx<T>* {
+    const int c = j % g.ncomp;
+    const int64_t ky = j < g.ncomp ? kyo : 2 * g.ny - kyo;
+    return H + g.h(c, 0, ky, live ? kx : 0);
+  };
+  Cplx<T> pf[K];
+  auto load = [&](int j) {
+    const Cplx<T>* col = column(j);
+    static_for<0, K>([&](auto k_) {
+      constexpr int k = decltype(k_)::value;
+      pf[k] = live ? col[(lr + TT * k) * zs] : Cplx<T>{T(0), T(0)};
     });
+  };
+  Cplx<T>* const line = xb + team * S;
+  load(0);
+  for (int j = 0; j < njobs; ++j) {
+    static_for<0, K>([&](auto k_) {
+      constexpr int k = decltype(k_)::value;
+      const int z = lr + TT * k;
+      xb[lw * S + z] = pf[k];
+    });
+    __syncthreads();
+    Cplx<T> v[E];
+    static_for<0, E>([&](auto e_) {
+      constexpr int e = decltype(e_)::value;
+      if constexpr (e < K) v[e] = line[t + TT * e];
+      else v[e] = Cplx<T>{T(0), T(0)};
+    });
+    rfft::wave_sync();
+    if (j + 1 < njobs) load(j + 1);
+    if (wave_live) {
+      team_fft<T, L, false>(v, line, tw, t);
+      static_for<0, E>([&](auto e_) {
+        constexpr int e = decltype(e_)::value;
+        v[e] = Cplx<T>{v[e].re * gv[e], v[e].im * gv[e]};
+      });
+      team_fft<T, L, true>(v, line, tw, t);
+    }
+    static_for<0, K>([&](auto e_) {
+      constexpr int e = decltype(e_)::value;
+      line[t + TT * e] = v[e];
+    });
+    __syncthreads();
+    if (live) {
+      Cplx<T>* col = column(j);
+      static_for<0, K>([&](auto k_) {
+        constexpr int k = decltype(k_)::value;
+        const int z = lr + TT * k;
+        col[z * zs] = xb[lw * S + z];
+      });
+    }
+    __syncthreads();
+  }
 }
 
 // K5: x complex-to-real of two rows per complex inverse FFT, crop to x < nx. The Hermitian
@@ -247,21 +324,21 @@ __global__ void __launch_bounds__((max_threads<T, L>())) k3_zconv(SlabGeom g, Cp
 // Z[L-k] = conj(A[k]) + i conj(B[k]); imaginary parts at kx = 0 and nx are dropped
 // (irfft semantics).
 template <typename T, int L>
-__global__ void __launch_bounds__((max_threads<T, L>())) k5_xc2r(SlabGeom g, const Cplx<T>* H, CompPtrsW out,
-                                                const Cplx<T>* tw) {
-  constexpr int E = LinePlan<L>::E, TT = LinePlan<L>::T;
-  constexpr int LPB = rows_per_block<T, L>();
+__global__ void __launch_bounds__((RowTile<T, L>::NT)) k5_xc2r(SlabGeom g, const Cplx<T>* H, CompPtrsW out,
+                                                             const Cplx<T>* tw) {
+  using RT = RowTile<T, L>;
+  constexpr int E = RT::E, TT = RT::TT;
   extern __shared__ unsigned char smem_raw[];
   Cplx<T>* xb = reinterpret_cast<Cplx<T>*>(smem_raw);
   const int t = threadIdx.x % TT;
-  const int line = threadIdx.x / TT;
+  const int team = threadIdx.x / TT;
   const int64_t rows = int64_t(g.ncomp) * g.nz * g.ny;
-  const int64_t q = int64_t(blockIdx.x) * LPB + line;
+  const int64_t q = int64_t(blockIdx.x) * RT::NTEAM + team;
   const int64_t ra = 2 * q, rb = 2 * q + 1;
   const int64_t nx = g.nx;
   const Cplx<T>* ha = ra < rows ? H + g.row_h(ra) : nullptr;
   const Cplx<T>* hb = rb < rows ? H + g.row_h(rb) : nullptr;
-  RowSlot<L> slot{line};
+  Cplx<T>* const line = xb + team * RT::S;
   for (int k = t; k <= nx; k += TT) {
     Cplx<T> a = {T(0), T(0)}, bb = {T(0), T(0)};
     if (ha) a = ha[k];
@@ -270,17 +347,17 @@ __global__ void __launch_bounds__((max_threads<T, L>())) k5_xc2r(SlabGeom g, con
       a.im = T(0);
       bb.im = T(0);
     }
-    xb[slot(k)] = Cplx<T>{a.re - bb.im, a.im + bb.re};
-    if (k > 0 && k < nx) xb[slot(L - k)] = Cplx<T>{a.re + bb.im, bb.re - a.im};
+    line[k] = Cplx<T>{a.re - bb.im, a.im + bb.re};
+    if (k > 0 && k < nx) line[L - k] = Cplx<T>{a.re + bb.im, bb.re - a.im};
   }
-  __syncthreads();
+  rfft::wave_sync();
   Cplx<T> v[E];
   static_for<0, E>([&](auto e_) {
     constexpr int e = decltype(e_)::value;
-    v[e] = xb[slot(t + TT * e)];
+    v[e] = line[t + TT * e];
   });
-  __syncthreads();  // exchange area reuse
-  team_fft<T, L, true>(v, xb, tw, t, slot);
+  rfft::wave_sync();
+  team_fft<T, L, true>(v, line, tw, t);
   auto row_ptr = [&](int64_t r) -> T* {
     const int c = int(r / (g.nz * g.ny));
     const int64_t rem = r % (g.nz * g.ny);
@@ -328,11 +405,17 @@ __global__ void k_dct1_axis(const double* in, double* out, int64_t n, int64_t st
   }
 }
 
+// scaled, converted to real_t and stored [y][x][z] (z fastest, line stride gz)
 template <typename T>
-__global__ void k_scale_convert(const double* in, T* out, int64_t n, double s) {
+__global__ void k_ghat_store(const double* in, T* out, int64_t nz1, int64_t ny1, int64_t nx1,
+                             int64_t gz, double s) {
+  const int64_t n = nz1 * ny1 * nx1;
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
-       i += int64_t(gridDim.x) * blockDim.x)
-    out[i] = T(in[i] * s);
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t z = i % nz1, yx = i / nz1;  // output order: consecutive threads, consecutive z
+    const int64_t x = yx % nx1, y = yx / nx1;
+    out[yx * gz + z] = T(in[(z * ny1 + y) * nx1 + x] * s);
+  }
 }
 
 // ------------------------------------------------------------------ dispatch on length
@@ -352,9 +435,8 @@ int launch_k1(int64_t L, SlabGeom g, CompPtrs in, Cplx<T>* H, const Cplx<T>* tw,
   const int64_t pairs = (rows + 1) / 2;
 #define X(n)                                                                          \
   if (L == n) {                                                                       \
-    constexpr int lpb = rows_per_block<T, n>();                                       \
-    const int threads = lpb * LinePlan<n>::T;                                         \
-    k1_xr2c<T, n><<<ceil_div(pairs, lpb), threads, row_lds_bytes<T, n>(), s>>>(g, in, H, tw); \
+    using RT = RowTile<T, n>;                                                         \
+    k1_xr2c<T, n><<<ceil_div(pairs, RT::NTEAM), RT::NT, RT::lds, s>>>(g, in, H, tw);  \
     SOPHT_CHECK_LAUNCH();                                                             \
     return SOPHT_OK;                                                                  \
   }
@@ -367,10 +449,9 @@ template <typename T, bool INV>
 int launch_ycols(int64_t L, SlabGeom g, Cplx<T>* H, const Cplx<T>* tw, hipStream_t s) {
 #define X(n)                                                                          \
   if (L == n) {                                                                       \
-    constexpr int wc = cols_per_block<T, n>();                                        \
-    const int threads = wc * LinePlan<n>::T;                                          \
-    const int64_t tiles = (g.nx + 1 + wc - 1) / wc;                                   \
-    k_ycols<T, n, INV><<<tiles * g.ncomp * g.nz, threads, col_lds_bytes<T, n>(), s>>>(g, H, tw); \
+    using CT = ColTile<T, n>;                                                         \
+    const int64_t nb = col_tile_blocks(g.nx + 1, int64_t(g.ncomp) * g.nz, CT::WC);    \
+    k_ycols<T, n, INV><<<nb, CT::NT, CT::lds, s>>>(g, H, tw);                         \
     SOPHT_CHECK_LAUNCH();                                                             \
     return SOPHT_OK;                                                                  \
   }
@@ -384,10 +465,9 @@ int launch_k3(int64_t L, SlabGeom g, Cplx<T>* H, const T* ghat, const Cplx<T>* t
               hipStream_t s) {
 #define X(n)                                                                          \
   if (L == n) {                                                                       \
-    constexpr int wc = cols_per_block<T, n>();                                        \
-    const int threads = wc * LinePlan<n>::T;                                          \
-    const int64_t tiles = (g.nx + 1 + wc - 1) / wc;                                   \
-    k3_zconv<T, n><<<tiles * g.ncomp * 2 * g.ny, threads, col_lds_bytes<T, n>(), s>>>(g, H, ghat, tw); \
+    using CT = ColTile<T, n>;                                                         \
+    const int64_t nb = col_tile_blocks(g.nx + 1, g.ny + 1, CT::WC);                   \
+    k3_zconv<T, n><<<nb, CT::NT, CT::lds, s>>>(g, H, ghat, tw);                       \
     SOPHT_CHECK_LAUNCH();                                                             \
     return SOPHT_OK;                                                                  \
   }
@@ -403,9 +483,8 @@ int launch_k5(int64_t L, SlabGeom g, const Cplx<T>* H, CompPtrsW out, const Cplx
   const int64_t pairs = (rows + 1) / 2;
 #define X(n)                                                                          \
   if (L == n) {                                                                       \
-    constexpr int lpb = rows_per_block<T, n>();                                       \
-    const int threads = lpb * LinePlan<n>::T;                                         \
-    k5_xc2r<T, n><<<ceil_div(pairs, lpb), threads, row_lds_bytes<T, n>(), s>>>(g, H, out, tw); \
+    using RT = RowTile<T, n>;                                                         \
+    k5_xc2r<T, n><<<ceil_div(pairs, RT::NTEAM), RT::NT, RT::lds, s>>>(g, H, out, tw); \
     SOPHT_CHECK_LAUNCH();                                                             \
     return SOPHT_OK;                                                                  \
   }
@@ -415,10 +494,16 @@ int launch_k5(int64_t L, SlabGeom g, const Cplx<T>* H, CompPtrsW out, const Cplx
 }
 
 template <typename T>
+SlabGeom slab_geom(const sopht_poisson3d* P, int ncomp) {
+  return SlabGeom{P->nz, P->ny, P->nx, kRowPad ? round_up(P->nx + 1, kLineCplx<T>) : P->nx + 1,
+                  P->gz, ncomp};
+}
+
+template <typename T>
 int solve_impl(sopht_poisson3d* P, int ncomp, void* const sol[3], const int64_t* ss,
                const void* const rhs[3], const int64_t* rs, hipStream_t s,
                StageTimer* timer = nullptr, int first_stage = 0) {
-  SlabGeom g{P->nz, P->ny, P->nx, ncomp};
+  const SlabGeom g = slab_geom<T>(P, ncomp);
   CompPtrs in;
   CompPtrsW out;
   for (int c = 0; c < 3; ++c) {
@@ -504,9 +589,9 @@ int build_ghat(sopht_poisson3d* P, hipStream_t s) {
   }
   const double scale = P->dx * P->dx * P->dx / (8.0 * double(P->nz) * double(P->ny) * double(P->nx));
   if (P->dtype == SOPHT_F32)
-    k_scale_convert<float><<<blocks, 256, 0, s>>>(src, static_cast<float*>(P->ghat), n, scale);
+    k_ghat_store<float><<<blocks, 256, 0, s>>>(src, static_cast<float*>(P->ghat), nz1, ny1, nx1, P->gz, scale);
   else
-    k_scale_convert<double><<<blocks, 256, 0, s>>>(src, static_cast<double*>(P->ghat), n, scale);
+    k_ghat_store<double><<<blocks, 256, 0, s>>>(src, static_cast<double*>(P->ghat), nz1, ny1, nx1, P->gz, scale);
   SOPHT_CHECK_LAUNCH();
   SOPHT_CHECK_HIP(hipStreamSynchronize(s));
   free(host);
@@ -552,8 +637,10 @@ int sopht_poisson3d_create(int dtype, int64_t nz, int64_t ny, int64_t nx, double
   if (dtype == SOPHT_F32) P->dx = double(float(P->dx));  // the reference's real_t(dx)
   const size_t cs = dtype == SOPHT_F32 ? sizeof(Cplx<float>) : sizeof(Cplx<double>);
   const size_t rsz = dtype == SOPHT_F32 ? sizeof(float) : sizeof(double);
-  P->slab_bytes = int64_t(3) * nz * 2 * ny * (nx + 1) * int64_t(cs);
-  P->ghat_bytes = (nz + 1) * (ny + 1) * (nx + 1) * int64_t(rsz);
+  const int64_t rs = kRowPad ? round_up(nx + 1, int64_t(128 / cs)) : nx + 1;
+  P->gz = kRowPad ? round_up(nz + 1, int64_t(128 / rsz)) : nz + 1;
+  P->slab_bytes = int64_t(3) * nz * 2 * ny * rs * int64_t(cs);
+  P->ghat_bytes = (ny + 1) * (nx + 1) * P->gz * int64_t(rsz);
   int rc = SOPHT_OK;
   auto fail = [&](int code) {
     sopht_poisson3d_destroy(P);
