@@ -157,6 +157,28 @@ __device__ __forceinline__ void rdft(Cplx<T>* v) {
   }
 }
 
+// DFT of length N whose inputs v[N/2 .. N) are zero: X[2m] = DFT_{N/2}(v)[m] and
+// X[2m+1] = DFT_{N/2}(v[r] w_N^r)[m]. (Constant propagation cannot drop the zeros itself:
+// x + 0.0 is not x for x = -0.0.)
+template <typename T, int N, bool INV>
+__device__ __forceinline__ void rdft_half_in(Cplx<T>* v) {
+  static_assert(N % 2 == 0, "half-zero codelet needs an even length");
+  constexpr int M = N / 2;
+  Cplx<T> a[M], b[M];
+  static_for<0, M>([&](auto m_) {
+    constexpr int m = decltype(m_)::value;
+    a[m] = v[m];
+    b[m] = twmul<T, N, m, INV>(v[m]);
+  });
+  rdft<T, M, INV>(a);
+  rdft<T, M, INV>(b);
+  static_for<0, M>([&](auto m_) {
+    constexpr int m = decltype(m_)::value;
+    v[2 * m] = a[m];
+    v[2 * m + 1] = b[m];
+  });
+}
+
 // ------------------------------------------------------------------ line plans
 // A line of length L is transformed by a team of T = L / E threads; thread t holds the E
 // elements x[t + T e] (e < E). Stage s of the Stockham sequence has radix R_s, with R_s | E:
@@ -196,6 +218,11 @@ SOPHT_LINE_PLAN(1024, 16, 16, 8, 8)
 #undef SOPHT_LINE_PLAN
 
 constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v / 2); }
+
+#ifndef SOPHT_TWIDDLE_PRODUCTS
+#define SOPHT_TWIDDLE_PRODUCTS 1
+#endif
+constexpr bool kTwiddleProducts = SOPHT_TWIDDLE_PRODUCTS;
 
 // LDS layout of the exchange after a stage of radix R and write stride NS: element p of the
 // team's line lives at p ^ (((p >> A) & (2^MB - 1)) << B). The stage writes
@@ -239,7 +266,9 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <typename T, int L, int R, int NS, bool INV>
+// HALF_IN (first stage only): the line's upper half x[L/2 .. L) is zero, i.e. the inputs
+// r >= R/2 of every butterfly.
+template <typename T, int L, int R, int NS, bool INV, bool HALF_IN>
 __device__ __forceinline__ void stockham_stage(Cplx<T>* v, Cplx<T>* xb, const Cplx<T>* tw,
                                                int t) {
   constexpr int E = LinePlan<L>::E, TT = LinePlan<L>::T;
@@ -251,14 +280,24 @@ __device__ __forceinline__ void stockham_stage(Cplx<T>* v, Cplx<T>* xb, const Cp
     Cplx<T> u[R];
     static_for<0, R>([&](auto r_) { u[decltype(r_)::value] = v[p + PER * decltype(r_)::value]; });
     if constexpr (NS > 1) {
+      // w^{r k}, w = exp(-2 pi i / (NS R)): the powers r = 2^b come from the table, the others
+      // are products of two of them (fewer vector-memory instructions per butterfly)
       const int k = (t + p * TT) % NS;
+      constexpr int step = L / (NS * R);
+      Cplx<T> wr[R];
       static_for<1, R>([&](auto r_) {
         constexpr int r = decltype(r_)::value;
-        const Cplx<T> w = tw[r * k * (L / (NS * R))];
-        u[r] = INV ? cmulc(u[r], w) : cmul(u[r], w);
+        constexpr int hb = kTwiddleProducts ? 1 << ilog2(r) : r;
+        if constexpr (hb == r) wr[r] = tw[r * k * step];
+        else wr[r] = cmul(wr[hb], wr[r - hb]);
+      });
+      static_for<1, R>([&](auto r_) {
+        constexpr int r = decltype(r_)::value;
+        u[r] = INV ? cmulc(u[r], wr[r]) : cmul(u[r], wr[r]);
       });
     }
-    rdft<T, R, INV>(u);
+    if constexpr (HALF_IN) rdft_half_in<T, R, INV>(u);
+    else rdft<T, R, INV>(u);
     static_for<0, R>([&](auto r_) { v[p + PER * decltype(r_)::value] = u[decltype(r_)::value]; });
   });
   if constexpr (NS * R < L) {
@@ -282,25 +321,27 @@ __device__ __forceinline__ void stockham_stage(Cplx<T>* v, Cplx<T>* xb, const Cp
   }
 }
 
-template <typename T, int L, bool INV, int NS, int R0, int... Rest>
+template <typename T, int L, bool INV, bool HALF_IN, int NS, int R0, int... Rest>
 __device__ __forceinline__ void stockham_run(Cplx<T>* v, Cplx<T>* xb, const Cplx<T>* tw, int t) {
-  stockham_stage<T, L, R0, NS, INV>(v, xb, tw, t);
-  if constexpr (sizeof...(Rest) > 0) stockham_run<T, L, INV, NS * R0, Rest...>(v, xb, tw, t);
+  stockham_stage<T, L, R0, NS, INV, HALF_IN && NS == 1>(v, xb, tw, t);
+  if constexpr (sizeof...(Rest) > 0)
+    stockham_run<T, L, INV, HALF_IN, NS * R0, Rest...>(v, xb, tw, t);
 }
 
-template <typename T, int L, bool INV, int... Rs>
+template <typename T, int L, bool INV, bool HALF_IN, int... Rs>
 __device__ __forceinline__ void stockham_plan(Cplx<T>* v, Cplx<T>* xb, const Cplx<T>* tw, int t,
                                               Radices<Rs...>) {
-  stockham_run<T, L, INV, 1, Rs...>(v, xb, tw, t);
+  stockham_run<T, L, INV, HALF_IN, 1, Rs...>(v, xb, tw, t);
 }
 
 // In-place transform of a line held as v[e] = x[t + T e] by thread t of its team; on return
 // v[e] = X[t + T e] (forward: exp(-), inverse: exp(+), unnormalised). `region` (L elements
-// of LDS) belongs to the team alone; its contents are undefined on return.
-template <typename T, int L, bool INV>
+// of LDS) belongs to the team alone; its contents are undefined on return. HALF_IN: the
+// caller guarantees x[L/2 .. L) = 0 (v[e] = 0 for e >= E/2), which prunes the first stage.
+template <typename T, int L, bool INV, bool HALF_IN = false>
 __device__ __forceinline__ void team_fft(Cplx<T>* v, Cplx<T>* region, const Cplx<T>* tw, int t) {
   static_assert(64 % LinePlan<L>::T == 0, "a team must sit inside one wavefront");
-  stockham_plan<T, L, INV>(v, region, tw, t, typename LinePlan<L>::R{});
+  stockham_plan<T, L, INV, HALF_IN>(v, region, tw, t, typename LinePlan<L>::R{});
 }
 
 }  // namespace rfft

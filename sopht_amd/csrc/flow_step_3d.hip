@@ -9,7 +9,7 @@
 //         (value at the clamped index times the per-axis sine factors) into omega; F = 0.
 //   Poisson (poisson_3d.hip) omega -> psi.
 //   S2  : u = (0.5/dx) curl_h(psi) (interior), 0 on the border, + U_inf everywhere.
-// Each sweep reads its neighbours through L2 with x-fastest lanes (coalesced rows).
+// The sweeps march up z in (x, y) tiles with x-fastest lanes (coalesced rows).
 #include <algorithm>
 #include <cmath>
 #include <new>
@@ -54,52 +54,34 @@ struct Grid {
   }
 };
 
-constexpr int BX = 64, BY = 4;
-
-inline dim3 cell_grid(int64_t nz, int64_t ny, int64_t nx) {
-  return dim3(unsigned((nx + BX - 1) / BX), unsigned((ny + BY - 1) / BY), unsigned(nz));
-}
-
-// curl_h of a vector field f (component planes f0, f1, f2) at an interior cell, unscaled.
-template <typename T>
-__device__ __forceinline__ void curl_at(const T* f0, const T* f1, const T* f2, const Grid& g,
-                                        int64_t z, int64_t y, int64_t x, T c[3]) {
-  const int64_t sy = g.nx, sz = g.nx * g.ny;
-  const int64_t i = g.at(z, y, x);
-  c[0] = f2[i + sy] - f2[i - sy] - f1[i + sz] + f1[i - sz];
-  c[1] = f0[i + sz] - f0[i - sz] - f2[i + 1] + f2[i - 1];
-  c[2] = f1[i + 1] - f1[i - 1] - f0[i + sy] + f0[i - sy];
-}
-
-// z planes per S1 work item (two extra planes of B and three of F are recomputed per item)
+// z planes per S1 work item (two extra planes of B are recomputed per item)
 constexpr int kS1Chunk = 16;
 
-// S1: omega2 = omega1 + pB curl_h(u x omega1), omega1 = omega +
-// pF curl_h(F), with neither omega1 nor B stored in HBM. A workgroup owns a TX x TY tile of
-// (x, y) columns over a chunk of ZC planes and marches up z. Iteration i
-//   (1) stages F plane i+2 (tile plus a 2-cell ring) in a 4-plane LDS ring,
-//   (2) produces omega1 and B on plane i over the tile plus its 1-cell edge ring (B goes to a
-//       3-plane LDS ring; the tile's own omega1, B(i-1), B(i-2) stay in registers),
-//   (3) after the one barrier, consumes plane i-1: its curl_h(B) takes the in-plane
+// S1: omega2 = omega1 + pB curl_h(u x omega1), omega1 = omega + pF curl_h(F), with neither
+// omega1 nor B stored in HBM. A workgroup owns a TX x TY tile of (x, y) columns over a chunk
+// of kS1Chunk planes and marches up z. Iteration i
+//   (1) produces omega1 and B on plane i over the tile plus its 1-cell edge ring (without
+//       corners); curl_h(F) reads F's neighbours through the caches; B goes to a 3-plane LDS
+//       ring, the tile's own omega1(i), B(i), B(i-1) stay in registers;
+//   (2) after the one barrier, consumes plane i-1: its curl_h(B) takes the in-plane
 //       neighbours from LDS and B(i), B(i-2) from registers.
-// Ring depths make one barrier per plane enough: the slot written at iteration i was last
-// read before the previous iteration's barrier.
+// The ring slot written at iteration i was last read before the previous iteration's barrier,
+// so one barrier per plane is enough. The small LDS footprint leaves room for several
+// workgroups per CU, whose loads overlap each other's barriers.
 template <typename T>
 struct S1Tile {
   static constexpr int TX = 64, TY = sizeof(T) == 4 ? 8 : 4, NT = TX * TY;
-  static constexpr int FX = TX + 4, FY = TY + 4, FP = FX * FY;  // F staging box (2-cell ring)
   static constexpr int GX = TX + 2, GY = TY + 2, GP = GX * GY;  // B box (1-cell ring)
   static constexpr int NEDGE = 2 * (TX + TY);                    // edge-ring items (no corners)
 };
 
 template <typename T>
-__global__ void __launch_bounds__(S1Tile<T>::NT) k_s1(Grid g, const T* w, const T* u, const T* F,
+__global__ void __launch_bounds__(S1Tile<T>::NT) __attribute__((amdgpu_waves_per_eu(6))) k_s1(Grid g, const T* w, const T* u, const T* F,
                                                       T* w2out, T pF, T pB, int with_forcing,
                                                       int zchunk, int tiles_x, int tiles_y,
                                                       int xcd_remap) {
   using S = S1Tile<T>;
-  constexpr int TX = S::TX, TY = S::TY, NT = S::NT, FX = S::FX, FP = S::FP, GX = S::GX, GP = S::GP;
-  __shared__ T fs[4][3][FP];
+  constexpr int TX = S::TX, TY = S::TY, GX = S::GX, GP = S::GP;
   __shared__ T bs[3][3][GP];
   // Work item of this workgroup. Workgroups are dealt to the 8 XCDs in turn; with the remap,
   // XCD k takes a contiguous run of items (neighbouring y tiles share halo rows in its L2).
@@ -110,10 +92,11 @@ __global__ void __launch_bounds__(S1Tile<T>::NT) k_s1(Grid g, const T* w, const 
   const int64_t zc_i = item / (int64_t(tiles_x) * tiles_y);
   const int tid = threadIdx.x;
   const int tx = tid % TX, ty = tid / TX;
-  const int64_t x0 = tx_i * TX, y0 = ty_i * TY;
-  const int64_t z0 = zc_i * zchunk;
-  const int64_t z1 = z0 + zchunk < g.nz ? z0 + zchunk : g.nz;
-  const int64_t n = g.nz * g.ny * g.nx, sz = g.ny * g.nx, sy = g.nx;
+  // 32-bit offsets (the host checks 3 nz ny nx < 2^31): one VGPR per address
+  const int x0 = int(tx_i) * TX, y0 = int(ty_i) * TY;
+  const int z0 = int(zc_i) * zchunk;
+  const int z1 = z0 + zchunk < g.nz ? z0 + zchunk : int(g.nz);
+  const int n = int(g.nz * g.ny * g.nx), sz = int(g.ny * g.nx), sy = int(g.nx);
   const bool tv = x0 + tx < g.nx && y0 + ty < g.ny;
   // edge-ring item: x = -1 / TX columns, then y = -1 / TY rows
   int ex = 0, ey = 0;
@@ -126,54 +109,34 @@ __global__ void __launch_bounds__(S1Tile<T>::NT) k_s1(Grid g, const T* w, const 
   }
   const bool ev = tid < S::NEDGE && x0 + ex >= 0 && x0 + ex < g.nx && y0 + ey >= 0 && y0 + ey < g.ny;
 
-  auto stage_f = [&](int64_t q) {  // F plane q -> ring slot q & 3 (zeros outside the grid)
-    T(*dst)[FP] = fs[q & 3];
-    const bool qin = q >= 0 && q < g.nz;
-    for (int k = tid; k < FP; k += NT) {
-      const int64_t x = x0 + (k % FX) - 2, y = y0 + (k / FX) - 2;
-      const bool in = qin && x >= 0 && x < g.nx && y >= 0 && y < g.ny;
-      const int64_t i = in ? q * sz + y * sy + x : 0;
-      dst[0][k] = in ? F[i] : T(0);
-      dst[1][k] = in ? F[n + i] : T(0);
-      dst[2][k] = in ? F[2 * n + i] : T(0);
-    }
-  };
   // omega1 and B = u x omega1 at (x0 + xr, y0 + yr, q)
-  auto produce = [&](int xr, int yr, int64_t q, T b[3], T a[3]) {
-    const int64_t x = x0 + xr, y = y0 + yr;
-    const int64_t i = q * sz + y * sy + x;
-    a[0] = w[i];
-    a[1] = w[n + i];
-    a[2] = w[2 * n + i];
+  auto produce = [&](int xr, int yr, int q, T b[3], T a[3]) {
+    const int x = x0 + xr, y = y0 + yr;
+    const int j = q * sz + y * sy + x;
+    a[0] = w[j];
+    a[1] = w[n + j];
+    a[2] = w[2 * n + j];
+    const T u0 = u[j], u1 = u[n + j], u2 = u[2 * n + j];
     if (with_forcing && g.interior(q, y, x)) {
-      const int c = (yr + 2) * FX + xr + 2;
-      const T(*Fc)[FP] = fs[q & 3];
-      const T(*Fm)[FP] = fs[(q - 1) & 3];
-      const T(*Fp)[FP] = fs[(q + 1) & 3];
-      const T c0 = Fc[2][c + FX] - Fc[2][c - FX] - Fp[1][c] + Fm[1][c];
-      const T c1 = Fp[0][c] - Fm[0][c] - Fc[2][c + 1] + Fc[2][c - 1];
-      const T c2 = Fc[1][c + 1] - Fc[1][c - 1] - Fc[0][c + FX] + Fc[0][c - FX];
+      const T* F0 = F;
+      const T* F1 = F + n;
+      const T* F2 = F + 2 * n;
+      const T c0 = F2[j + sy] - F2[j - sy] - F1[j + sz] + F1[j - sz];
+      const T c1 = F0[j + sz] - F0[j - sz] - F2[j + 1] + F2[j - 1];
+      const T c2 = F1[j + 1] - F1[j - 1] - F0[j + sy] + F0[j - sy];
       a[0] = a[0] + pF * c0;
       a[1] = a[1] + pF * c1;
       a[2] = a[2] + pF * c2;
     }
-    const T u0 = u[i], u1 = u[n + i], u2 = u[2 * n + i];
     // elementwise_ops_3d.py:427-447: B_i = u_j w_k - w_j u_k
     b[0] = u1 * a[2] - a[1] * u2;
     b[1] = u2 * a[0] - a[2] * u0;
     b[2] = u0 * a[1] - a[0] * u1;
   };
 
-  if (with_forcing) {
-    stage_f(z0 - 2);
-    stage_f(z0 - 1);
-    stage_f(z0);
-  }
-  __syncthreads();
   T bm2[3] = {T(0), T(0), T(0)}, bm1[3] = {T(0), T(0), T(0)}, a1prev[3] = {T(0), T(0), T(0)};
   const int gt = (ty + 1) * GX + tx + 1;
-  for (int64_t i = z0 - 1; i <= z1; ++i) {
-    if (with_forcing) stage_f(i + 2);
+  for (int i = z0 - 1; i <= z1; ++i) {
     T b0[3] = {T(0), T(0), T(0)}, a1[3] = {T(0), T(0), T(0)};
     if (i >= 0 && i < g.nz) {
       T(*Bq)[GP] = bs[i % 3];
@@ -193,9 +156,9 @@ __global__ void __launch_bounds__(S1Tile<T>::NT) k_s1(Grid g, const T* w, const 
       }
     }
     __syncthreads();
-    const int64_t r = i - 1;  // consumed plane: B(r-1) = bm2, B(r+1) = b0, omega1(r) = a1prev
+    const int r = i - 1;  // consumed plane: B(r-1) = bm2, B(r+1) = b0, omega1(r) = a1prev
     if (r >= z0 && tv) {
-      const int64_t x = x0 + tx, y = y0 + ty;
+      const int x = x0 + tx, y = y0 + ty;
       T o0 = a1prev[0], o1 = a1prev[1], o2 = a1prev[2];
       if (g.interior(r, y, x)) {
         const T(*Bc)[GP] = bs[r % 3];
@@ -206,7 +169,7 @@ __global__ void __launch_bounds__(S1Tile<T>::NT) k_s1(Grid g, const T* w, const 
         o1 = o1 + pB * c1;
         o2 = o2 + pB * c2;
       }
-      const int64_t j = r * sz + y * sy + x;
+      const int j = r * sz + y * sy + x;
       w2out[j] = o0;
       w2out[n + j] = o1;
       w2out[2 * n + j] = o2;
@@ -225,76 +188,150 @@ struct Pen {
   double f[3][16];
 };
 
-template <typename T>
-__device__ __forceinline__ T diffused(const T* a, const Grid& g, int64_t z, int64_t y, int64_t x,
-                                      T p) {
-  const int64_t i = g.at(z, y, x);
-  if (!g.interior(z, y, x)) return a[i];
-  const int64_t sy = g.nx, sz = g.nx * g.ny;
-  const T flux = p * (a[i + sz] + a[i - sz] + a[i + sy] + a[i - sy] + a[i + 1] + a[i - 1] -
-                      T(6) * a[i]);
-  return a[i] + flux;
+// Column-marching sweeps (SC, S2): a workgroup owns an MX x MY tile of (x, y) columns over
+// kMarchChunk planes, and each thread walks its column up z with the z neighbours in
+// registers (each plane is read once per column instead of three times through L2). Work
+// items are dealt so that each XCD takes a contiguous run of them (y neighbours share rows
+// in that XCD's L2). Offsets are 32-bit (3 nz ny nx < 2^31 is checked at plan creation).
+constexpr int MX = 64, MY = 4, kMarchChunk = 16;
+
+struct MarchTile {
+  int x, y, z0, z1;
+};
+
+__device__ __forceinline__ MarchTile march_tile(const Grid& g, int tiles_x, int tiles_y, int remap) {
+  int64_t item = blockIdx.x;
+  if (remap) item = (item % 8) * (gridDim.x / 8) + item / 8;
+  MarchTile m;
+  m.x = int(item % tiles_x) * MX + int(threadIdx.x % MX);
+  m.y = int((item / tiles_x) % tiles_y) * MY + int(threadIdx.x / MX);
+  m.z0 = int(item / (int64_t(tiles_x) * tiles_y)) * kMarchChunk;
+  m.z1 = m.z0 + kMarchChunk < int(g.nz) ? m.z0 + kMarchChunk : int(g.nz);
+  return m;
 }
 
-__device__ __forceinline__ int64_t clampi(int64_t v, int64_t lo, int64_t hi) {
-  return v < lo ? lo : (v > hi ? hi : v);
+struct MarchLaunch {
+  unsigned blocks;
+  int tiles_x, tiles_y, remap;
+};
+
+inline MarchLaunch march_launch(int64_t nz, int64_t ny, int64_t nx) {
+  MarchLaunch l;
+  l.tiles_x = int(ceil_div(nx, MX));
+  l.tiles_y = int(ceil_div(ny, MY));
+  const int64_t items = int64_t(l.tiles_x) * l.tiles_y * ceil_div(nz, kMarchChunk);
+  l.blocks = unsigned(items);
+  l.remap = items % 8 == 0 ? 1 : 0;
+  return l;
 }
 
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// SC: diffusion of omega2 at the clamped index, penalised with the per-axis sine factors
+// (penalise_field_boundary_3d.py:384-403, x then y then z), into omega; F = 0.
 template <typename T>
-__global__ void __launch_bounds__(BX * BY) k_sc(Grid g, const T* w2, T* w, T* F, T pD, Pen pen,
-                                                int zero_forcing) {
-  const int64_t x = blockIdx.x * int64_t(BX) + threadIdx.x;
-  const int64_t y = blockIdx.y * int64_t(BY) + threadIdx.y;
-  const int64_t z = blockIdx.z;
-  if (x >= g.nx || y >= g.ny) return;
-  const int64_t n = g.nz * g.ny * g.nx;
-  const int64_t i = g.at(z, y, x);
+__global__ void __launch_bounds__(MX * MY) k_sc(Grid g, const T* w2, T* w, T* F, T pD, Pen pen,
+                                                int zero_forcing, int tiles_x, int tiles_y,
+                                                int remap) {
+  const MarchTile m = march_tile(g, tiles_x, tiles_y, remap);
+  const int nz = int(g.nz), ny = int(g.ny), nx = int(g.nx);
+  if (m.x >= nx || m.y >= ny) return;
+  const int n = nz * ny * nx, sz = ny * nx, sy = nx;
   const int W = pen.w;
-  int64_t cz = z, cy = y, cx = x;
-  T sx = T(1), sy = T(1), sz = T(1);
+  int cx = m.x, cy = m.y;
+  T fx = T(1), fy = T(1);
   if (W > 0) {
-    cx = clampi(x, W - 1, g.nx - W);
-    cy = clampi(y, W - 1, g.ny - W);
-    cz = clampi(z, W - 1, g.nz - W);
-    if (x < W) sx = T(pen.f[0][x]);
-    else if (x >= g.nx - W) sx = T(pen.f[0][W + x - (g.nx - W)]);
-    if (y < W) sy = T(pen.f[1][y]);
-    else if (y >= g.ny - W) sy = T(pen.f[1][W + y - (g.ny - W)]);
-    if (z < W) sz = T(pen.f[2][z]);
-    else if (z >= g.nz - W) sz = T(pen.f[2][W + z - (g.nz - W)]);
+    cx = clampi(m.x, W - 1, nx - W);
+    cy = clampi(m.y, W - 1, ny - W);
+    if (m.x < W) fx = T(pen.f[0][m.x]);
+    else if (m.x >= nx - W) fx = T(pen.f[0][W + m.x - (nx - W)]);
+    if (m.y < W) fy = T(pen.f[1][m.y]);
+    else if (m.y >= ny - W) fy = T(pen.f[1][W + m.y - (ny - W)]);
   }
-  const bool pen_cell = (sx != T(1)) || (sy != T(1)) || (sz != T(1)) || cx != x || cy != y || cz != z;
+  const bool inxy = cx > 0 && cy > 0 && cx < nx - 1 && cy < ny - 1;
+  const int col = cy * sy + cx;
+  // window of omega2 at (cx, cy): planes z-1, z, z+1
+  T am[3] = {T(0), T(0), T(0)}, a0[3] = {T(0), T(0), T(0)}, ap[3];
   for (int c = 0; c < 3; ++c) {
-    const T* a = w2 + c * n;
-    T v = diffused<T>(a, g, cz, cy, cx, pD);
-    if (pen_cell) v = ((v * sx) * sy) * sz;  // penalise_field_boundary_3d.py:384-403, x then y then z
-    w[c * n + i] = v;
+    if (m.z0 - 1 >= 0) am[c] = w2[c * n + (m.z0 - 1) * sz + col];
+    a0[c] = w2[c * n + m.z0 * sz + col];
   }
-  if (zero_forcing) {
-    F[i] = T(0);
-    F[n + i] = T(0);
-    F[2 * n + i] = T(0);
+  for (int z = m.z0; z < m.z1; ++z) {
+    for (int c = 0; c < 3; ++c) ap[c] = z + 1 < nz ? w2[c * n + (z + 1) * sz + col] : T(0);
+    int cz = z;
+    T fz = T(1);
+    if (W > 0) {
+      cz = clampi(z, W - 1, nz - W);
+      if (z < W) fz = T(pen.f[2][z]);
+      else if (z >= nz - W) fz = T(pen.f[2][W + z - (nz - W)]);
+    }
+    const bool pen_cell = fx != T(1) || fy != T(1) || fz != T(1) || cx != m.x || cy != m.y || cz != z;
+    const int i = cz * sz + col;
+    const bool inner = inxy && cz > 0 && cz < nz - 1;
+    T v[3];
+    for (int c = 0; c < 3; ++c) {
+      const T* a = w2 + c * n;
+      if (cz == z) {  // the column's own window
+        v[c] = inner ? a0[c] + pD * (ap[c] + am[c] + a[i + sy] + a[i - sy] + a[i + 1] + a[i - 1] -
+                                     T(6) * a0[c])
+                     : a0[c];
+      } else {  // z penalty zone: the diffused value of plane cz
+        v[c] = inner ? a[i] + pD * (a[i + sz] + a[i - sz] + a[i + sy] + a[i - sy] + a[i + 1] +
+                                    a[i - 1] - T(6) * a[i])
+                     : a[i];
+      }
+      if (pen_cell) v[c] = ((v[c] * fx) * fy) * fz;
+    }
+    const int o = z * sz + m.y * sy + m.x;
+    for (int c = 0; c < 3; ++c) w[c * n + o] = v[c];
+    if (zero_forcing)
+      for (int c = 0; c < 3; ++c) F[c * n + o] = T(0);
+    for (int c = 0; c < 3; ++c) {
+      am[c] = a0[c];
+      a0[c] = ap[c];
+    }
   }
 }
 
+// S2: u = p curl_h(psi) (interior), 0 on the border, + U_inf.
 template <typename T>
-__global__ void __launch_bounds__(BX * BY) k_s2(Grid g, const T* psi, T* u, T p, T U0, T U1, T U2) {
-  const int64_t x = blockIdx.x * int64_t(BX) + threadIdx.x;
-  const int64_t y = blockIdx.y * int64_t(BY) + threadIdx.y;
-  const int64_t z = blockIdx.z;
-  if (x >= g.nx || y >= g.ny) return;
-  const int64_t n = g.nz * g.ny * g.nx;
-  const int64_t i = g.at(z, y, x);
-  T c[3] = {T(0), T(0), T(0)};
-  if (g.interior(z, y, x)) {
-    curl_at<T>(psi, psi + n, psi + 2 * n, g, z, y, x, c);
-    c[0] = p * c[0];
-    c[1] = p * c[1];
-    c[2] = p * c[2];
+__global__ void __launch_bounds__(MX * MY) k_s2(Grid g, const T* psi, T* u, T p, T U0, T U1, T U2,
+                                                int tiles_x, int tiles_y, int remap) {
+  const MarchTile m = march_tile(g, tiles_x, tiles_y, remap);
+  const int nz = int(g.nz), ny = int(g.ny), nx = int(g.nx);
+  if (m.x >= nx || m.y >= ny) return;
+  const int n = nz * ny * nx, sz = ny * nx, sy = nx;
+  const T* f0 = psi;
+  const T* f1 = psi + n;
+  const T* f2 = psi + 2 * n;
+  const bool inxy = m.x > 0 && m.y > 0 && m.x < nx - 1 && m.y < ny - 1;
+  const int col = m.y * sy + m.x;
+  // windows of psi0 and psi1 (the components differentiated along z)
+  T p0m = T(0), p1m = T(0), p0c, p1c, p0p, p1p;
+  if (m.z0 - 1 >= 0) {
+    p0m = f0[(m.z0 - 1) * sz + col];
+    p1m = f1[(m.z0 - 1) * sz + col];
   }
-  u[i] = c[0] + U0;
-  u[n + i] = c[1] + U1;
-  u[2 * n + i] = c[2] + U2;
+  p0c = f0[m.z0 * sz + col];
+  p1c = f1[m.z0 * sz + col];
+  for (int z = m.z0; z < m.z1; ++z) {
+    const int i = z * sz + col;
+    p0p = z + 1 < nz ? f0[i + sz] : T(0);
+    p1p = z + 1 < nz ? f1[i + sz] : T(0);
+    T c0 = T(0), c1 = T(0), c2 = T(0);
+    if (inxy && z > 0 && z < nz - 1) {
+      c0 = p * (f2[i + sy] - f2[i - sy] - p1p + p1m);
+      c1 = p * (p0p - p0m - f2[i + 1] + f2[i - 1]);
+      c2 = p * (f1[i + 1] - f1[i - 1] - f0[i + sy] + f0[i - sy]);
+    }
+    u[i] = c0 + U0;
+    u[n + i] = c1 + U1;
+    u[2 * n + i] = c2 + U2;
+    p0m = p0c;
+    p1m = p1c;
+    p0c = p0p;
+    p1c = p1p;
+  }
 }
 
 // max over cells of |u0| + |u1| + |u2| (sum in component order, as numpy's axis-0 sum)
@@ -342,8 +379,6 @@ template <typename T>
 int step_impl(sopht_ns3d* P, void* vort, void* vel, void* psi, void* forcing, void* buffer,
               double dt, const double* U, hipStream_t s) {
   Grid g{P->nz, P->ny, P->nx};
-  const dim3 grid = cell_grid(g.nz, g.ny, g.nx);
-  const dim3 block(BX, BY);
   T* w = static_cast<T*>(vort);
   T* u = static_cast<T*>(vel);
   T* ps = static_cast<T*>(psi);
@@ -369,7 +404,9 @@ int step_impl(sopht_ns3d* P, void* vort, void* vel, void* psi, void* forcing, vo
     SOPHT_CHECK_LAUNCH();
   }
   tm.mark(0, s);
-  k_sc<T><<<grid, block, 0, s>>>(g, ps, w, F, pD, make_pen(P), wf);
+  const MarchLaunch ml = march_launch(g.nz, g.ny, g.nx);
+  k_sc<T><<<ml.blocks, MX * MY, 0, s>>>(g, ps, w, F, pD, make_pen(P), wf, ml.tiles_x, ml.tiles_y,
+                                        ml.remap);
   SOPHT_CHECK_LAUNCH();
   tm.mark(1, s);
   const int64_t n = g.nz * g.ny * g.nx;
@@ -381,7 +418,7 @@ int step_impl(sopht_ns3d* P, void* vort, void* vel, void* psi, void* forcing, vo
   const T U0 = P->with_free_stream ? T(U[0]) : T(0);
   const T U1 = P->with_free_stream ? T(U[1]) : T(0);
   const T U2 = P->with_free_stream ? T(U[2]) : T(0);
-  k_s2<T><<<grid, block, 0, s>>>(g, ps, u, pC, U0, U1, U2);
+  k_s2<T><<<ml.blocks, MX * MY, 0, s>>>(g, ps, u, pC, U0, U1, U2, ml.tiles_x, ml.tiles_y, ml.remap);
   SOPHT_CHECK_LAUNCH();
   tm.mark(7, s);
   tm.end();
@@ -408,8 +445,9 @@ int recover_impl(sopht_ns3d* P, void* vort, void* vel, void* psi, const double* 
   const T U0 = P->with_free_stream ? T(U[0]) : T(0);
   const T U1 = P->with_free_stream ? T(U[1]) : T(0);
   const T U2 = P->with_free_stream ? T(U[2]) : T(0);
-  k_s2<T><<<cell_grid(g.nz, g.ny, g.nx), dim3(BX, BY), 0, s>>>(g, ps, static_cast<T*>(vel),
-                                                                T(0.5 / P->dx), U0, U1, U2);
+  const MarchLaunch ml = march_launch(g.nz, g.ny, g.nx);
+  k_s2<T><<<ml.blocks, MX * MY, 0, s>>>(g, ps, static_cast<T*>(vel), T(0.5 / P->dx), U0, U1, U2,
+                                        ml.tiles_x, ml.tiles_y, ml.remap);
   SOPHT_CHECK_LAUNCH();
   return SOPHT_OK;
 }
@@ -430,6 +468,7 @@ int sopht_ns3d_create(int dtype, int64_t nz, int64_t ny, int64_t nx, double x_ra
   SOPHT_REQUIRE(nz >= 3 && ny >= 3 && nx >= 3, "grid must be at least 3 cells per axis");
   SOPHT_REQUIRE(width >= 0 && width <= 8, "penalty_zone_width must be in [0, 8]");
   SOPHT_REQUIRE(2 * width <= nz && 2 * width <= ny && 2 * width <= nx, "grid too small for penalty width");
+  SOPHT_REQUIRE(3 * nz * ny * nx < (int64_t(1) << 31), "grid too large (3 nz ny nx must be below 2^31)");
   sopht_ns3d* P = new (std::nothrow) sopht_ns3d();
   SOPHT_REQUIRE(P != nullptr, "out of host memory");
   P->dtype = dtype;

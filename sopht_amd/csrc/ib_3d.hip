@@ -24,7 +24,15 @@ struct sopht_spread3d {
   int64_t padded;     // power of two >= capacity
   uint64_t* keys;     // (key << 32 | node), sorted, padded
   uint64_t* prev;     // unsorted keys of the previous call, padded
-  int* flag;          // [0] out-of-domain, [1] keys changed since the last sort
+  int* flag;          // [0] out-of-domain, [1] keys changed since the plan was built,
+                      // [2] fill counter of the plan's contributor list
+  // Gather plan, rebuilt only when the keys change: for support slot t = 64 p + s of sorted
+  // node p that owns its cell, pcount[t] contributors starting at plist[poff[t]], each
+  // (node << 6 | slot) in increasing node order; pcount[t] = 0 for the other slots.
+  int* pcount;
+  int* poff;
+  uint32_t* plist;
+  int64_t plan_nz, plan_ny, plan_nx;  // grid the plan was built for (host copy)
 };
 
 namespace sopht {
@@ -248,8 +256,9 @@ __global__ void k_vbf_time_step(T* dX, const T* dV, int64_t m, T dt) {
 
 // -------------------------------------------------------------------- spreading
 __global__ void k_spread_keys(const int64_t* idx, int64_t n, int64_t padded, int64_t nz,
-                              int64_t ny, int64_t nx, uint64_t* prev, int* flag) {
+                              int64_t ny, int64_t nx, uint64_t* prev, int* flag, int force) {
   const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (force && i == 0) atomicOr(flag + 1, 1);
   if (i >= padded) return;
   uint64_t key = ~uint64_t(0);
   if (i < n) {
@@ -261,7 +270,7 @@ __global__ void k_spread_keys(const int64_t* idx, int64_t n, int64_t padded, int
       atomicOr(flag, 1);
     }
   }
-  // the sorted order is reused while every node keeps its support block
+  // the sorted order and the gather plan are reused while every node keeps its support block
   if (prev[i] != key) {
     atomicOr(flag + 1, 1);
     prev[i] = key;
@@ -269,11 +278,13 @@ __global__ void k_spread_keys(const int64_t* idx, int64_t n, int64_t padded, int
 }
 
 // Stable sort of the packed keys (bitonic on unique 64-bit keys) in one workgroup, only
-// when some node changed its support block since the previous call.
+// when some node changed its support block since the plan was built; clears the plan's fill
+// counter for the rebuild.
 __global__ void __launch_bounds__(1024) k_bitonic_lds(const uint64_t* prev, uint64_t* keys,
                                                       int padded, int* flag) {
   extern __shared__ uint64_t sk[];
   if (atomicAdd(flag + 1, 0) == 0) return;  // uniform for the workgroup
+  if (threadIdx.x == 0) atomicExch(flag + 2, 0);
   for (int i = threadIdx.x; i < padded; i += blockDim.x) sk[i] = prev[i];
   __syncthreads();
   for (int k = 2; k <= padded; k <<= 1) {
@@ -293,14 +304,12 @@ __global__ void __launch_bounds__(1024) k_bitonic_lds(const uint64_t* prev, uint
     }
   }
   for (int i = threadIdx.x; i < padded; i += blockDim.x) keys[i] = sk[i];
-  __syncthreads();
-  if (threadIdx.x == 0) atomicExch(flag + 1, 0);
 }
 
 __global__ void k_copy_keys(const uint64_t* src, uint64_t* dst, int64_t m, int* flag) {
   const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (i < m) dst[i] = src[i];
-  if (i == 0) atomicExch(flag + 1, 0);
+  if (i == 0) atomicExch(flag + 2, 0);
 }
 
 // One compare-exchange step of a global-memory bitonic sort (for large node counts).
@@ -336,13 +345,15 @@ struct EulW3 {
   int64_t sz, sy, sx;
 };
 
-template <typename T, bool LDS_KEYS>
-__global__ void __launch_bounds__(512) k_spread_gather(int ncomp, EulW3<T> F, int64_t nz, int64_t ny,
-                                                       int64_t nx, const T* lag, const T* weights,
-                                                       const uint64_t* gkeys, int64_t n) {
-  // every multiply and add below is rounded separately, exactly as the reference's
-  // `F[block] += F_n * w_n` (no fused multiply-add contraction)
-#pragma clang fp contract(off)
+// Plan build (only when the keys changed): support slot t = 64 p + s of sorted node p owns
+// its cell c when p is c's first contributor in key order. Contributors of c are the nodes
+// whose base cell lies in c - [0, 3]^3: for each (dz, dy) they form one contiguous range of
+// the sorted keys (base x from cx - 3 to cx). The owner lists them in increasing node order.
+template <bool LDS_KEYS>
+__global__ void __launch_bounds__(512) k_spread_plan(int64_t nz, int64_t ny, int64_t nx,
+                                                     const uint64_t* gkeys, int64_t n, int* flag,
+                                                     int* pcount, int* poff, uint32_t* plist) {
+  if (atomicAdd(flag + 1, 0) == 0) return;  // plan still valid (uniform for the launch)
   extern __shared__ uint64_t skeys[];
   const uint64_t* keys = gkeys;
   if constexpr (LDS_KEYS) {
@@ -355,15 +366,18 @@ __global__ void __launch_bounds__(512) k_spread_gather(int ncomp, EulW3<T> F, in
   const int slot = int(gid & 63);
   if (p >= n) return;
   const uint64_t kp = keys[p];
-  if (kp == ~uint64_t(0)) return;  // out-of-domain node (sorted to the end)
+  if (kp == ~uint64_t(0)) {  // out-of-domain node (sorted to the end)
+    pcount[gid] = 0;
+    return;
+  }
   const int64_t cell0 = int64_t(kp >> 32);
   const int64_t bx = cell0 % nx, by = (cell0 / nx) % ny, bz = cell0 / (nx * ny);
   const int kz = slot >> 4, ky = (slot >> 2) & 3, kx = slot & 3;
   const int64_t cz = bz + kz, cy = by + ky, cx = bx + kx;
-  // ranges of contributors: for each (dz, dy) the bases (cz-dz, cy-dy, cx-dx), dx = 3..0
   int64_t lo[16], hi[16];
   bool owner = true;
   bool found_first = false;
+  int m = 0;
 #pragma unroll
   for (int g = 0; g < 16; ++g) {
     const int dz = 3 - (g >> 2), dy = 3 - (g & 3);  // increasing key order
@@ -380,6 +394,7 @@ __global__ void __launch_bounds__(512) k_spread_gather(int ncomp, EulW3<T> F, in
     const int64_t b = lower_bound_cell(keys, n, c1 + 1);
     lo[g] = a;
     hi[g] = b;
+    m += int(b - a);
     if (!found_first && a < b) {
       found_first = true;
       if (a != p) {
@@ -388,13 +403,16 @@ __global__ void __launch_bounds__(512) k_spread_gather(int ncomp, EulW3<T> F, in
       }
     }
   }
-  if (!owner) return;
-  // accumulate contributors in increasing node order
-  T acc[3];
-  const int64_t cidx = cz * F.sz + cy * F.sy + cx * F.sx;
-  for (int c = 0; c < ncomp; ++c) acc[c] = F.p[c][cidx];
+  if (!owner) {
+    pcount[gid] = 0;
+    return;
+  }
+  // the list's place in plist depends on scheduling; its contents do not
+  const int base = atomicAdd(flag + 2, m);
+  pcount[gid] = m;
+  poff[gid] = base;
   int64_t last = -1;
-  while (true) {
+  for (int k = 0; k < m; ++k) {
     int64_t best = INT64_MAX;
     int64_t best_pos = -1;
 #pragma unroll
@@ -406,13 +424,43 @@ __global__ void __launch_bounds__(512) k_spread_gather(int ncomp, EulW3<T> F, in
           best_pos = q;
         }
       }
-    if (best_pos < 0) break;
     const int64_t cell = int64_t(keys[best_pos] >> 32);
     const int64_t qx = cell % nx, qy = (cell / nx) % ny, qz = cell / (nx * ny);
-    const int s = int(((cz - qz) * W4 + (cy - qy)) * W4 + (cx - qx));
-    const T w = weights[int64_t(s) * n + best];
-    for (int c = 0; c < ncomp; ++c) acc[c] = acc[c] + lag[c * n + best] * w;
+    const int sl = int(((cz - qz) * W4 + (cy - qy)) * W4 + (cx - qx));
+    plist[base + k] = (uint32_t(best) << 6) | uint32_t(sl);
     last = best;
+  }
+}
+
+// Plan replay (every call): each owned cell adds its contributors F_lag[node] w[slot][node]
+// in increasing node order, every product and sum rounded separately, exactly as the
+// reference's serial `F[block] += F_n * w_n`. Clears the keys-changed flag for the next call.
+template <typename T>
+__global__ void __launch_bounds__(256) k_spread_replay(int ncomp, EulW3<T> F, int64_t nz,
+                                                       int64_t ny, int64_t nx, const T* lag,
+                                                       const T* weights, const uint64_t* keys,
+                                                       int64_t n, const int* pcount,
+                                                       const int* poff, const uint32_t* plist,
+                                                       int* flag) {
+#pragma clang fp contract(off)
+  const int64_t gid = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (gid == 0) atomicExch(flag + 1, 0);
+  if (gid >= 64 * n) return;
+  const int m = pcount[gid];
+  if (m == 0) return;
+  const int64_t cell0 = int64_t(keys[gid >> 6] >> 32);
+  const int slot = int(gid & 63);
+  const int64_t cx = cell0 % nx + (slot & 3), cy = (cell0 / nx) % ny + ((slot >> 2) & 3),
+                cz = cell0 / (nx * ny) + (slot >> 4);
+  const int64_t cidx = cz * F.sz + cy * F.sy + cx * F.sx;
+  T acc[3];
+  for (int c = 0; c < ncomp; ++c) acc[c] = F.p[c][cidx];
+  const int o = poff[gid];
+  for (int k = 0; k < m; ++k) {
+    const uint32_t e = plist[o + k];
+    const int64_t node = int64_t(e >> 6);
+    const T w = weights[int64_t(e & 63u) * n + node];
+    for (int c = 0; c < ncomp; ++c) acc[c] = acc[c] + lag[c * n + node] * w;
   }
   for (int c = 0; c < ncomp; ++c) F.p[c][cidx] = acc[c];
 }
@@ -484,17 +532,21 @@ int sopht_eul_to_lag_interp_3d(int dtype, int ncomp, void* lag, const void* cons
 
 int sopht_spread3d_create(int64_t max_nodes, sopht_spread3d** out) {
   SOPHT_REQUIRE(out != nullptr && max_nodes >= 0, "invalid spread workspace arguments");
-  SOPHT_REQUIRE(max_nodes < (int64_t(1) << 31), "too many Lagrangian nodes");
+  SOPHT_REQUIRE(max_nodes < (int64_t(1) << 25), "too many Lagrangian nodes (at most 2^25)");
   sopht_spread3d* w = new (std::nothrow) sopht_spread3d();
   SOPHT_REQUIRE(w != nullptr, "out of host memory");
   w->capacity = max_nodes;
   int64_t p = 1;
   while (p < max_nodes) p <<= 1;
   w->padded = p;
-  const int init_flags[2] = {0, 1};  // no out-of-domain node yet; first call sorts
+  const int init_flags[3] = {0, 1, 0};  // no out-of-domain node yet; first call builds
+  const int64_t slots = 64 * (max_nodes > 0 ? max_nodes : 1);
   if (hipMalloc(&w->keys, sizeof(uint64_t) * p) != hipSuccess ||
       hipMalloc(&w->prev, sizeof(uint64_t) * p) != hipSuccess ||
-      hipMalloc(&w->flag, 2 * sizeof(int)) != hipSuccess ||
+      hipMalloc(&w->flag, 3 * sizeof(int)) != hipSuccess ||
+      hipMalloc(&w->pcount, sizeof(int) * slots) != hipSuccess ||
+      hipMalloc(&w->poff, sizeof(int) * slots) != hipSuccess ||
+      hipMalloc(&w->plist, sizeof(uint32_t) * slots) != hipSuccess ||
       hipMemset(w->prev, 0xff, sizeof(uint64_t) * p) != hipSuccess ||
       hipMemcpy(w->flag, init_flags, sizeof(init_flags), hipMemcpyHostToDevice) != hipSuccess) {
     sopht_spread3d_destroy(w);
@@ -510,6 +562,9 @@ int sopht_spread3d_destroy(sopht_spread3d* w) {
   if (w->keys) (void)hipFree(w->keys);
   if (w->prev) (void)hipFree(w->prev);
   if (w->flag) (void)hipFree(w->flag);
+  if (w->pcount) (void)hipFree(w->pcount);
+  if (w->poff) (void)hipFree(w->poff);
+  if (w->plist) (void)hipFree(w->plist);
   delete w;
   return SOPHT_OK;
 }
@@ -528,8 +583,13 @@ int sopht_lag_to_eul_spread_3d(int dtype, int ncomp, sopht_spread3d* ws, void* c
   while (padded < n) padded <<= 1;
   // keys of this call into ws->prev (and the "changed" flag) over the workspace's padded
   // length, so a later call with fewer nodes still sees its stale tail as changed
+  // a plan built for another grid is rebuilt even if the packed keys happen to agree
+  const int force = (ws->plan_nz != nz || ws->plan_ny != ny || ws->plan_nx != nx) ? 1 : 0;
+  ws->plan_nz = nz;
+  ws->plan_ny = ny;
+  ws->plan_nx = nx;
   k_spread_keys<<<blocks_for(ws->padded, 256), 256, 0, s>>>(idx, n, ws->padded, nz, ny, nx,
-                                                            ws->prev, ws->flag);
+                                                            ws->prev, ws->flag, force);
   SOPHT_CHECK_LAUNCH();
   if (padded <= 8192) {
     k_bitonic_lds<<<1, 1024, padded * sizeof(uint64_t), s>>>(ws->prev, ws->keys, int(padded), ws->flag);
@@ -543,20 +603,21 @@ int sopht_lag_to_eul_spread_3d(int dtype, int ncomp, sopht_spread3d* ws, void* c
         SOPHT_CHECK_LAUNCH();
       }
   }
-  // sorted keys in LDS for the gather's binary searches when they fit (128 KiB)
-  const bool lds_keys = n <= 16384;
+  // sorted keys in LDS for the plan build's binary searches when they fit (128 KiB)
+  if (n <= 16384)
+    k_spread_plan<true><<<blocks_for(n * 64, 512), 512, n * sizeof(uint64_t), s>>>(
+        nz, ny, nx, ws->keys, n, ws->flag, ws->pcount, ws->poff, ws->plist);
+  else
+    k_spread_plan<false><<<blocks_for(n * 64, 512), 512, 0, s>>>(nz, ny, nx, ws->keys, n, ws->flag,
+                                                                 ws->pcount, ws->poff, ws->plist);
+  SOPHT_CHECK_LAUNCH();
   IB_DISPATCH(dtype, ([&] {
                 EulW3<T> F;
                 for (int c = 0; c < 3; ++c) F.p[c] = c < ncomp ? static_cast<T*>(eul[c]) : nullptr;
                 F.sz = es[0]; F.sy = es[1]; F.sx = es[2];
-                if (lds_keys)
-                  k_spread_gather<T, true><<<blocks_for(n * 64, 512), 512, n * sizeof(uint64_t), s>>>(
-                      ncomp, F, nz, ny, nx, static_cast<const T*>(lag), static_cast<const T*>(weights),
-                      ws->keys, n);
-                else
-                  k_spread_gather<T, false><<<blocks_for(n * 64, 512), 512, 0, s>>>(
-                      ncomp, F, nz, ny, nx, static_cast<const T*>(lag), static_cast<const T*>(weights),
-                      ws->keys, n);
+                k_spread_replay<T><<<blocks_for(n * 64, 256), 256, 0, s>>>(
+                    ncomp, F, nz, ny, nx, static_cast<const T*>(lag), static_cast<const T*>(weights),
+                    ws->keys, n, ws->pcount, ws->poff, ws->plist, ws->flag);
               }()));
   SOPHT_CHECK_LAUNCH();
   return SOPHT_OK;

@@ -94,6 +94,11 @@ struct SlabGeom {
   __device__ __forceinline__ int64_t h(int c, int64_t z, int64_t ky, int64_t kx) const {
     return ((int64_t(c) * nz + z) * (2 * ny) + ky) * rs + kx;
   }
+  // the same as a 32-bit offset (plan creation checks that the slab has < 2^31 elements):
+  // loads then take a uniform base plus one VGPR of offset
+  __device__ __forceinline__ int hi(int c, int z, int ky, int kx) const {
+    return ((c * int(nz) + z) * (2 * int(ny)) + ky) * int(rs) + kx;
+  }
   __device__ __forceinline__ int64_t row_h(int64_t r) const {  // H row of input row r
     const int c = int(r / (nz * ny));
     const int64_t rem = r % (nz * ny);
@@ -164,7 +169,7 @@ __global__ void __launch_bounds__((RowTile<T, L>::NT)) k1_xr2c(SlabGeom g, CompP
     }
   });
   Cplx<T>* const line = xb + team * RT::S;
-  team_fft<T, L, false>(v, line, tw, t);
+  team_fft<T, L, false, true>(v, line, tw, t);
   static_for<0, E>([&](auto e_) {
     constexpr int e = decltype(e_)::value;
     line[t + TT * e] = v[e];
@@ -201,12 +206,12 @@ __global__ void __launch_bounds__((ColTile<T, L>::NT)) k_ycols(SlabGeom g, Cplx<
   const bool live = kx < ncols;
   // a wavefront whose teams all have kx >= ncols skips its transforms
   const bool wave_live = kx0 + (tid / 64) * (64 / TT) < ncols;
-  Cplx<T>* col = H + g.h(c, z, 0, live ? kx : 0);
-  const int64_t ys = g.rs;  // stride between consecutive y (or ky)
+  const int col = g.hi(c, int(z), 0, live ? int(kx) : 0);
+  const int ys = int(g.rs);  // stride between consecutive y (or ky)
   static_for<0, KIN>([&](auto k_) {
     constexpr int k = decltype(k_)::value;
     const int y = lr + TT * k;
-    xb[lw * S + y] = live ? col[y * ys] : Cplx<T>{T(0), T(0)};
+    xb[lw * S + y] = live ? H[col + y * ys] : Cplx<T>{T(0), T(0)};
   });
   __syncthreads();
   Cplx<T>* const line = xb + team * S;
@@ -217,7 +222,7 @@ __global__ void __launch_bounds__((ColTile<T, L>::NT)) k_ycols(SlabGeom g, Cplx<
     else v[e] = Cplx<T>{T(0), T(0)};
   });
   rfft::wave_sync();
-  if (wave_live) team_fft<T, L, INV>(v, line, tw, t);
+  if (wave_live) team_fft<T, L, INV, !INV>(v, line, tw, t);
   static_for<0, KOUT>([&](auto e_) {
     constexpr int e = decltype(e_)::value;
     line[t + TT * e] = v[e];
@@ -227,9 +232,14 @@ __global__ void __launch_bounds__((ColTile<T, L>::NT)) k_ycols(SlabGeom g, Cplx<
     static_for<0, KOUT>([&](auto k_) {
       constexpr int k = decltype(k_)::value;
       const int y = lr + TT * k;
-      col[y * ys] = xb[lw * S + y];
+      H[col + y * ys] = xb[lw * S + y];
     });
 }
+
+// Waves per SIMD the z pass is compiled for (its VGPR budget: 128 in FP32; a budget of 80,
+// for three 512-thread workgroups per CU, spills).
+template <typename T>
+constexpr int kZconvWaves = sizeof(T) == 4 ? 4 : 2;
 
 // K3: z pencils (c, ky, kx): zero-padded forward FFT, times G^ (real, even), inverse FFT,
 // crop to z < nz. A workgroup owns one kx tile of one |ky| = kyo and runs the pencils of
@@ -237,7 +247,7 @@ __global__ void __launch_bounds__((ColTile<T, L>::NT)) k_ycols(SlabGeom g, Cplx<
 // stays in registers, and the next pencil tile's loads are in flight during the transforms.
 // G^ is stored [kyo][kx][kz], kz fastest, so a team reads its line contiguously.
 template <typename T, int L>
-__global__ void __launch_bounds__((ColTile<T, L>::NT)) k3_zconv(SlabGeom g, Cplx<T>* H, const T* ghat,
+__global__ void __launch_bounds__((ColTile<T, L>::NT)) __attribute__((amdgpu_waves_per_eu(kZconvWaves<T>))) k3_zconv(SlabGeom g, Cplx<T>* H, const T* ghat,
                                                               const Cplx<T>* tw) {
   using CT = ColTile<T, L>;
   constexpr int E = CT::E, TT = CT::TT, WC = CT::WC, S = CT::S, K = E / 2;
@@ -256,25 +266,25 @@ __global__ void __launch_bounds__((ColTile<T, L>::NT)) k3_zconv(SlabGeom g, Cplx
   T gv[E];
   {
     const int64_t kxt = kx0 + team;
-    const T* gq = ghat + (kyo * ncols + (kxt < ncols ? kxt : 0)) * g.gz;
+    const T* gq = ghat + int((kyo * ncols + (kxt < ncols ? kxt : 0)) * g.gz);
     static_for<0, E>([&](auto e_) {
       constexpr int e = decltype(e_)::value;
       const int kz = t + TT * e;
       gv[e] = gq[kz <= L / 2 ? kz : L - kz];
     });
   }
-  const int64_t zs = 2 * g.ny * g.rs;  // stride between consecutive z
-  auto column = [&](int j) -> Cplx<T>* {
+  const int zs = 2 * int(g.ny) * int(g.rs);  // stride between consecutive z
+  auto column = [&](int j) -> int {
     const int c = j % g.ncomp;
-    const int64_t ky = j < g.ncomp ? kyo : 2 * g.ny - kyo;
-    return H + g.h(c, 0, ky, live ? kx : 0);
+    const int ky = j < g.ncomp ? int(kyo) : 2 * int(g.ny) - int(kyo);
+    return g.hi(c, 0, ky, live ? int(kx) : 0);
   };
   Cplx<T> pf[K];
   auto load = [&](int j) {
-    const Cplx<T>* col = column(j);
+    const int col = column(j);
     static_for<0, K>([&](auto k_) {
       constexpr int k = decltype(k_)::value;
-      pf[k] = live ? col[(lr + TT * k) * zs] : Cplx<T>{T(0), T(0)};
+      pf[k] = live ? H[col + (lr + TT * k) * zs] : Cplx<T>{T(0), T(0)};
     });
   };
   Cplx<T>* const line = xb + team * S;
@@ -295,7 +305,7 @@ __global__ void __launch_bounds__((ColTile<T, L>::NT)) k3_zconv(SlabGeom g, Cplx
     rfft::wave_sync();
     if (j + 1 < njobs) load(j + 1);
     if (wave_live) {
-      team_fft<T, L, false>(v, line, tw, t);
+      team_fft<T, L, false, true>(v, line, tw, t);
       static_for<0, E>([&](auto e_) {
         constexpr int e = decltype(e_)::value;
         v[e] = Cplx<T>{v[e].re * gv[e], v[e].im * gv[e]};
@@ -308,11 +318,11 @@ __global__ void __launch_bounds__((ColTile<T, L>::NT)) k3_zconv(SlabGeom g, Cplx
     });
     __syncthreads();
     if (live) {
-      Cplx<T>* col = column(j);
+      const int col = column(j);
       static_for<0, K>([&](auto k_) {
         constexpr int k = decltype(k_)::value;
         const int z = lr + TT * k;
-        col[z * zs] = xb[lw * S + z];
+        H[col + z * zs] = xb[lw * S + z];
       });
     }
     __syncthreads();
@@ -640,6 +650,11 @@ int sopht_poisson3d_create(int dtype, int64_t nz, int64_t ny, int64_t nx, double
   const int64_t rs = kRowPad ? round_up(nx + 1, int64_t(128 / cs)) : nx + 1;
   P->gz = kRowPad ? round_up(nz + 1, int64_t(128 / rsz)) : nz + 1;
   P->slab_bytes = int64_t(3) * nz * 2 * ny * rs * int64_t(cs);
+  if (int64_t(3) * nz * 2 * ny * rs >= (int64_t(1) << 31)) {
+    delete P;
+    set_last_error("Poisson grid too large: the spectral slab must have < 2^31 elements", __FILE__, __LINE__);
+    return SOPHT_ERR_UNSUPPORTED;
+  }
   P->ghat_bytes = (ny + 1) * (nx + 1) * P->gz * int64_t(rsz);
   int rc = SOPHT_OK;
   auto fail = [&](int code) {

@@ -86,10 +86,17 @@ class ThreeDimensionalRigidBodyForcingGrid(ImmersedBodyForcingGrid):
         self.position_field[...] = self.rigid_body.position_collection + self.global_frame_relative_position_field
 
     def compute_lag_grid_velocity_field(self):
-        global_frame_omega = np.dot(self.rigid_body.director_collection[:, :, 0].T, self.rigid_body.omega_collection)
-        self.velocity_field[...] = self.rigid_body.velocity_collection + _batch_cross(
-            global_frame_omega * np.ones(self.num_lag_nodes), self.global_frame_relative_position_field
-        )
+        # V + (omega x r) with omega broadcast over the nodes, written per component with the
+        # operations np.cross performs (a1 b2 - a2 b1, a2 b0 - a0 b2, a0 b1 - a1 b0), so the
+        # result is bit-identical to the reference's np.cross(omega * ones, r) at a fraction of
+        # its cost (it sits between the CFL synchronisation and the next kernel launch).
+        om = np.dot(self.rigid_body.director_collection[:, :, 0].T, self.rigid_body.omega_collection)[:, 0]
+        r = self.global_frame_relative_position_field
+        v = self.velocity_field
+        v0 = self.rigid_body.velocity_collection[:, 0]
+        v[0] = v0[0] + (om[1] * r[2] - om[2] * r[1])
+        v[1] = v0[1] + (om[2] * r[0] - om[0] * r[2])
+        v[2] = v0[2] + (om[0] * r[1] - om[1] * r[0])
 
     def transfer_forcing_from_grid_to_body(self, body_flow_forces, body_flow_torques, lag_grid_forcing_field):
         f = lag_grid_forcing_field

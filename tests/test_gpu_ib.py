@@ -98,6 +98,41 @@ def test_spread_bit_exact_vs_serial_order(dev, real_t, m, cluster, rng):
     np.testing.assert_array_equal(to_host(F2), to_host(F))
 
 
+def test_spread_plan_reuse_and_rebuild(dev, real_t, rng):
+    """One communicator (one gather plan) across calls: the same supports with new forces and
+    weights (plan replayed), moved nodes (plan rebuilt), and a different grid whose packed
+    keys would coincide (plan rebuilt) all stay bit-exact against the serial loop."""
+    n, m = 32, 400
+    dx = real_t(1.0 / n)
+    shift = real_t(dx / 2)
+    c = comm_cls()(dx, shift, m, 2, real_t, n_components=3)
+
+    def check(X, shape, seed):
+        r = np.random.default_rng(seed)
+        idx, d = ib.local_eulerian_grid_support(X, dx, shift, real_t)
+        w, _ = ib.cosine_weights(d, dx, real_t)
+        w = (w * (1 + 0.1 * r.random(w.shape))).astype(real_t)  # weights change between calls too
+        Fl = r.standard_normal((3, m)).astype(real_t)
+        F0 = r.standard_normal((3,) + shape).astype(real_t)
+        ref = F0.copy()
+        ib.spread(ref, Fl, w, idx)
+        F = to_dev(F0, dev)
+        c.lagrangian_to_eulerian_grid_interpolation_kernel(F, to_dev(Fl, dev), to_dev(w, dev), to_dev(idx, dev))
+        c.check_support_in_domain()
+        np.testing.assert_array_equal(to_host(F), ref)
+
+    X1 = random_nodes(rng, m, 0.25, 0.75)
+    check(X1, (n, n, n), 1)
+    check(X1, (n, n, n), 2)  # replay
+    X2 = X1 + 1.7 * dx * (rng.random((3, m)) - 0.5)
+    check(X2, (n, n, n), 3)  # rebuild
+    check(X2, (n, n, n), 4)  # replay
+    # same node coordinates on a grid with more z planes: every packed key agrees with the
+    # previous call's, the cells do not
+    check(X2, (n + 8, n, n), 5)
+    check(X2, (n, n, n), 6)
+
+
 def test_spread_scalar_and_out_of_domain(dev, rng):
     n, m = 16, 5
     real_t = np.float64
