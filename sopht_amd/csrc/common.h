@@ -61,7 +61,7 @@ inline View3<T> make_view(const void* p, const int64_t* s) {
 
 // Complex arithmetic in registers -----------------------------------------------------------
 template <typename T>
-struct Cplx {
+struct alignas(2 * sizeof(T)) Cplx {  // aligned as a pair: one 64/128-bit access, not two
   T re, im;
 };
 

@@ -188,23 +188,55 @@ struct Pen {
   double f[3][16];
 };
 
-// Column-marching sweeps (SC, S2): a workgroup owns an MX x MY tile of (x, y) columns over
-// kMarchChunk planes, and each thread walks its column up z with the z neighbours in
-// registers (each plane is read once per column instead of three times through L2). Work
-// items are dealt so that each XCD takes a contiguous run of them (y neighbours share rows
-// in that XCD's L2). Offsets are 32-bit (3 nz ny nx < 2^31 is checked at plan creation).
-constexpr int MX = 64, MY = 4, kMarchChunk = 16;
+// Column-marching sweeps (SC, S2): a workgroup owns a 64-cell-wide tile of rows over
+// kMarchChunk planes. Each thread holds VEC neighbouring cells of one row (one 16-byte
+// access per row instead of VEC scalar ones) and walks up z with the z neighbours in
+// registers; x neighbours across threads come from the adjacent lane of the row's lane
+// group (tile edges load them). Work items are dealt so that each XCD takes a contiguous run
+// (y neighbours share rows in that XCD's L2). Offsets are 32-bit (3 nz ny nx < 2^31 is
+// checked at plan creation). VEC = 1 covers row lengths that are not a multiple of the
+// vector width and penalty zones wider than a vector.
+constexpr int MX = 64, kMarchThreads = 256, kMarchChunk = 16;
 
-struct MarchTile {
-  int x, y, z0, z1;
+template <typename T, int VEC>
+struct alignas(sizeof(T) * VEC) VecT {
+  T v[VEC];
 };
 
+template <typename T, int VEC>
+__device__ __forceinline__ VecT<T, VEC> vload(const T* __restrict__ p, int off) {
+  return *reinterpret_cast<const VecT<T, VEC>*>(p + off);
+}
+template <typename T, int VEC>
+__device__ __forceinline__ void vstore(T* __restrict__ p, int off, const VecT<T, VEC>& v) {
+  *reinterpret_cast<VecT<T, VEC>*>(p + off) = v;
+}
+template <typename T, int VEC>
+__device__ __forceinline__ VecT<T, VEC> vzero() {
+  VecT<T, VEC> r;
+  for (int k = 0; k < VEC; ++k) r.v[k] = T(0);
+  return r;
+}
+
+template <int VEC>
+struct MarchGeom {
+  static constexpr int MXT = MX / VEC;              // threads per row
+  static constexpr int MYV = kMarchThreads / MXT;   // rows per tile
+};
+
+struct MarchTile {
+  int tx, x0, y, z0, z1;
+};
+
+template <int VEC>
 __device__ __forceinline__ MarchTile march_tile(const Grid& g, int tiles_x, int tiles_y, int remap) {
+  using G = MarchGeom<VEC>;
   int64_t item = blockIdx.x;
   if (remap) item = (item % 8) * (gridDim.x / 8) + item / 8;
   MarchTile m;
-  m.x = int(item % tiles_x) * MX + int(threadIdx.x % MX);
-  m.y = int((item / tiles_x) % tiles_y) * MY + int(threadIdx.x / MX);
+  m.tx = int(threadIdx.x % G::MXT);
+  m.x0 = int(item % tiles_x) * MX + m.tx * VEC;
+  m.y = int((item / tiles_x) % tiles_y) * G::MYV + int(threadIdx.x / G::MXT);
   m.z0 = int(item / (int64_t(tiles_x) * tiles_y)) * kMarchChunk;
   m.z1 = m.z0 + kMarchChunk < int(g.nz) ? m.z0 + kMarchChunk : int(g.nz);
   return m;
@@ -212,52 +244,106 @@ __device__ __forceinline__ MarchTile march_tile(const Grid& g, int tiles_x, int 
 
 struct MarchLaunch {
   unsigned blocks;
-  int tiles_x, tiles_y, remap;
+  int tiles_x, tiles_y, remap, vec;
 };
 
-inline MarchLaunch march_launch(int64_t nz, int64_t ny, int64_t nx) {
+inline MarchLaunch march_launch(int64_t nz, int64_t ny, int64_t nx, int vec) {
   MarchLaunch l;
+  l.vec = vec;
   l.tiles_x = int(ceil_div(nx, MX));
-  l.tiles_y = int(ceil_div(ny, MY));
+  l.tiles_y = int(ceil_div(ny, kMarchThreads / (MX / vec)));
   const int64_t items = int64_t(l.tiles_x) * l.tiles_y * ceil_div(nz, kMarchChunk);
   l.blocks = unsigned(items);
   l.remap = items % 8 == 0 ? 1 : 0;
   return l;
 }
 
+// x neighbours of a row vector spread over the MXT lanes of a row: the element left of
+// v[0] and right of v[VEC-1]. Every lane of the row must call this (lane shuffles).
+template <typename T, int VEC>
+__device__ __forceinline__ void x_nbrs(const VecT<T, VEC>& v, const T* __restrict__ row, int x0,
+                                       int nx, int tx, bool act, T& left, T& right) {
+  constexpr int MXT = MarchGeom<VEC>::MXT;
+  if constexpr (VEC == 1) {
+    // scalar path: a lane's column may be a clamped one (penalty zone), so the adjacent
+    // lane does not necessarily hold the adjacent column
+    left = (act && x0 > 0) ? row[x0 - 1] : T(0);
+    right = (act && x0 + 1 < nx) ? row[x0 + 1] : T(0);
+  } else {
+    left = __shfl_up(v.v[VEC - 1], 1, MXT);
+    right = __shfl_down(v.v[0], 1, MXT);
+    if (tx == 0) left = (act && x0 > 0) ? row[x0 - 1] : T(0);
+    if (tx == MXT - 1) right = (act && x0 + VEC < nx) ? row[x0 + VEC] : T(0);
+  }
+}
+
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 // SC: diffusion of omega2 at the clamped index, penalised with the per-axis sine factors
-// (penalise_field_boundary_3d.py:384-403, x then y then z), into omega; F = 0.
-template <typename T>
-__global__ void __launch_bounds__(MX * MY) k_sc(Grid g, const T* w2, T* w, T* F, T pD, Pen pen,
-                                                int zero_forcing, int tiles_x, int tiles_y,
-                                                int remap) {
-  const MarchTile m = march_tile(g, tiles_x, tiles_y, remap);
+// (penalise_field_boundary_3d.py:384-403, x then y then z), into omega; F = 0. A thread's
+// row is the clamped row cy; the x clamp stays inside the thread's vector (VEC >= width).
+template <typename T, int VEC>
+__global__ void __launch_bounds__(kMarchThreads) k_sc(Grid g, const T* __restrict__ w2,
+                                                      T* __restrict__ w, T* __restrict__ F, T pD,
+                                                      Pen pen, int zero_forcing, int tiles_x,
+                                                      int tiles_y, int remap) {
+  using V = VecT<T, VEC>;
+  const MarchTile m = march_tile<VEC>(g, tiles_x, tiles_y, remap);
   const int nz = int(g.nz), ny = int(g.ny), nx = int(g.nx);
-  if (m.x >= nx || m.y >= ny) return;
   const int n = nz * ny * nx, sz = ny * nx, sy = nx;
+  const bool act = m.x0 < nx && m.y < ny;  // whole vectors (nx % VEC == 0)
   const int W = pen.w;
-  int cx = m.x, cy = m.y;
-  T fx = T(1), fy = T(1);
+  const int y = act ? m.y : 0;
+  const int cy = W > 0 ? clampi(y, W - 1, ny - W) : y;
+  T fy = T(1);
   if (W > 0) {
-    cx = clampi(m.x, W - 1, nx - W);
-    cy = clampi(m.y, W - 1, ny - W);
-    if (m.x < W) fx = T(pen.f[0][m.x]);
-    else if (m.x >= nx - W) fx = T(pen.f[0][W + m.x - (nx - W)]);
-    if (m.y < W) fy = T(pen.f[1][m.y]);
-    else if (m.y >= ny - W) fy = T(pen.f[1][W + m.y - (ny - W)]);
+    if (y < W) fy = T(pen.f[1][y]);
+    else if (y >= ny - W) fy = T(pen.f[1][W + y - (ny - W)]);
   }
-  const bool inxy = cx > 0 && cy > 0 && cx < nx - 1 && cy < ny - 1;
-  const int col = cy * sy + cx;
-  // window of omega2 at (cx, cy): planes z-1, z, z+1
-  T am[3] = {T(0), T(0), T(0)}, a0[3] = {T(0), T(0), T(0)}, ap[3];
+  // VEC == 1: the thread's source cell is the clamped one; VEC > 1: its own vector, with the
+  // clamp resolved inside it
+  const int x0 = act ? m.x0 : 0;
+  const int xs0 = (VEC == 1 && W > 0) ? clampi(x0, W - 1, nx - W) : x0;
+  int src[VEC];
+  T fx[VEC];
+  for (int k = 0; k < VEC; ++k) {
+    const int x = x0 + k;
+    const int cx = W > 0 ? clampi(x, W - 1, nx - W) : x;
+    src[k] = cx - xs0;
+    fx[k] = T(1);
+    if (W > 0) {
+      if (x < W) fx[k] = T(pen.f[0][x]);
+      else if (x >= nx - W) fx[k] = T(pen.f[0][W + x - (nx - W)]);
+    }
+  }
+  const bool yin = cy > 0 && cy < ny - 1;
+  const int row = cy * sy + xs0;
+  // diffused vector of plane q from its three planes (a: q-1, q, q+1 of row cy)
+  auto diffuse = [&](const T* __restrict__ f, int q, const V& am, const V& a0, const V& ap, V& out) {
+    const int i = q * sz + row;
+    const bool qin = act && yin && q > 0 && q < nz - 1;
+    const V yp = qin ? vload<T, VEC>(f, i + sy) : vzero<T, VEC>();
+    const V ym = qin ? vload<T, VEC>(f, i - sy) : vzero<T, VEC>();
+    T left, right;
+    x_nbrs<T, VEC>(a0, f + q * sz + cy * sy, xs0, nx, m.tx, act, left, right);
+    for (int k = 0; k < VEC; ++k) {
+      const int x = xs0 + k;
+      const T xl = k > 0 ? a0.v[k - 1] : left;
+      const T xr = k < VEC - 1 ? a0.v[k + 1] : right;
+      const bool inner = qin && x > 0 && x < nx - 1;
+      out.v[k] = inner ? a0.v[k] + pD * (ap.v[k] + am.v[k] + yp.v[k] + ym.v[k] + xr + xl -
+                                         T(6) * a0.v[k])
+                       : a0.v[k];
+    }
+  };
+  V am[3], a0[3], ap[3];
   for (int c = 0; c < 3; ++c) {
-    if (m.z0 - 1 >= 0) am[c] = w2[c * n + (m.z0 - 1) * sz + col];
-    a0[c] = w2[c * n + m.z0 * sz + col];
+    const T* f = w2 + c * n;
+    am[c] = (act && m.z0 - 1 >= 0) ? vload<T, VEC>(f, (m.z0 - 1) * sz + row) : vzero<T, VEC>();
+    a0[c] = act ? vload<T, VEC>(f, m.z0 * sz + row) : vzero<T, VEC>();
   }
+#pragma unroll 2
   for (int z = m.z0; z < m.z1; ++z) {
-    for (int c = 0; c < 3; ++c) ap[c] = z + 1 < nz ? w2[c * n + (z + 1) * sz + col] : T(0);
     int cz = z;
     T fz = T(1);
     if (W > 0) {
@@ -265,27 +351,35 @@ __global__ void __launch_bounds__(MX * MY) k_sc(Grid g, const T* w2, T* w, T* F,
       if (z < W) fz = T(pen.f[2][z]);
       else if (z >= nz - W) fz = T(pen.f[2][W + z - (nz - W)]);
     }
-    const bool pen_cell = fx != T(1) || fy != T(1) || fz != T(1) || cx != m.x || cy != m.y || cz != z;
-    const int i = cz * sz + col;
-    const bool inner = inxy && cz > 0 && cz < nz - 1;
-    T v[3];
+    V out[3];
     for (int c = 0; c < 3; ++c) {
-      const T* a = w2 + c * n;
+      const T* f = w2 + c * n;
+      ap[c] = (act && z + 1 < nz) ? vload<T, VEC>(f, (z + 1) * sz + row) : vzero<T, VEC>();
       if (cz == z) {  // the column's own window
-        v[c] = inner ? a0[c] + pD * (ap[c] + am[c] + a[i + sy] + a[i - sy] + a[i + 1] + a[i - 1] -
-                                     T(6) * a0[c])
-                     : a0[c];
-      } else {  // z penalty zone: the diffused value of plane cz
-        v[c] = inner ? a[i] + pD * (a[i + sz] + a[i - sz] + a[i + sy] + a[i - sy] + a[i + 1] +
-                                    a[i - 1] - T(6) * a[i])
-                     : a[i];
+        diffuse(f, z, am[c], a0[c], ap[c], out[c]);
+      } else {        // z penalty zone (uniform for the workgroup): the planes around cz
+        const V bm = act ? vload<T, VEC>(f, (cz - 1) * sz + row) : vzero<T, VEC>();
+        const V b0 = act ? vload<T, VEC>(f, cz * sz + row) : vzero<T, VEC>();
+        const V bp = act ? vload<T, VEC>(f, (cz + 1) * sz + row) : vzero<T, VEC>();
+        diffuse(f, cz, bm, b0, bp, out[c]);
       }
-      if (pen_cell) v[c] = ((v[c] * fx) * fy) * fz;
     }
-    const int o = z * sz + m.y * sy + m.x;
-    for (int c = 0; c < 3; ++c) w[c * n + o] = v[c];
-    if (zero_forcing)
-      for (int c = 0; c < 3; ++c) F[c * n + o] = T(0);
+    if (act) {
+      const int o = z * sz + m.y * sy + m.x0;
+      for (int c = 0; c < 3; ++c) {
+        V r;
+        for (int k = 0; k < VEC; ++k) {
+          T v = out[c].v[0];  // out[c].v[src[k]] as selects (a dynamic index would spill)
+          for (int j = 1; j < VEC; ++j) v = src[k] == j ? out[c].v[j] : v;
+          const bool pen_cell = fx[k] != T(1) || fy != T(1) || fz != T(1) || src[k] != k ||
+                                cy != m.y || cz != z || xs0 != x0;
+          if (pen_cell) v = ((v * fx[k]) * fy) * fz;
+          r.v[k] = v;
+        }
+        vstore<T, VEC>(w + c * n, o, r);
+        if (zero_forcing) vstore<T, VEC>(F + c * n, o, vzero<T, VEC>());
+      }
+    }
     for (int c = 0; c < 3; ++c) {
       am[c] = a0[c];
       a0[c] = ap[c];
@@ -294,44 +388,96 @@ __global__ void __launch_bounds__(MX * MY) k_sc(Grid g, const T* w2, T* w, T* F,
 }
 
 // S2: u = p curl_h(psi) (interior), 0 on the border, + U_inf.
-template <typename T>
-__global__ void __launch_bounds__(MX * MY) k_s2(Grid g, const T* psi, T* u, T p, T U0, T U1, T U2,
-                                                int tiles_x, int tiles_y, int remap) {
-  const MarchTile m = march_tile(g, tiles_x, tiles_y, remap);
+template <typename T, int VEC>
+__global__ void __launch_bounds__(kMarchThreads) k_s2(Grid g, const T* __restrict__ psi,
+                                                      T* __restrict__ u, T p, T U0, T U1, T U2,
+                                                      int tiles_x, int tiles_y, int remap) {
+  using V = VecT<T, VEC>;
+  const MarchTile m = march_tile<VEC>(g, tiles_x, tiles_y, remap);
   const int nz = int(g.nz), ny = int(g.ny), nx = int(g.nx);
-  if (m.x >= nx || m.y >= ny) return;
   const int n = nz * ny * nx, sz = ny * nx, sy = nx;
+  const bool act = m.x0 < nx && m.y < ny;
+  const int y = act ? m.y : 0, x0 = act ? m.x0 : 0;
   const T* f0 = psi;
   const T* f1 = psi + n;
   const T* f2 = psi + 2 * n;
-  const bool inxy = m.x > 0 && m.y > 0 && m.x < nx - 1 && m.y < ny - 1;
-  const int col = m.y * sy + m.x;
+  const bool yin = act && y > 0 && y < ny - 1;
+  const int row = y * sy + x0;
   // windows of psi0 and psi1 (the components differentiated along z)
-  T p0m = T(0), p1m = T(0), p0c, p1c, p0p, p1p;
-  if (m.z0 - 1 >= 0) {
-    p0m = f0[(m.z0 - 1) * sz + col];
-    p1m = f1[(m.z0 - 1) * sz + col];
+  V p0m = vzero<T, VEC>(), p1m = vzero<T, VEC>(), p0c = vzero<T, VEC>(), p1c = vzero<T, VEC>();
+  if (act && m.z0 - 1 >= 0) {
+    p0m = vload<T, VEC>(f0, (m.z0 - 1) * sz + row);
+    p1m = vload<T, VEC>(f1, (m.z0 - 1) * sz + row);
   }
-  p0c = f0[m.z0 * sz + col];
-  p1c = f1[m.z0 * sz + col];
+  if (act) {
+    p0c = vload<T, VEC>(f0, m.z0 * sz + row);
+    p1c = vload<T, VEC>(f1, m.z0 * sz + row);
+  }
+  const T U[3] = {U0, U1, U2};
+#pragma unroll 2
   for (int z = m.z0; z < m.z1; ++z) {
-    const int i = z * sz + col;
-    p0p = z + 1 < nz ? f0[i + sz] : T(0);
-    p1p = z + 1 < nz ? f1[i + sz] : T(0);
-    T c0 = T(0), c1 = T(0), c2 = T(0);
-    if (inxy && z > 0 && z < nz - 1) {
-      c0 = p * (f2[i + sy] - f2[i - sy] - p1p + p1m);
-      c1 = p * (p0p - p0m - f2[i + 1] + f2[i - 1]);
-      c2 = p * (f1[i + 1] - f1[i - 1] - f0[i + sy] + f0[i - sy]);
+    const int i = z * sz + row;
+    const bool zin = yin && z > 0 && z < nz - 1;
+    const V p0p = (act && z + 1 < nz) ? vload<T, VEC>(f0, i + sz) : vzero<T, VEC>();
+    const V p1p = (act && z + 1 < nz) ? vload<T, VEC>(f1, i + sz) : vzero<T, VEC>();
+    const V q2c = act ? vload<T, VEC>(f2, i) : vzero<T, VEC>();
+    const V q2p = zin ? vload<T, VEC>(f2, i + sy) : vzero<T, VEC>();
+    const V q2m = zin ? vload<T, VEC>(f2, i - sy) : vzero<T, VEC>();
+    const V q0p = zin ? vload<T, VEC>(f0, i + sy) : vzero<T, VEC>();
+    const V q0m = zin ? vload<T, VEC>(f0, i - sy) : vzero<T, VEC>();
+    T l1, r1, l2, r2;
+    x_nbrs<T, VEC>(p1c, f1 + z * sz + y * sy, x0, nx, m.tx, act, l1, r1);
+    x_nbrs<T, VEC>(q2c, f2 + z * sz + y * sy, x0, nx, m.tx, act, l2, r2);
+    V o[3];
+    for (int k = 0; k < VEC; ++k) {
+      const int x = x0 + k;
+      T c0 = T(0), c1 = T(0), c2 = T(0);
+      if (zin && x > 0 && x < nx - 1) {
+        const T f1r = k < VEC - 1 ? p1c.v[k + 1] : r1, f1l = k > 0 ? p1c.v[k - 1] : l1;
+        const T f2r = k < VEC - 1 ? q2c.v[k + 1] : r2, f2l = k > 0 ? q2c.v[k - 1] : l2;
+        c0 = p * (q2p.v[k] - q2m.v[k] - p1p.v[k] + p1m.v[k]);
+        c1 = p * (p0p.v[k] - p0m.v[k] - f2r + f2l);
+        c2 = p * (f1r - f1l - q0p.v[k] + q0m.v[k]);
+      }
+      o[0].v[k] = c0 + U[0];
+      o[1].v[k] = c1 + U[1];
+      o[2].v[k] = c2 + U[2];
     }
-    u[i] = c0 + U0;
-    u[n + i] = c1 + U1;
-    u[2 * n + i] = c2 + U2;
+    if (act)
+      for (int c = 0; c < 3; ++c) vstore<T, VEC>(u + c * n, i, o[c]);
     p0m = p0c;
     p1m = p1c;
     p0c = p0p;
     p1c = p1p;
   }
+}
+
+// vector width for a row length and penalty width: whole 16-byte vectors per row, and the x
+// clamp of the penalty zone inside one vector
+template <typename T>
+inline int march_vec(int64_t nx, int width) {
+  const int v = 16 / int(sizeof(T));
+  return (nx % v == 0 && width <= v && nx >= 2 * v) ? v : 1;
+}
+
+template <typename T>
+void launch_sc(const MarchLaunch& l, Grid g, const T* w2, T* w, T* F, T pD, const Pen& pen, int wf,
+               hipStream_t s) {
+  if (l.vec == 1)
+    k_sc<T, 1><<<l.blocks, kMarchThreads, 0, s>>>(g, w2, w, F, pD, pen, wf, l.tiles_x, l.tiles_y, l.remap);
+  else
+    k_sc<T, 16 / sizeof(T)><<<l.blocks, kMarchThreads, 0, s>>>(g, w2, w, F, pD, pen, wf, l.tiles_x,
+                                                                l.tiles_y, l.remap);
+}
+
+template <typename T>
+void launch_s2(const MarchLaunch& l, Grid g, const T* psi, T* u, T p, T U0, T U1, T U2,
+               hipStream_t s) {
+  if (l.vec == 1)
+    k_s2<T, 1><<<l.blocks, kMarchThreads, 0, s>>>(g, psi, u, p, U0, U1, U2, l.tiles_x, l.tiles_y, l.remap);
+  else
+    k_s2<T, 16 / sizeof(T)><<<l.blocks, kMarchThreads, 0, s>>>(g, psi, u, p, U0, U1, U2, l.tiles_x,
+                                                                l.tiles_y, l.remap);
 }
 
 // max over cells of |u0| + |u1| + |u2| (sum in component order, as numpy's axis-0 sum)
@@ -404,9 +550,8 @@ int step_impl(sopht_ns3d* P, void* vort, void* vel, void* psi, void* forcing, vo
     SOPHT_CHECK_LAUNCH();
   }
   tm.mark(0, s);
-  const MarchLaunch ml = march_launch(g.nz, g.ny, g.nx);
-  k_sc<T><<<ml.blocks, MX * MY, 0, s>>>(g, ps, w, F, pD, make_pen(P), wf, ml.tiles_x, ml.tiles_y,
-                                        ml.remap);
+  const MarchLaunch ml = march_launch(g.nz, g.ny, g.nx, march_vec<T>(g.nx, P->width));
+  launch_sc<T>(ml, g, ps, w, F, pD, make_pen(P), wf, s);
   SOPHT_CHECK_LAUNCH();
   tm.mark(1, s);
   const int64_t n = g.nz * g.ny * g.nx;
@@ -418,7 +563,7 @@ int step_impl(sopht_ns3d* P, void* vort, void* vel, void* psi, void* forcing, vo
   const T U0 = P->with_free_stream ? T(U[0]) : T(0);
   const T U1 = P->with_free_stream ? T(U[1]) : T(0);
   const T U2 = P->with_free_stream ? T(U[2]) : T(0);
-  k_s2<T><<<ml.blocks, MX * MY, 0, s>>>(g, ps, u, pC, U0, U1, U2, ml.tiles_x, ml.tiles_y, ml.remap);
+  launch_s2<T>(ml, g, ps, u, pC, U0, U1, U2, s);
   SOPHT_CHECK_LAUNCH();
   tm.mark(7, s);
   tm.end();
@@ -445,9 +590,8 @@ int recover_impl(sopht_ns3d* P, void* vort, void* vel, void* psi, const double* 
   const T U0 = P->with_free_stream ? T(U[0]) : T(0);
   const T U1 = P->with_free_stream ? T(U[1]) : T(0);
   const T U2 = P->with_free_stream ? T(U[2]) : T(0);
-  const MarchLaunch ml = march_launch(g.nz, g.ny, g.nx);
-  k_s2<T><<<ml.blocks, MX * MY, 0, s>>>(g, ps, static_cast<T*>(vel), T(0.5 / P->dx), U0, U1, U2,
-                                        ml.tiles_x, ml.tiles_y, ml.remap);
+  const MarchLaunch ml = march_launch(g.nz, g.ny, g.nx, march_vec<T>(g.nx, P->width));
+  launch_s2<T>(ml, g, ps, static_cast<T*>(vel), T(0.5 / P->dx), U0, U1, U2, s);
   SOPHT_CHECK_LAUNCH();
   return SOPHT_OK;
 }
