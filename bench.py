@@ -57,6 +57,20 @@ def stage_words(nz, ny, nx):
     }
 
 
+def hbm_traffic(stage):
+    """HBM bytes per launch of a stage at 256^3 FP32, from the committed rocprofv3 summary
+    (FETCH_SIZE and WRITE_SIZE runs, calibrated with the copy probe); None if absent."""
+    path = ROOT / "profiles" / "flow_step_256_fp32_hbm_traffic.md"
+    try:
+        for line in path.read_text().splitlines():
+            cols = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cols) == 7 and cols[0] == stage:
+                return int(cols[4])
+    except (OSError, ValueError):
+        return None
+    return None
+
+
 def build(n, real_t, seed=42, n_lag=3000, radius=0.08, nu=5e-4, K=8):
     from sopht_amd import simulator as sps
     from sopht_amd.utils import synthetic
@@ -217,13 +231,7 @@ def main():
     dom = max(avg_ms, key=avg_ms.get)
     alg_bytes = words[dom] * cells * word
     achieved = alg_bytes / (avg_ms[dom] * 1e-3) / 1e9
-    traffic = None
-    pmc = ROOT / "profiles" / f"{dom}_traffic.json"
-    if pmc.exists():
-        try:
-            traffic = json.loads(pmc.read_text()).get("hbm_bytes_per_launch")
-        except (ValueError, OSError):
-            traffic = None
+    traffic = hbm_traffic(dom) if (n == 256 and real_t == np.float32) else None
     roof = {
         "bound": "hbm",
         "achieved": round(achieved, 1),
