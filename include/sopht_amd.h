@@ -262,6 +262,45 @@ int sopht_ns3d_enable_timing(sopht_ns3d* plan, int enable);
 int sopht_ns3d_stage_times(sopht_ns3d* plan, double* ms, int n, int64_t* steps, int reset);
 const char* sopht_ns3d_stage_name(int stage);
 
+/* ---- batched step (no counterpart in the reference: many independent simulators, the
+ * reference driver's loop body flow_past_sphere_case.py:191-197 per member, advanced together
+ * with one launch per stage; each member's results are those of stepping it alone) */
+typedef struct sopht_ns3d_batch sopht_ns3d_batch;
+typedef struct sopht_batch_member {
+  int64_t nz, ny, nx;
+  double x_range, nu, rho, cfl;  /* UnboundedNavierStokesFlowSimulator3D arguments */
+  int width, with_forcing, with_free_stream, kernel_type;
+  double free_stream[3];         /* U_inf (x, y, z), used when with_free_stream */
+  double time;                   /* simulator time before the first step */
+  void* vorticity;               /* (3, nz, ny, nx) contiguous real_t device fields */
+  void* velocity;
+  void* psi;
+  void* forcing;
+  /* virtual-boundary body (VirtualBoundaryForcing.py:20-283); n_lag = 0 for none */
+  int64_t n_lag;                 /* at most 8192 */
+  double dx, shift, k, c;        /* grid spacing, coordinate shift, rescaled stiffness, damping */
+  const double* X;               /* (3, n_lag) float64 positions, device */
+  const double* V;               /* (3, n_lag) float64 body velocities, device */
+  int64_t* idx;                  /* VirtualBoundaryForcing buffers, device, real_t unless noted */
+  void* support;
+  void* weights;
+  void* lag_u;
+  void* dV;
+  void* dX;
+  void* Fl;
+} sopht_batch_member;
+
+/* Copies the member descriptions (fields stay owned by the caller and must outlive the batch). */
+int sopht_ns3d_batch_create(int dtype, int count, const sopht_batch_member* members, void* stream,
+                            sopht_ns3d_batch** batch);
+/* `steps` steps of every member: dt from each member's own CFL condition (on the device),
+ * its virtual-boundary time step and forcing, then the flow step. With use_graph the steps
+ * after the first are replays of one captured hipGraph. */
+int sopht_ns3d_batch_step(sopht_ns3d_batch* batch, int steps, int use_graph, void* stream);
+/* Synchronises and reads each member's last dt and its time (either pointer may be NULL). */
+int sopht_ns3d_batch_state(sopht_ns3d_batch* batch, double* dt, double* time, void* stream);
+int sopht_ns3d_batch_destroy(sopht_ns3d_batch* batch);
+
 #ifdef __cplusplus
 }
 #endif

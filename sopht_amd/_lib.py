@@ -32,6 +32,26 @@ _p3 = _vp * 3
 _pi64 = C.POINTER(C.c_int64)
 _pd = C.POINTER(C.c_double)
 
+
+
+class BatchMember(C.Structure):
+    """``sopht_batch_member`` (include/sopht_amd.h)."""
+
+    _fields_ = [
+        ("nz", C.c_int64), ("ny", C.c_int64), ("nx", C.c_int64),
+        ("x_range", C.c_double), ("nu", C.c_double), ("rho", C.c_double), ("cfl", C.c_double),
+        ("width", C.c_int), ("with_forcing", C.c_int), ("with_free_stream", C.c_int), ("kernel_type", C.c_int),
+        ("free_stream", C.c_double * 3),
+        ("time", C.c_double),
+        ("vorticity", C.c_void_p), ("velocity", C.c_void_p), ("psi", C.c_void_p), ("forcing", C.c_void_p),
+        ("n_lag", C.c_int64),
+        ("dx", C.c_double), ("shift", C.c_double), ("k", C.c_double), ("c", C.c_double),
+        ("X", C.c_void_p), ("V", C.c_void_p),
+        ("idx", C.c_void_p), ("support", C.c_void_p), ("weights", C.c_void_p), ("lag_u", C.c_void_p),
+        ("dV", C.c_void_p), ("dX", C.c_void_p), ("Fl", C.c_void_p),
+    ]
+
+
 # name -> argtypes (restype int unless listed in _RESTYPES)
 SIGNATURES = {
     "sopht_last_error": [],
@@ -79,6 +99,10 @@ SIGNATURES = {
     "sopht_ns3d_enable_timing": [_vp, _i],
     "sopht_ns3d_stage_times": [_vp, _pd, _i, _pi64, _i],
     "sopht_ns3d_stage_name": [_i],
+    "sopht_ns3d_batch_create": [_i, _i, C.POINTER(BatchMember), _vp, C.POINTER(_vp)],
+    "sopht_ns3d_batch_step": [_vp, _i, _i, _vp],
+    "sopht_ns3d_batch_state": [_vp, _pd, _pd, _vp],
+    "sopht_ns3d_batch_destroy": [_vp],
 }
 _RESTYPES = {
     "sopht_last_error": C.c_char_p,

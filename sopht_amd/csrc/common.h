@@ -154,4 +154,11 @@ namespace sopht {
 int poisson_vector_solve_timed(sopht_poisson3d* P, void* const sol[3], const int64_t* ss,
                                const void* const rhs[3], const int64_t* rs, hipStream_t s,
                                StageTimer* timer, int first_stage);
+// A plan whose slab holds `comps` components (one Poisson plan per grid size of a batch).
+int poisson_create(int dtype, int64_t nz, int64_t ny, int64_t nx, double x_range, int comps,
+                   void* stream, sopht_poisson3d** out);
+// Solve ncomp components whose fields come from device tables of pointers (sol_tab[c],
+// rhs_tab[c]); all share the element strides ss and rs.
+int poisson_table_solve(sopht_poisson3d* P, int ncomp, void* const* sol_tab, const int64_t* ss,
+                        const void* const* rhs_tab, const int64_t* rs, hipStream_t s);
 }  // namespace sopht

@@ -31,6 +31,7 @@ struct sopht_poisson3d {
   void* ghat;       // real octant
   void* tw[3];      // twiddles for lengths 2nx, 2ny, 2nz (x, y, z)
   int64_t gz;       // G^ line stride (elements): nz + 1 rounded up to a 128-byte line
+  int slab_comps;   // components the slab holds (3, or 3 per instance of a batched class)
   int64_t slab_bytes, ghat_bytes;
 };
 
@@ -106,13 +107,19 @@ struct SlabGeom {
   }
 };
 
+// Component fields of a solve: up to three pointers by value, or (batched solves over many
+// instances of one grid size) a device table of one pointer per component.
 struct CompPtrs {
   const void* p[3];
+  const void* const* tab;
   int64_t sz, sy, sx;
+  __device__ __forceinline__ const void* get(int c) const { return tab ? tab[c] : p[c]; }
 };
 struct CompPtrsW {
   void* p[3];
+  void* const* tab;
   int64_t sz, sy, sx;
+  __device__ __forceinline__ void* get(int c) const { return tab ? tab[c] : p[c]; }
 };
 
 // Row passes (x): one team of T threads per row pair, RowTile::NTEAM teams per workgroup.
@@ -154,7 +161,7 @@ __global__ void __launch_bounds__((RowTile<T, L>::NT)) k1_xr2c(SlabGeom g, CompP
   auto row_ptr = [&](int64_t r) -> const T* {
     const int c = int(r / (g.nz * g.ny));
     const int64_t rem = r % (g.nz * g.ny);
-    return static_cast<const T*>(in.p[c]) + (rem / g.ny) * in.sz + (rem % g.ny) * in.sy;
+    return static_cast<const T*>(in.get(c)) + (rem / g.ny) * in.sz + (rem % g.ny) * in.sy;
   };
   const T* pa = ra < rows ? row_ptr(ra) : nullptr;
   const T* pb = rb < rows ? row_ptr(rb) : nullptr;
@@ -240,6 +247,8 @@ __global__ void __launch_bounds__((ColTile<T, L>::NT)) k_ycols(SlabGeom g, Cplx<
 // for three 512-thread workgroups per CU, spills).
 template <typename T>
 constexpr int kZconvWaves = sizeof(T) == 4 ? 4 : 2;
+// pencil tiles per z-pass workgroup at most (all of them for a 3-component solve)
+constexpr int kZconvJobs = 6;
 
 // K3: z pencils (c, ky, kx): zero-padded forward FFT, times G^ (real, even), inverse FFT,
 // crop to z < nz. A workgroup owns one kx tile of one |ky| = kyo and runs the pencils of
@@ -259,7 +268,11 @@ __global__ void __launch_bounds__((ColTile<T, L>::NT)) __attribute__((amdgpu_wav
   const int64_t ncols = g.nx + 1;
   int64_t kyo, kx0;  // kyo = 0..ny
   if (!col_tile(blockIdx.x, ncols, g.ny + 1, WC, kyo, kx0)) return;
-  const int njobs = (kyo == 0 || kyo == g.ny) ? g.ncomp : 2 * g.ncomp;
+  // jobs (component, ky or 2ny - ky) of this workgroup: chunk blockIdx.y of kZconvJobs
+  const int alljobs = (kyo == 0 || kyo == g.ny) ? g.ncomp : 2 * g.ncomp;
+  const int jbeg = int(blockIdx.y) * kZconvJobs;
+  if (jbeg >= alljobs) return;
+  const int njobs = alljobs - jbeg < kZconvJobs ? alljobs - jbeg : kZconvJobs;
   const int64_t kx = kx0 + lw;
   const bool live = kx < ncols;
   const bool wave_live = kx0 + (tid / 64) * (64 / TT) < ncols;
@@ -274,7 +287,8 @@ __global__ void __launch_bounds__((ColTile<T, L>::NT)) __attribute__((amdgpu_wav
     });
   }
   const int zs = 2 * int(g.ny) * int(g.rs);  // stride between consecutive z
-  auto column = [&](int j) -> int {
+  auto column = [&](int jl) -> int {
+    const int j = jbeg + jl;
     const int c = j % g.ncomp;
     const int ky = j < g.ncomp ? int(kyo) : 2 * int(g.ny) - int(kyo);
     return g.hi(c, 0, ky, live ? int(kx) : 0);
@@ -371,7 +385,7 @@ __global__ void __launch_bounds__((RowTile<T, L>::NT)) k5_xc2r(SlabGeom g, const
   auto row_ptr = [&](int64_t r) -> T* {
     const int c = int(r / (g.nz * g.ny));
     const int64_t rem = r % (g.nz * g.ny);
-    return static_cast<T*>(out.p[c]) + (rem / g.ny) * out.sz + (rem % g.ny) * out.sy;
+    return static_cast<T*>(out.get(c)) + (rem / g.ny) * out.sz + (rem % g.ny) * out.sy;
   };
   T* pa = ra < rows ? row_ptr(ra) : nullptr;
   T* pb = rb < rows ? row_ptr(rb) : nullptr;
@@ -477,7 +491,8 @@ int launch_k3(int64_t L, SlabGeom g, Cplx<T>* H, const T* ghat, const Cplx<T>* t
   if (L == n) {                                                                       \
     using CT = ColTile<T, n>;                                                         \
     const int64_t nb = col_tile_blocks(g.nx + 1, g.ny + 1, CT::WC);                   \
-    k3_zconv<T, n><<<nb, CT::NT, CT::lds, s>>>(g, H, ghat, tw);                       \
+    const unsigned nch = unsigned((2 * g.ncomp + kZconvJobs - 1) / kZconvJobs);        \
+    k3_zconv<T, n><<<dim3(unsigned(nb), nch), CT::NT, CT::lds, s>>>(g, H, ghat, tw);   \
     SOPHT_CHECK_LAUNCH();                                                             \
     return SOPHT_OK;                                                                  \
   }
@@ -509,16 +524,25 @@ SlabGeom slab_geom(const sopht_poisson3d* P, int ncomp) {
                   P->gz, ncomp};
 }
 
+// ncomp <= 3 components by value (sol, rhs), or any number from device pointer tables
+// (sol_tab, rhs_tab; the slab must hold ncomp components).
 template <typename T>
 int solve_impl(sopht_poisson3d* P, int ncomp, void* const sol[3], const int64_t* ss,
                const void* const rhs[3], const int64_t* rs, hipStream_t s,
-               StageTimer* timer = nullptr, int first_stage = 0) {
+               StageTimer* timer = nullptr, int first_stage = 0, void* const* sol_tab = nullptr,
+               const void* const* rhs_tab = nullptr) {
+  if (ncomp > P->slab_comps) {
+    set_last_error("more components than the Poisson plan's slab holds", __FILE__, __LINE__);
+    return SOPHT_ERR_INVALID;
+  }
   const SlabGeom g = slab_geom<T>(P, ncomp);
   CompPtrs in;
   CompPtrsW out;
+  in.tab = rhs_tab;
+  out.tab = sol_tab;
   for (int c = 0; c < 3; ++c) {
-    in.p[c] = c < ncomp ? rhs[c] : nullptr;
-    out.p[c] = c < ncomp ? sol[c] : nullptr;
+    in.p[c] = (!rhs_tab && c < ncomp) ? rhs[c] : nullptr;
+    out.p[c] = (!sol_tab && c < ncomp) ? sol[c] : nullptr;
   }
   in.sz = rs[0]; in.sy = rs[1]; in.sx = rs[2];
   out.sz = ss[0]; out.sy = ss[1]; out.sx = ss[2];
@@ -620,6 +644,13 @@ int poisson_vector_solve_timed(sopht_poisson3d* P, void* const sol[3], const int
                                : solve_impl<double>(P, 3, sol, ss, rhs, rs, s, timer, first_stage);
 }
 
+int poisson_table_solve(sopht_poisson3d* P, int ncomp, void* const* sol_tab, const int64_t* ss,
+                        const void* const* rhs_tab, const int64_t* rs, hipStream_t s) {
+  return P->dtype == SOPHT_F32
+             ? solve_impl<float>(P, ncomp, nullptr, ss, nullptr, rs, s, nullptr, 0, sol_tab, rhs_tab)
+             : solve_impl<double>(P, ncomp, nullptr, ss, nullptr, rs, s, nullptr, 0, sol_tab, rhs_tab);
+}
+
 }  // namespace sopht
 
 using namespace sopht;
@@ -628,7 +659,17 @@ extern "C" {
 
 int sopht_poisson3d_create(int dtype, int64_t nz, int64_t ny, int64_t nx, double x_range,
                            void* stream, sopht_poisson3d** out) {
+  return sopht::poisson_create(dtype, nz, ny, nx, x_range, 3, stream, out);
+}
+
+}  // extern "C"
+
+namespace sopht {
+
+int poisson_create(int dtype, int64_t nz, int64_t ny, int64_t nx, double x_range, int comps,
+                   void* stream, sopht_poisson3d** out) {
   SOPHT_REQUIRE(out != nullptr, "plan pointer is null");
+  SOPHT_REQUIRE(comps >= 1, "a Poisson plan holds at least one component");
   *out = nullptr;
   SOPHT_REQUIRE(dtype == SOPHT_F32 || dtype == SOPHT_F64, "dtype must be SOPHT_F32 or SOPHT_F64");
   SOPHT_REQUIRE(nz > 0 && ny > 0 && nx > 0 && x_range > 0, "grid sizes and x_range must be positive");
@@ -649,8 +690,9 @@ int sopht_poisson3d_create(int dtype, int64_t nz, int64_t ny, int64_t nx, double
   const size_t rsz = dtype == SOPHT_F32 ? sizeof(float) : sizeof(double);
   const int64_t rs = kRowPad ? round_up(nx + 1, int64_t(128 / cs)) : nx + 1;
   P->gz = kRowPad ? round_up(nz + 1, int64_t(128 / rsz)) : nz + 1;
-  P->slab_bytes = int64_t(3) * nz * 2 * ny * rs * int64_t(cs);
-  if (int64_t(3) * nz * 2 * ny * rs >= (int64_t(1) << 31)) {
+  P->slab_comps = comps;
+  P->slab_bytes = int64_t(comps) * nz * 2 * ny * rs * int64_t(cs);
+  if (int64_t(comps) * nz * 2 * ny * rs >= (int64_t(1) << 31)) {
     delete P;
     set_last_error("Poisson grid too large: the spectral slab must have < 2^31 elements", __FILE__, __LINE__);
     return SOPHT_ERR_UNSUPPORTED;
@@ -679,6 +721,10 @@ int sopht_poisson3d_create(int dtype, int64_t nz, int64_t ny, int64_t nx, double
   *out = P;
   return SOPHT_OK;
 }
+
+}  // namespace sopht
+
+extern "C" {
 
 int sopht_poisson3d_destroy(sopht_poisson3d* P) {
   if (!P) return SOPHT_OK;

@@ -1,0 +1,507 @@
+// Device code of the fused flow-step sweeps (S1, SC, S2) and the CFL reduction, shared by
+// the single-simulator step (flow_step_3d.hip) and the batched step (batch_3d.hip). Each sweep
+// is a __device__ body over one work item of one instance plus thin kernels that map the
+// workgroup index to (instance, item).
+#pragma once
+
+#include "common.h"
+
+namespace sopht {
+namespace {
+
+struct Grid {
+  int64_t nz, ny, nx;
+  __device__ __forceinline__ int64_t at(int64_t z, int64_t y, int64_t x) const {
+    return (z * ny + y) * nx + x;
+  }
+  __device__ __forceinline__ bool interior(int64_t z, int64_t y, int64_t x) const {
+    return z > 0 && y > 0 && x > 0 && z < nz - 1 && y < ny - 1 && x < nx - 1;
+  }
+};
+
+// z planes per S1 work item (two extra planes of B are recomputed per item)
+constexpr int kS1Chunk = 16;
+
+// S1: omega2 = omega1 + pB curl_h(u x omega1), omega1 = omega + pF curl_h(F), with neither
+// omega1 nor B stored in HBM. A workgroup owns a TX x TY tile of (x, y) columns over a chunk
+// of kS1Chunk planes and marches up z. Iteration i
+//   (1) produces omega1 and B on plane i over the tile plus its 1-cell edge ring (without
+//       corners); curl_h(F) reads F's neighbours through the caches; B goes to a 3-plane LDS
+//       ring, the tile's own omega1(i), B(i), B(i-1) stay in registers;
+//   (2) after the one barrier, consumes plane i-1: its curl_h(B) takes the in-plane
+//       neighbours from LDS and B(i), B(i-2) from registers.
+// The ring slot written at iteration i was last read before the previous iteration's barrier,
+// so one barrier per plane is enough. The small LDS footprint leaves room for several
+// workgroups per CU, whose loads overlap each other's barriers.
+template <typename T>
+struct S1Tile {
+  static constexpr int TX = 64, TY = sizeof(T) == 4 ? 8 : 4, NT = TX * TY;
+  static constexpr int GX = TX + 2, GY = TY + 2, GP = GX * GY;  // B box (1-cell ring)
+  static constexpr int NEDGE = 2 * (TX + TY);                    // edge-ring items (no corners)
+};
+
+template <typename T>
+__device__ __forceinline__ void s1_body(Grid g, const T* w, const T* u, const T* F, T* w2out, T pF,
+                                        T pB, int with_forcing, int zchunk, int tiles_x,
+                                        int tiles_y, int64_t item) {
+  using S = S1Tile<T>;
+  constexpr int TX = S::TX, TY = S::TY, GX = S::GX, GP = S::GP;
+  __shared__ T bs[3][3][GP];
+  const int64_t tx_i = item % tiles_x;
+  const int64_t ty_i = (item / tiles_x) % tiles_y;
+  const int64_t zc_i = item / (int64_t(tiles_x) * tiles_y);
+  const int tid = threadIdx.x;
+  const int tx = tid % TX, ty = tid / TX;
+  // 32-bit offsets (the host checks 3 nz ny nx < 2^31): one VGPR per address
+  const int x0 = int(tx_i) * TX, y0 = int(ty_i) * TY;
+  const int z0 = int(zc_i) * zchunk;
+  const int z1 = z0 + zchunk < g.nz ? z0 + zchunk : int(g.nz);
+  const int n = int(g.nz * g.ny * g.nx), sz = int(g.ny * g.nx), sy = int(g.nx);
+  const bool tv = x0 + tx < g.nx && y0 + ty < g.ny;
+  // edge-ring item: x = -1 / TX columns, then y = -1 / TY rows
+  int ex = 0, ey = 0;
+  if (tid < 2 * TY) {
+    ex = (tid & 1) ? TX : -1;
+    ey = tid >> 1;
+  } else if (tid < S::NEDGE) {
+    ex = (tid - 2 * TY) >> 1;
+    ey = ((tid - 2 * TY) & 1) ? TY : -1;
+  }
+  const bool ev = tid < S::NEDGE && x0 + ex >= 0 && x0 + ex < g.nx && y0 + ey >= 0 && y0 + ey < g.ny;
+
+  // omega1 and B = u x omega1 at (x0 + xr, y0 + yr, q)
+  auto produce = [&](int xr, int yr, int q, T b[3], T a[3]) {
+    const int x = x0 + xr, y = y0 + yr;
+    const int j = q * sz + y * sy + x;
+    a[0] = w[j];
+    a[1] = w[n + j];
+    a[2] = w[2 * n + j];
+    const T u0 = u[j], u1 = u[n + j], u2 = u[2 * n + j];
+    if (with_forcing && g.interior(q, y, x)) {
+      const T* F0 = F;
+      const T* F1 = F + n;
+      const T* F2 = F + 2 * n;
+      const T c0 = F2[j + sy] - F2[j - sy] - F1[j + sz] + F1[j - sz];
+      const T c1 = F0[j + sz] - F0[j - sz] - F2[j + 1] + F2[j - 1];
+      const T c2 = F1[j + 1] - F1[j - 1] - F0[j + sy] + F0[j - sy];
+      a[0] = a[0] + pF * c0;
+      a[1] = a[1] + pF * c1;
+      a[2] = a[2] + pF * c2;
+    }
+    // elementwise_ops_3d.py:427-447: B_i = u_j w_k - w_j u_k
+    b[0] = u1 * a[2] - a[1] * u2;
+    b[1] = u2 * a[0] - a[2] * u0;
+    b[2] = u0 * a[1] - a[0] * u1;
+  };
+
+  T bm2[3] = {T(0), T(0), T(0)}, bm1[3] = {T(0), T(0), T(0)}, a1prev[3] = {T(0), T(0), T(0)};
+  const int gt = (ty + 1) * GX + tx + 1;
+  for (int i = z0 - 1; i <= z1; ++i) {
+    T b0[3] = {T(0), T(0), T(0)}, a1[3] = {T(0), T(0), T(0)};
+    if (i >= 0 && i < g.nz) {
+      T(*Bq)[GP] = bs[i % 3];
+      if (tv) {
+        produce(tx, ty, i, b0, a1);
+        Bq[0][gt] = b0[0];
+        Bq[1][gt] = b0[1];
+        Bq[2][gt] = b0[2];
+      }
+      if (ev) {
+        T eb[3], ea[3];
+        produce(ex, ey, i, eb, ea);
+        const int ge = (ey + 1) * GX + ex + 1;
+        Bq[0][ge] = eb[0];
+        Bq[1][ge] = eb[1];
+        Bq[2][ge] = eb[2];
+      }
+    }
+    __syncthreads();
+    const int r = i - 1;  // consumed plane: B(r-1) = bm2, B(r+1) = b0, omega1(r) = a1prev
+    if (r >= z0 && tv) {
+      const int x = x0 + tx, y = y0 + ty;
+      T o0 = a1prev[0], o1 = a1prev[1], o2 = a1prev[2];
+      if (g.interior(r, y, x)) {
+        const T(*Bc)[GP] = bs[r % 3];
+        const T c0 = Bc[2][gt + GX] - Bc[2][gt - GX] - b0[1] + bm2[1];
+        const T c1 = b0[0] - bm2[0] - Bc[2][gt + 1] + Bc[2][gt - 1];
+        const T c2 = Bc[1][gt + 1] - Bc[1][gt - 1] - Bc[0][gt + GX] + Bc[0][gt - GX];
+        o0 = o0 + pB * c0;
+        o1 = o1 + pB * c1;
+        o2 = o2 + pB * c2;
+      }
+      const int j = r * sz + y * sy + x;
+      w2out[j] = o0;
+      w2out[n + j] = o1;
+      w2out[2 * n + j] = o2;
+    }
+    for (int c = 0; c < 3; ++c) {
+      bm2[c] = bm1[c];
+      bm1[c] = b0[c];
+      a1prev[c] = a1[c];
+    }
+  }
+}
+
+// Workgroups are dealt to the 8 XCDs in turn; with the remap, XCD k takes a contiguous run of
+// work items (neighbouring y tiles share halo rows in its L2).
+__device__ __forceinline__ int64_t xcd_item(int remap) {
+  int64_t item = blockIdx.x;
+  if (remap) item = (item % 8) * (gridDim.x / 8) + item / 8;
+  return item;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(S1Tile<T>::NT) __attribute__((amdgpu_waves_per_eu(6)))
+k_s1(Grid g, const T* w, const T* u, const T* F, T* w2out, T pF, T pB, int with_forcing,
+     int zchunk, int tiles_x, int tiles_y, int remap) {
+  s1_body<T>(g, w, u, F, w2out, pF, pB, with_forcing, zchunk, tiles_x, tiles_y, xcd_item(remap));
+}
+
+struct Pen {
+  int w;
+  // per-axis factors: index 0..w-1 front, then w back entries for n-w..n-1
+  double f[3][16];
+};
+
+// Column-marching sweeps (SC, S2): a workgroup owns a 64-cell-wide tile of rows over
+// kMarchChunk planes. Each thread holds VEC neighbouring cells of one row (one 16-byte
+// access per row instead of VEC scalar ones) and walks up z with the z neighbours in
+// registers; x neighbours across threads come from the adjacent lane of the row's lane
+// group (tile edges load them). Work items are dealt so that each XCD takes a contiguous run
+// (y neighbours share rows in that XCD's L2). Offsets are 32-bit (3 nz ny nx < 2^31 is
+// checked at plan creation). VEC = 1 covers row lengths that are not a multiple of the
+// vector width and penalty zones wider than a vector.
+constexpr int MX = 64, kMarchThreads = 256, kMarchChunk = 16;
+
+template <typename T, int VEC>
+struct alignas(sizeof(T) * VEC) VecT {
+  T v[VEC];
+};
+
+template <typename T, int VEC>
+__device__ __forceinline__ VecT<T, VEC> vload(const T* __restrict__ p, int off) {
+  return *reinterpret_cast<const VecT<T, VEC>*>(p + off);
+}
+template <typename T, int VEC>
+__device__ __forceinline__ void vstore(T* __restrict__ p, int off, const VecT<T, VEC>& v) {
+  *reinterpret_cast<VecT<T, VEC>*>(p + off) = v;
+}
+template <typename T, int VEC>
+__device__ __forceinline__ VecT<T, VEC> vzero() {
+  VecT<T, VEC> r;
+  for (int k = 0; k < VEC; ++k) r.v[k] = T(0);
+  return r;
+}
+
+template <int VEC>
+struct MarchGeom {
+  static constexpr int MXT = MX / VEC;              // threads per row
+  static constexpr int MYV = kMarchThreads / MXT;   // rows per tile
+};
+
+struct MarchTile {
+  int tx, x0, y, z0, z1;
+};
+
+template <int VEC>
+__device__ __forceinline__ MarchTile march_tile(const Grid& g, int tiles_x, int tiles_y,
+                                                int64_t item) {
+  using G = MarchGeom<VEC>;
+  MarchTile m;
+  m.tx = int(threadIdx.x % G::MXT);
+  m.x0 = int(item % tiles_x) * MX + m.tx * VEC;
+  m.y = int((item / tiles_x) % tiles_y) * G::MYV + int(threadIdx.x / G::MXT);
+  m.z0 = int(item / (int64_t(tiles_x) * tiles_y)) * kMarchChunk;
+  m.z1 = m.z0 + kMarchChunk < int(g.nz) ? m.z0 + kMarchChunk : int(g.nz);
+  return m;
+}
+
+struct MarchLaunch {
+  unsigned blocks;
+  int tiles_x, tiles_y, remap, vec;
+};
+
+inline MarchLaunch march_launch(int64_t nz, int64_t ny, int64_t nx, int vec) {
+  MarchLaunch l;
+  l.vec = vec;
+  l.tiles_x = int(ceil_div(nx, MX));
+  l.tiles_y = int(ceil_div(ny, kMarchThreads / (MX / vec)));
+  const int64_t items = int64_t(l.tiles_x) * l.tiles_y * ceil_div(nz, kMarchChunk);
+  l.blocks = unsigned(items);
+  l.remap = items % 8 == 0 ? 1 : 0;
+  return l;
+}
+
+// x neighbours of a row vector spread over the MXT lanes of a row: the element left of
+// v[0] and right of v[VEC-1]. Every lane of the row must call this (lane shuffles).
+template <typename T, int VEC>
+__device__ __forceinline__ void x_nbrs(const VecT<T, VEC>& v, const T* __restrict__ row, int x0,
+                                       int nx, int tx, bool act, T& left, T& right) {
+  constexpr int MXT = MarchGeom<VEC>::MXT;
+  if constexpr (VEC == 1) {
+    // scalar path: a lane's column may be a clamped one (penalty zone), so the adjacent
+    // lane does not necessarily hold the adjacent column
+    left = (act && x0 > 0) ? row[x0 - 1] : T(0);
+    right = (act && x0 + 1 < nx) ? row[x0 + 1] : T(0);
+  } else {
+    left = __shfl_up(v.v[VEC - 1], 1, MXT);
+    right = __shfl_down(v.v[0], 1, MXT);
+    if (tx == 0) left = (act && x0 > 0) ? row[x0 - 1] : T(0);
+    if (tx == MXT - 1) right = (act && x0 + VEC < nx) ? row[x0 + VEC] : T(0);
+  }
+}
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// SC: diffusion of omega2 at the clamped index, penalised with the per-axis sine factors
+// (penalise_field_boundary_3d.py:384-403, x then y then z), into omega; F = 0. A thread's
+// row is the clamped row cy; the x clamp stays inside the thread's vector (VEC >= width).
+template <typename T, int VEC>
+__device__ __forceinline__ void sc_body(Grid g, const T* __restrict__ w2, T* __restrict__ w,
+                                        T* __restrict__ F, T pD, const Pen& pen,
+                                        int zero_forcing, int tiles_x, int tiles_y,
+                                        int64_t item) {
+  using V = VecT<T, VEC>;
+  const MarchTile m = march_tile<VEC>(g, tiles_x, tiles_y, item);
+  const int nz = int(g.nz), ny = int(g.ny), nx = int(g.nx);
+  const int n = nz * ny * nx, sz = ny * nx, sy = nx;
+  const bool act = m.x0 < nx && m.y < ny;  // whole vectors (nx % VEC == 0)
+  const int W = pen.w;
+  const int y = act ? m.y : 0;
+  const int cy = W > 0 ? clampi(y, W - 1, ny - W) : y;
+  T fy = T(1);
+  if (W > 0) {
+    if (y < W) fy = T(pen.f[1][y]);
+    else if (y >= ny - W) fy = T(pen.f[1][W + y - (ny - W)]);
+  }
+  // VEC == 1: the thread's source cell is the clamped one; VEC > 1: its own vector, with the
+  // clamp resolved inside it
+  const int x0 = act ? m.x0 : 0;
+  const int xs0 = (VEC == 1 && W > 0) ? clampi(x0, W - 1, nx - W) : x0;
+  int src[VEC];
+  T fx[VEC];
+  for (int k = 0; k < VEC; ++k) {
+    const int x = x0 + k;
+    const int cx = W > 0 ? clampi(x, W - 1, nx - W) : x;
+    src[k] = cx - xs0;
+    fx[k] = T(1);
+    if (W > 0) {
+      if (x < W) fx[k] = T(pen.f[0][x]);
+      else if (x >= nx - W) fx[k] = T(pen.f[0][W + x - (nx - W)]);
+    }
+  }
+  const bool yin = cy > 0 && cy < ny - 1;
+  const int row = cy * sy + xs0;
+  // diffused vector of plane q from its three planes (a: q-1, q, q+1 of row cy)
+  auto diffuse = [&](const T* __restrict__ f, int q, const V& am, const V& a0, const V& ap, V& out) {
+    const int i = q * sz + row;
+    const bool qin = act && yin && q > 0 && q < nz - 1;
+    const V yp = qin ? vload<T, VEC>(f, i + sy) : vzero<T, VEC>();
+    const V ym = qin ? vload<T, VEC>(f, i - sy) : vzero<T, VEC>();
+    T left, right;
+    x_nbrs<T, VEC>(a0, f + q * sz + cy * sy, xs0, nx, m.tx, act, left, right);
+    for (int k = 0; k < VEC; ++k) {
+      const int x = xs0 + k;
+      const T xl = k > 0 ? a0.v[k - 1] : left;
+      const T xr = k < VEC - 1 ? a0.v[k + 1] : right;
+      const bool inner = qin && x > 0 && x < nx - 1;
+      out.v[k] = inner ? a0.v[k] + pD * (ap.v[k] + am.v[k] + yp.v[k] + ym.v[k] + xr + xl -
+                                         T(6) * a0.v[k])
+                       : a0.v[k];
+    }
+  };
+  V am[3], a0[3], ap[3];
+  for (int c = 0; c < 3; ++c) {
+    const T* f = w2 + c * n;
+    am[c] = (act && m.z0 - 1 >= 0) ? vload<T, VEC>(f, (m.z0 - 1) * sz + row) : vzero<T, VEC>();
+    a0[c] = act ? vload<T, VEC>(f, m.z0 * sz + row) : vzero<T, VEC>();
+  }
+#pragma unroll 2
+  for (int z = m.z0; z < m.z1; ++z) {
+    int cz = z;
+    T fz = T(1);
+    if (W > 0) {
+      cz = clampi(z, W - 1, nz - W);
+      if (z < W) fz = T(pen.f[2][z]);
+      else if (z >= nz - W) fz = T(pen.f[2][W + z - (nz - W)]);
+    }
+    V out[3];
+    for (int c = 0; c < 3; ++c) {
+      const T* f = w2 + c * n;
+      ap[c] = (act && z + 1 < nz) ? vload<T, VEC>(f, (z + 1) * sz + row) : vzero<T, VEC>();
+      if (cz == z) {  // the column's own window
+        diffuse(f, z, am[c], a0[c], ap[c], out[c]);
+      } else {        // z penalty zone (uniform for the workgroup): the planes around cz
+        const V bm = act ? vload<T, VEC>(f, (cz - 1) * sz + row) : vzero<T, VEC>();
+        const V b0 = act ? vload<T, VEC>(f, cz * sz + row) : vzero<T, VEC>();
+        const V bp = act ? vload<T, VEC>(f, (cz + 1) * sz + row) : vzero<T, VEC>();
+        diffuse(f, cz, bm, b0, bp, out[c]);
+      }
+    }
+    if (act) {
+      const int o = z * sz + m.y * sy + m.x0;
+      for (int c = 0; c < 3; ++c) {
+        V r;
+        for (int k = 0; k < VEC; ++k) {
+          T v = out[c].v[0];  // out[c].v[src[k]] as selects (a dynamic index would spill)
+          for (int j = 1; j < VEC; ++j) v = src[k] == j ? out[c].v[j] : v;
+          const bool pen_cell = fx[k] != T(1) || fy != T(1) || fz != T(1) || src[k] != k ||
+                                cy != m.y || cz != z || xs0 != x0;
+          if (pen_cell) v = ((v * fx[k]) * fy) * fz;
+          r.v[k] = v;
+        }
+        vstore<T, VEC>(w + c * n, o, r);
+        if (zero_forcing) vstore<T, VEC>(F + c * n, o, vzero<T, VEC>());
+      }
+    }
+    for (int c = 0; c < 3; ++c) {
+      am[c] = a0[c];
+      a0[c] = ap[c];
+    }
+  }
+}
+
+template <typename T, int VEC>
+__global__ void __launch_bounds__(kMarchThreads) k_sc(Grid g, const T* __restrict__ w2,
+                                                      T* __restrict__ w, T* __restrict__ F, T pD,
+                                                      Pen pen, int zero_forcing, int tiles_x,
+                                                      int tiles_y, int remap) {
+  sc_body<T, VEC>(g, w2, w, F, pD, pen, zero_forcing, tiles_x, tiles_y, xcd_item(remap));
+}
+
+// S2: u = p curl_h(psi) (interior), 0 on the border, + U_inf.
+template <typename T, int VEC>
+__device__ __forceinline__ void s2_body(Grid g, const T* __restrict__ psi, T* __restrict__ u, T p,
+                                        T U0, T U1, T U2, int tiles_x, int tiles_y,
+                                        int64_t item) {
+  using V = VecT<T, VEC>;
+  const MarchTile m = march_tile<VEC>(g, tiles_x, tiles_y, item);
+  const int nz = int(g.nz), ny = int(g.ny), nx = int(g.nx);
+  const int n = nz * ny * nx, sz = ny * nx, sy = nx;
+  const bool act = m.x0 < nx && m.y < ny;
+  const int y = act ? m.y : 0, x0 = act ? m.x0 : 0;
+  const T* f0 = psi;
+  const T* f1 = psi + n;
+  const T* f2 = psi + 2 * n;
+  const bool yin = act && y > 0 && y < ny - 1;
+  const int row = y * sy + x0;
+  // windows of psi0 and psi1 (the components differentiated along z)
+  V p0m = vzero<T, VEC>(), p1m = vzero<T, VEC>(), p0c = vzero<T, VEC>(), p1c = vzero<T, VEC>();
+  if (act && m.z0 - 1 >= 0) {
+    p0m = vload<T, VEC>(f0, (m.z0 - 1) * sz + row);
+    p1m = vload<T, VEC>(f1, (m.z0 - 1) * sz + row);
+  }
+  if (act) {
+    p0c = vload<T, VEC>(f0, m.z0 * sz + row);
+    p1c = vload<T, VEC>(f1, m.z0 * sz + row);
+  }
+  const T U[3] = {U0, U1, U2};
+#pragma unroll 2
+  for (int z = m.z0; z < m.z1; ++z) {
+    const int i = z * sz + row;
+    const bool zin = yin && z > 0 && z < nz - 1;
+    const V p0p = (act && z + 1 < nz) ? vload<T, VEC>(f0, i + sz) : vzero<T, VEC>();
+    const V p1p = (act && z + 1 < nz) ? vload<T, VEC>(f1, i + sz) : vzero<T, VEC>();
+    const V q2c = act ? vload<T, VEC>(f2, i) : vzero<T, VEC>();
+    const V q2p = zin ? vload<T, VEC>(f2, i + sy) : vzero<T, VEC>();
+    const V q2m = zin ? vload<T, VEC>(f2, i - sy) : vzero<T, VEC>();
+    const V q0p = zin ? vload<T, VEC>(f0, i + sy) : vzero<T, VEC>();
+    const V q0m = zin ? vload<T, VEC>(f0, i - sy) : vzero<T, VEC>();
+    T l1, r1, l2, r2;
+    x_nbrs<T, VEC>(p1c, f1 + z * sz + y * sy, x0, nx, m.tx, act, l1, r1);
+    x_nbrs<T, VEC>(q2c, f2 + z * sz + y * sy, x0, nx, m.tx, act, l2, r2);
+    V o[3];
+    for (int k = 0; k < VEC; ++k) {
+      const int x = x0 + k;
+      T c0 = T(0), c1 = T(0), c2 = T(0);
+      if (zin && x > 0 && x < nx - 1) {
+        const T f1r = k < VEC - 1 ? p1c.v[k + 1] : r1, f1l = k > 0 ? p1c.v[k - 1] : l1;
+        const T f2r = k < VEC - 1 ? q2c.v[k + 1] : r2, f2l = k > 0 ? q2c.v[k - 1] : l2;
+        c0 = p * (q2p.v[k] - q2m.v[k] - p1p.v[k] + p1m.v[k]);
+        c1 = p * (p0p.v[k] - p0m.v[k] - f2r + f2l);
+        c2 = p * (f1r - f1l - q0p.v[k] + q0m.v[k]);
+      }
+      o[0].v[k] = c0 + U[0];
+      o[1].v[k] = c1 + U[1];
+      o[2].v[k] = c2 + U[2];
+    }
+    if (act)
+      for (int c = 0; c < 3; ++c) vstore<T, VEC>(u + c * n, i, o[c]);
+    p0m = p0c;
+    p1m = p1c;
+    p0c = p0p;
+    p1c = p1p;
+  }
+}
+
+template <typename T, int VEC>
+__global__ void __launch_bounds__(kMarchThreads) k_s2(Grid g, const T* __restrict__ psi,
+                                                      T* __restrict__ u, T p, T U0, T U1, T U2,
+                                                      int tiles_x, int tiles_y, int remap) {
+  s2_body<T, VEC>(g, psi, u, p, U0, U1, U2, tiles_x, tiles_y, xcd_item(remap));
+}
+
+// vector width for a row length and penalty width: whole 16-byte vectors per row, and the x
+// clamp of the penalty zone inside one vector
+template <typename T>
+inline int march_vec(int64_t nx, int width) {
+  const int v = 16 / int(sizeof(T));
+  return (nx % v == 0 && width <= v && nx >= 2 * v) ? v : 1;
+}
+
+template <typename T>
+void launch_sc(const MarchLaunch& l, Grid g, const T* w2, T* w, T* F, T pD, const Pen& pen, int wf,
+               hipStream_t s) {
+  if (l.vec == 1)
+    k_sc<T, 1><<<l.blocks, kMarchThreads, 0, s>>>(g, w2, w, F, pD, pen, wf, l.tiles_x, l.tiles_y, l.remap);
+  else
+    k_sc<T, 16 / sizeof(T)><<<l.blocks, kMarchThreads, 0, s>>>(g, w2, w, F, pD, pen, wf, l.tiles_x,
+                                                                l.tiles_y, l.remap);
+}
+
+template <typename T>
+void launch_s2(const MarchLaunch& l, Grid g, const T* psi, T* u, T p, T U0, T U1, T U2,
+               hipStream_t s) {
+  if (l.vec == 1)
+    k_s2<T, 1><<<l.blocks, kMarchThreads, 0, s>>>(g, psi, u, p, U0, U1, U2, l.tiles_x, l.tiles_y, l.remap);
+  else
+    k_s2<T, 16 / sizeof(T)><<<l.blocks, kMarchThreads, 0, s>>>(g, psi, u, p, U0, U1, U2, l.tiles_x,
+                                                                l.tiles_y, l.remap);
+}
+
+// max over cells of |u0| + |u1| + |u2| (sum in component order, as numpy's axis-0 sum)
+template <typename T>
+__global__ void __launch_bounds__(256) k_max_abs_sum(const T* u, int64_t n, T* partial) {
+  __shared__ T red[256];
+  T m = T(0);
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const T s = (fabs(u[i]) + fabs(u[n + i])) + fabs(u[2 * n + i]);
+    m = s > m ? s : m;
+  }
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) red[threadIdx.x] = red[threadIdx.x + k] > red[threadIdx.x] ? red[threadIdx.x + k] : red[threadIdx.x];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_max_final(const T* partial, int nb, T* out) {
+  __shared__ T red[256];
+  T m = T(0);
+  for (int i = threadIdx.x; i < nb; i += 256) m = partial[i] > m ? partial[i] : m;
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) red[threadIdx.x] = red[threadIdx.x + k] > red[threadIdx.x] ? red[threadIdx.x + k] : red[threadIdx.x];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
+
+}  // namespace
+}  // namespace sopht

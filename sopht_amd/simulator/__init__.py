@@ -1,5 +1,6 @@
 """Simulator layer: the 3D flow simulator and the immersed-body glue."""
 
+from .batch import BatchedFlowSimulation
 from .flow import (
     FlowSimulator,
     UnboundedNavierStokesFlowSimulator3D,
@@ -18,6 +19,7 @@ from .immersed_body import (
 )
 
 __all__ = [
+    "BatchedFlowSimulation",
     "FibonacciEllipsoidForcingGrid",
     "FibonacciSphereForcingGrid",
     "FlowSimulator",
