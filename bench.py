@@ -135,6 +135,89 @@ def cpu_baseline(n, real_t, steps, n_lag=3000):
     }
 
 
+# the batched workload: 64 uneven members (sizes and counts)
+BATCH_SIZES = [(48, 16), (64, 24), (96, 16), (128, 8)]
+
+
+def batched_members(seed=7, n_lag=1000, nu=1e-3, K=6):
+    """64 independent FP32 members: per-member seed (random-phase vorticity |k| <= 6,
+    amplitude ~ k^-2, max|omega| = 10), a fixed sphere of radius ~ U[0.05, 0.12] at the
+    centre with 1000 Fibonacci points, free stream (U, 0, 0) with U ~ U[0.5, 1.5]."""
+    import torch
+
+    from sopht_amd import simulator as sps
+    from sopht_amd.utils import synthetic
+
+    rng = np.random.default_rng(seed)
+    members = []
+    k = 0
+    for n, count in BATCH_SIZES:
+        for _ in range(count):
+            flow = sps.UnboundedNavierStokesFlowSimulator3D(
+                grid_size=(n, n, n), x_range=1.0, kinematic_viscosity=nu, cfl=0.1, real_t=np.float32,
+                with_forcing=True, with_free_stream_flow=True,
+            )
+            w0 = synthetic.random_phase_vorticity((n, n, n), 1.0, K, 1000 + k, 10.0)
+            flow.vorticity_field.copy_(torch.from_numpy(w0.astype(np.float32)))
+            U = np.array([rng.uniform(0.5, 1.5), 0.0, 0.0])
+            flow.recover_velocity(U)
+            body = sps.RigidBody((0.5, 0.5, 0.5), rng.uniform(0.05, 0.12))
+            inter = sps.RigidBodyFlowInteraction(
+                body, flow.eul_grid_forcing_field, flow.velocity_field, -5e4, -10.0, flow.dx, 3,
+                sps.FibonacciSphereForcingGrid, real_t=np.float32, num_lag_nodes=n_lag,
+            )
+            members.append((flow, inter, U))
+            k += 1
+    return members
+
+
+def batched_vs_sequential(steps, seq_steps, warmup=2):
+    """Cell-updates/s of the 64 members advanced by the batched step (one hipGraph replay
+    per step) and by stepping them one after another with the reference driver loop."""
+    import logging
+
+    import torch
+
+    from sopht_amd.simulator import BatchedFlowSimulation
+
+    logging.getLogger("sopht_amd").setLevel(logging.ERROR)
+    members = batched_members()
+    cells = sum(int(np.prod(f.grid_size)) for f, _, _ in members)
+    for _ in range(warmup):
+        for flow, inter, U in members:
+            step(flow, inter, U)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(seq_steps):
+        for flow, inter, U in members:
+            step(flow, inter, U)
+    torch.cuda.synchronize()
+    seq = time.perf_counter() - t0
+    sim = BatchedFlowSimulation(members)
+    sim.step(1 + warmup)
+    sim.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sim.step(steps)
+    sim.synchronize()
+    bat = time.perf_counter() - t0
+    sim.close()
+    seq_rate = cells * seq_steps / seq
+    bat_rate = cells * steps / bat
+    return {
+        "members": len(members),
+        "sizes": [f"{count}x{n}^3" for n, count in BATCH_SIZES],
+        "cells_per_step": cells,
+        "batched_cell_updates_per_s": bat_rate,
+        "batched_ms_per_step": round(1e3 * bat / steps, 3),
+        "sequential_cell_updates_per_s": seq_rate,
+        "sequential_ms_per_step": round(1e3 * seq / seq_steps, 3),
+        "speedup": round(bat_rate / seq_rate, 2),
+        "steps": steps,
+        "sequential_steps": seq_steps,
+    }
+
+
 def copy_probe(n, reps):
     """A plain device copy of known size (one (n, n, n) fp32 field per launch, read once
     and written once) through the library's elementwise copy kernel: the calibration run
@@ -180,6 +263,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2, help="steps of the CPU baseline sample (0 = skip)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="REPS",
                     help="only time REPS launches of a plain (grid^3 fp32) copy kernel and exit")
+    ap.add_argument("--batch-steps", type=int, default=20,
+                    help="steps of the batched 64-member comparison (0 = skip)")
+    ap.add_argument("--batch-only", action="store_true", help="run only the batched comparison")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -190,6 +276,12 @@ def main():
         return 2
     if args.copy_probe > 0:
         return copy_probe(args.grid, args.copy_probe)
+    if args.batch_only:
+        import torch
+
+        torch.cuda.set_device(0)
+        print(json.dumps(batched_vs_sequential(args.batch_steps, max(2, args.batch_steps // 4))))
+        return 0
 
     import torch
 
@@ -247,6 +339,11 @@ def main():
     kernels_gbs = {
         k: round(words[k] * cells * word / (avg_ms[k] * 1e-3) / 1e9, 1) for k in avg_ms if avg_ms[k] > 0
     }
+    batched = None
+    if args.batch_steps > 0:
+        del flow, inter
+        torch.cuda.empty_cache()
+        batched = batched_vs_sequential(args.batch_steps, max(2, args.batch_steps // 4))
     cpu = None
     if args.cpu_steps > 0:
         try:
@@ -277,6 +374,7 @@ def main():
         "stage_ms": {k: round(v, 4) for k, v in avg_ms.items()},
         "stage_gbs": kernels_gbs,
         "roofline": roof,
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
