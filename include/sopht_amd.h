@@ -301,6 +301,47 @@ int sopht_ns3d_batch_step(sopht_ns3d_batch* batch, int steps, int use_graph, voi
 int sopht_ns3d_batch_state(sopht_ns3d_batch* batch, double* dt, double* time, void* stream);
 int sopht_ns3d_batch_destroy(sopht_ns3d_batch* batch);
 
+/* ---- reverse-mode adjoint of the coupled step (no counterpart in the reference, which has
+ * no differentiation; the step it differentiates is navier_stokes_flow_simulators.py:449-498
+ * with the forcing of VirtualBoundaryForcing.py:149-283 and the driver loop of
+ * flow_past_sphere_case.py:191-197; dt is frozen) */
+
+/* The flow step of sopht_ns3d_step out of place, for recording trajectories: reads
+ * (vorticity_in, velocity_in, forcing), writes (vorticity_out, velocity_out); forcing is left
+ * unchanged (the in-place step zeroes it). stream_function receives psi as in the step. */
+int sopht_ns3d_trajectory_step(sopht_ns3d* plan, const void* vorticity_in, const void* velocity_in,
+                               const void* forcing, void* vorticity_out, void* velocity_out,
+                               void* stream_function, double dt, const double* free_stream,
+                               void* stream);
+/* Transpose of one flow step with inputs (vorticity, velocity, forcing) and step dt.
+ * vorticity_adj: in, the adjoint of the vorticity after the step; out, the adjoint of the
+ * vorticity before it. velocity_adj: in, the adjoint of the velocity after the step; out, the
+ * adjoint of the velocity before it through the advection term (the virtual-boundary
+ * interpolation part is added by sopht_vbf_adjoint_3d). The adjoint of the forcing is not
+ * stored: sopht_vbf_adjoint_3d evaluates it at the body's support from vorticity_adj.
+ * work: 2 x (3, nz, ny, nx) real_t scratch. Fields contiguous. */
+int sopht_ns3d_adjoint_step(sopht_ns3d* plan, const void* vorticity, const void* velocity,
+                            const void* forcing, double dt, void* vorticity_adj,
+                            void* velocity_adj, void* work, void* stream);
+/* Transpose of the virtual-boundary time step and forcing of one step (cosine kernel):
+ * dX += dt dV_prev; U = dx^3 I(u; w(X)); dV = U - V; Fl = k dX + c dV; F = S(Fl; w(X));
+ * body force f = -sum Fl. Inputs: vorticity_adj (the flow-step adjoint's output; the forcing
+ * adjoint is forcing_prefactor * curl_h of it at the support cells), velocity (u of the step),
+ * lag_positions X (float64), lag_forcing Fl of the step, body_force_adj (3 real_t, device).
+ * position_mismatch_adj / velocity_mismatch_adj: the adjoint state of dX and dV_prev, updated
+ * in place. Outputs: lag_position_adj (dJ/dX), lag_velocity_adj (dJ/dV), and
+ * velocity_adj += dx^3 S(dJ/dU) through ws (deterministic). weights (64 n), nearest_index
+ * (3 n) and lag_scratch (3 n) are scratch. */
+int sopht_vbf_adjoint_3d(int dtype, const void* vorticity_adj, double forcing_prefactor,
+                         const void* velocity, int64_t nz, int64_t ny, int64_t nx,
+                         const double* lag_positions, const void* lag_forcing,
+                         int64_t num_lag_nodes, double dx, double eul_grid_coord_shift,
+                         double stiffness, double damping, double dt, const void* body_force_adj,
+                         void* position_mismatch_adj, void* velocity_mismatch_adj,
+                         void* lag_position_adj, void* lag_velocity_adj, void* weights,
+                         int64_t* nearest_index, void* lag_scratch, sopht_spread3d* ws,
+                         void* velocity_adj, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

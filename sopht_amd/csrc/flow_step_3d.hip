@@ -15,19 +15,8 @@
 #include <new>
 
 #include "common.h"
+#include "ns3d_plan.h"
 #include "sweeps_3d.h"
-
-struct sopht_ns3d {
-  int dtype;
-  int64_t nz, ny, nx;
-  double x_range, dx, nu, rho;
-  int width, with_forcing, with_free_stream;
-  sopht_poisson3d* poisson;
-  double sine[6 * 8];  // x, y, z: width front factors then width back factors
-  void* partial;       // per-block partial maxima for the CFL reduction
-  int64_t partial_cap;
-  sopht::StageTimer timer;
-};
 
 // Stage order of one step, for sopht_ns3d_stage_times.
 static const char* const kStageNames[] = {
@@ -45,23 +34,19 @@ static constexpr int kNumStages = 8;
 namespace sopht {
 namespace {
 
-Pen make_pen(const sopht_ns3d* P) {
-  Pen pen;
-  pen.w = P->width;
-  for (int a = 0; a < 3; ++a)
-    for (int i = 0; i < 2 * P->width; ++i) pen.f[a][i] = P->sine[a * 2 * P->width + i];
-  return pen;
-}
-
+// One step from (w_in, u_in, F) to (w_out, u_out); w_out may be w_in and u_out u_in (the
+// in-place step of the reference). zero_forcing clears F in the SC sweep.
 template <typename T>
-int step_impl(sopht_ns3d* P, void* vort, void* vel, void* psi, void* forcing, void* buffer,
-              double dt, const double* U, hipStream_t s) {
+int step_impl(sopht_ns3d* P, const void* vort_in, void* vort_out, const void* vel_in,
+              void* vel_out, void* psi, void* forcing, int zero_forcing, double dt,
+              const double* U, hipStream_t s) {
   Grid g{P->nz, P->ny, P->nx};
-  T* w = static_cast<T*>(vort);
-  T* u = static_cast<T*>(vel);
+  const T* w_in = static_cast<const T*>(vort_in);
+  T* w = static_cast<T*>(vort_out);
+  const T* u_in = static_cast<const T*>(vel_in);
+  T* u = static_cast<T*>(vel_out);
   T* ps = static_cast<T*>(psi);
   T* F = static_cast<T*>(forcing);
-  (void)buffer;  // u x omega1 never leaves the S1 sweep
   const T dx = T(P->dx);
   const int wf = P->with_forcing && F != nullptr;
   // prefactors as the reference forms them: real_t(dt / (2 * dx * rho)), real_t(dt / (2 dx)),
@@ -77,13 +62,13 @@ int step_impl(sopht_ns3d* P, void* vort, void* vel, void* psi, void* forcing, vo
     const int zchunk = kS1Chunk;
     const int tiles_x = int(ceil_div(g.nx, ST::TX)), tiles_y = int(ceil_div(g.ny, ST::TY));
     const int64_t items = int64_t(tiles_x) * tiles_y * ceil_div(g.nz, zchunk);
-    k_s1<T><<<unsigned(items), ST::NT, 0, s>>>(g, w, u, F, ps, pF, pB, wf, zchunk, tiles_x, tiles_y,
-                                              items % 8 == 0 ? 1 : 0);
+    k_s1<T><<<unsigned(items), ST::NT, 0, s>>>(g, w_in, u_in, F, ps, pF, pB, wf, zchunk, tiles_x,
+                                              tiles_y, items % 8 == 0 ? 1 : 0);
     SOPHT_CHECK_LAUNCH();
   }
   tm.mark(0, s);
   const MarchLaunch ml = march_launch(g.nz, g.ny, g.nx, march_vec<T>(g.nx, P->width));
-  launch_sc<T>(ml, g, ps, w, F, pD, make_pen(P), wf, s);
+  launch_sc<T>(ml, g, ps, w, F, pD, make_pen(P), wf && zero_forcing, s);
   SOPHT_CHECK_LAUNCH();
   tm.mark(1, s);
   const int64_t n = g.nz * g.ny * g.nx;
@@ -224,8 +209,25 @@ int sopht_ns3d_step(sopht_ns3d* P, void* vort, void* vel, void* psi, void* forci
   SOPHT_REQUIRE(!P->with_forcing || forcing, "forcing field required when with_forcing");
   SOPHT_REQUIRE(!P->with_free_stream || U, "free stream velocity required");
   hipStream_t s = static_cast<hipStream_t>(stream);
-  return P->dtype == SOPHT_F32 ? step_impl<float>(P, vort, vel, psi, forcing, buffer, dt, U, s)
-                               : step_impl<double>(P, vort, vel, psi, forcing, buffer, dt, U, s);
+  (void)buffer;  // u x omega1 never leaves the S1 sweep
+  return P->dtype == SOPHT_F32
+             ? step_impl<float>(P, vort, vort, vel, vel, psi, forcing, 1, dt, U, s)
+             : step_impl<double>(P, vort, vort, vel, vel, psi, forcing, 1, dt, U, s);
+}
+
+int sopht_ns3d_trajectory_step(sopht_ns3d* P, const void* vort_in, const void* vel_in,
+                               const void* forcing, void* vort_out, void* vel_out, void* psi,
+                               double dt, const double* U, void* stream) {
+  SOPHT_REQUIRE(P != nullptr, "plan is null");
+  SOPHT_REQUIRE(vort_in && vel_in && vort_out && vel_out && psi, "null field pointer");
+  SOPHT_REQUIRE(vort_in != vort_out && vel_in != vel_out, "trajectory step is out of place");
+  SOPHT_REQUIRE(!P->with_forcing || forcing, "forcing field required when with_forcing");
+  SOPHT_REQUIRE(!P->with_free_stream || U, "free stream velocity required");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  void* F = const_cast<void*>(forcing);  // read only: zero_forcing is 0
+  return P->dtype == SOPHT_F32
+             ? step_impl<float>(P, vort_in, vort_out, vel_in, vel_out, psi, F, 0, dt, U, s)
+             : step_impl<double>(P, vort_in, vort_out, vel_in, vel_out, psi, F, 0, dt, U, s);
 }
 
 int sopht_ns3d_recover_velocity(sopht_ns3d* P, void* vort, void* vel, void* psi, const double* U,

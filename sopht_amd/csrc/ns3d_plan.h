@@ -1,0 +1,32 @@
+// Plan of the fused 3D flow step (sopht_ns3d), shared by the step (flow_step_3d.hip) and its
+// adjoint (adjoint_3d.hip).
+#pragma once
+
+#include "common.h"
+#include "sweeps_3d.h"
+
+struct sopht_ns3d {
+  int dtype;
+  int64_t nz, ny, nx;
+  double x_range, dx, nu, rho;
+  int width, with_forcing, with_free_stream;
+  sopht_poisson3d* poisson;
+  double sine[6 * 8];  // x, y, z: width front factors then width back factors
+  void* partial;       // per-block partial maxima for the CFL reduction
+  int64_t partial_cap;
+  sopht::StageTimer timer;
+};
+
+namespace sopht {
+namespace {
+
+Pen make_pen(const sopht_ns3d* P) {
+  Pen pen;
+  pen.w = P->width;
+  for (int a = 0; a < 3; ++a)
+    for (int i = 0; i < 2 * P->width; ++i) pen.f[a][i] = P->sine[a * 2 * P->width + i];
+  return pen;
+}
+
+}  // namespace
+}  // namespace sopht

@@ -1,5 +1,12 @@
 """Simulator layer: the 3D flow simulator and the immersed-body glue."""
 
+from .adjoint import (
+    CoupledFlowRollout,
+    HeaveKinematics,
+    IBFlowRolloutFunction,
+    body_forces,
+    time_averaged_force,
+)
 from .batch import BatchedFlowSimulation
 from .flow import (
     FlowSimulator,
@@ -20,6 +27,11 @@ from .immersed_body import (
 
 __all__ = [
     "BatchedFlowSimulation",
+    "CoupledFlowRollout",
+    "HeaveKinematics",
+    "IBFlowRolloutFunction",
+    "body_forces",
+    "time_averaged_force",
     "FibonacciEllipsoidForcingGrid",
     "FibonacciSphereForcingGrid",
     "FlowSimulator",
