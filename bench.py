@@ -218,6 +218,73 @@ def batched_vs_sequential(steps, seq_steps, warmup=2):
     }
 
 
+ADJ_WORKLOAD = ("heaving ellipsoid (0.1, 0.05, 0.05), 2000 Fibonacci points, 128^3 fp32, nu=1e-3, "
+                "U=(1,0,0), 200 steps, 16 Fourier heave coefficients ~ N(0, 0.1), checkpoint every 20")
+
+
+def adjoint_vs_forward(n=128, steps=200, checkpoint=20, seed=5):
+    """Time of the forward rollout (the driver loop with CFL dt) against the forward pass with
+    checkpoints plus the adjoint (recompute + reverse sweep), for the time-averaged body
+    x-force of a heaving ellipsoid; both from the same start state."""
+    import logging
+
+    import torch
+
+    from sopht_amd import simulator as sps
+    from sopht_amd.numeric.immersed_boundary_ops import VirtualBoundaryForcing
+    from sopht_amd.utils import synthetic
+
+    logging.getLogger("sopht_amd").setLevel(logging.ERROR)
+    real_t = np.float32
+    n_lag, axes = 2000, (0.1, 0.05, 0.05)
+    flow = sps.UnboundedNavierStokesFlowSimulator3D(
+        grid_size=(n, n, n), x_range=1.0, kinematic_viscosity=1e-3, cfl=0.1, real_t=real_t,
+        with_forcing=True, with_free_stream_flow=True)
+    U = np.array([1.0, 0.0, 0.0])
+    flow.recover_velocity(U)
+    grid = sps.FibonacciEllipsoidForcingGrid(3, sps.RigidBody((0.5, 0.5, 0.5)), n_lag, axes)
+    ds = grid.get_maximum_lagrangian_grid_spacing()
+    inter = VirtualBoundaryForcing(-5e4 * ds**2, -10.0 * ds**2, 3, flow.dx, n_lag, real_t,
+                                   enable_eul_grid_forcing_reset=False)
+    kin = sps.HeaveKinematics(synthetic.fibonacci_ellipsoid(n_lag, axes, (0.5, 0.5, 0.5)), period=0.25)
+    ro = sps.CoupledFlowRollout(flow, inter, kin, U, steps, checkpoint_every=checkpoint)
+    theta = torch.tensor(np.random.default_rng(seed).normal(0.0, 0.1, 16), device="cuda",
+                         requires_grad=True)
+    # warm-up (allocations, plans, the shadow interactor)
+    wro = sps.CoupledFlowRollout(flow, inter, kin, U, 2, checkpoint_every=1)
+    sps.time_averaged_force(sps.body_forces(theta, wro), wro.dts).backward()
+    del wro
+    theta.grad = None
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    f_plain = ro.run(theta, record=False)
+    torch.cuda.synchronize()
+    fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    forces = sps.body_forces(theta, ro)
+    J = sps.time_averaged_force(forces, ro.dts)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    J.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    same = bool(torch.equal(f_plain, forces.detach()))
+    g = theta.grad.detach().cpu().numpy()
+    return {
+        "workload": ADJ_WORKLOAD,
+        "forward_s": round(fwd, 4),
+        "forward_recorded_s": round(t1 - t0, 4),
+        "adjoint_s": round(t2 - t1, 4),
+        "forward_plus_adjoint_s": round(t2 - t0, 4),
+        "ratio_fwd_adj_to_fwd": round((t2 - t0) / fwd, 3),
+        "forward_ms_per_step": round(1e3 * fwd / steps, 3),
+        "objective": float(J.detach()),
+        "grad_norm": float(np.linalg.norm(g)),
+        "forward_recorded_bitwise_equal": same,
+        "gradient_check": "tests/test_gpu_adjoint.py: 64^3 fp64 gradient vs central differences (h=1e-4)",
+    }
+
+
 def copy_probe(n, reps):
     """A plain device copy of known size (one (n, n, n) fp32 field per launch, read once
     and written once) through the library's elementwise copy kernel: the calibration run
@@ -266,6 +333,9 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=20,
                     help="steps of the batched 64-member comparison (0 = skip)")
     ap.add_argument("--batch-only", action="store_true", help="run only the batched comparison")
+    ap.add_argument("--adjoint-steps", type=int, default=200,
+                    help="steps of the forward vs forward+adjoint comparison (0 = skip)")
+    ap.add_argument("--adjoint-only", action="store_true", help="run only the adjoint comparison")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -276,6 +346,12 @@ def main():
         return 2
     if args.copy_probe > 0:
         return copy_probe(args.grid, args.copy_probe)
+    if args.adjoint_only:
+        import torch
+
+        torch.cuda.set_device(0)
+        print(json.dumps(adjoint_vs_forward(steps=args.adjoint_steps)))
+        return 0
     if args.batch_only:
         import torch
 
@@ -344,6 +420,10 @@ def main():
         del flow, inter
         torch.cuda.empty_cache()
         batched = batched_vs_sequential(args.batch_steps, max(2, args.batch_steps // 4))
+    adjoint = None
+    if args.adjoint_steps > 0:
+        torch.cuda.empty_cache()
+        adjoint = adjoint_vs_forward(steps=args.adjoint_steps)
     cpu = None
     if args.cpu_steps > 0:
         try:
@@ -375,6 +455,7 @@ def main():
         "stage_gbs": kernels_gbs,
         "roofline": roof,
         "batched": batched,
+        "adjoint": adjoint,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -158,7 +158,7 @@ __global__ void __launch_bounds__(512) k_spread_plan_batch(const Member<T>* M, c
   extern __shared__ uint64_t sk[];
   const int m = find_member(off, count, blockIdx.x);
   const Member<T>& S = M[m];
-  if (atomicAdd(S.flag + 1, 0) == 0) return;  // plan still valid (uniform for the workgroup)
+  if (flag_load(S.flag + 1) == 0) return;  // plan still valid (uniform for the workgroup)
   const int64_t n = S.vb.n;
   for (int64_t i = threadIdx.x; i < n; i += blockDim.x) sk[i] = S.keys[i];
   __syncthreads();

@@ -91,14 +91,15 @@ int sopht_spread3d_create(int64_t max_nodes, sopht_spread3d** out) {
   int64_t p = 1;
   while (p < max_nodes) p <<= 1;
   w->padded = p;
-  const int init_flags[3] = {0, 1, 0};  // no out-of-domain node yet; first call builds
+  const int init_flags[4] = {0, 1, 0, 0};  // no out-of-domain node yet; first call builds
   const int64_t slots = 64 * (max_nodes > 0 ? max_nodes : 1);
   if (hipMalloc(&w->keys, sizeof(uint64_t) * p) != hipSuccess ||
       hipMalloc(&w->prev, sizeof(uint64_t) * p) != hipSuccess ||
-      hipMalloc(&w->flag, 3 * sizeof(int)) != hipSuccess ||
+      hipMalloc(&w->flag, 4 * sizeof(int)) != hipSuccess ||
       hipMalloc(&w->pcount, sizeof(int) * slots) != hipSuccess ||
       hipMalloc(&w->poff, sizeof(int) * slots) != hipSuccess ||
       hipMalloc(&w->plist, sizeof(uint32_t) * slots) != hipSuccess ||
+      hipMalloc(&w->owners, sizeof(uint32_t) * slots) != hipSuccess ||
       hipMemset(w->prev, 0xff, sizeof(uint64_t) * p) != hipSuccess ||
       hipMemcpy(w->flag, init_flags, sizeof(init_flags), hipMemcpyHostToDevice) != hipSuccess) {
     sopht_spread3d_destroy(w);
@@ -117,6 +118,7 @@ int sopht_spread3d_destroy(sopht_spread3d* w) {
   if (w->pcount) (void)hipFree(w->pcount);
   if (w->poff) (void)hipFree(w->poff);
   if (w->plist) (void)hipFree(w->plist);
+  if (w->owners) (void)hipFree(w->owners);
   delete w;
   return SOPHT_OK;
 }
@@ -155,13 +157,17 @@ int sopht_lag_to_eul_spread_3d(int dtype, int ncomp, sopht_spread3d* ws, void* c
         SOPHT_CHECK_LAUNCH();
       }
   }
-  // sorted keys in LDS for the plan build's binary searches when they fit (128 KiB)
+  // plan rebuild (a no-op unless the keys changed): owners, then the merged contributor
+  // lists; sorted keys in LDS for the owners' binary searches when they fit (128 KiB)
   if (n <= 16384)
-    k_spread_plan<true><<<blocks_for(n * 64, 512), 512, n * sizeof(uint64_t), s>>>(
-        nz, ny, nx, ws->keys, n, ws->flag, ws->pcount, ws->poff, ws->plist);
+    k_spread_owner<true><<<blocks_for(n * 64, 512), 512, n * sizeof(uint64_t), s>>>(
+        nz, ny, nx, ws->keys, n, ws->flag, ws->pcount, ws->owners);
   else
-    k_spread_plan<false><<<blocks_for(n * 64, 512), 512, 0, s>>>(nz, ny, nx, ws->keys, n, ws->flag,
-                                                                 ws->pcount, ws->poff, ws->plist);
+    k_spread_owner<false><<<blocks_for(n * 64, 512), 512, 0, s>>>(nz, ny, nx, ws->keys, n, ws->flag,
+                                                                  ws->pcount, ws->owners);
+  SOPHT_CHECK_LAUNCH();
+  k_spread_merge<<<std::min(2048, blocks_for(n * 64, 64 * kMergeWaves)), 64 * kMergeWaves, 0, s>>>(
+      nz, ny, nx, ws->keys, n, ws->flag, ws->pcount, ws->poff, ws->owners, ws->plist);
   SOPHT_CHECK_LAUNCH();
   IB_DISPATCH(dtype, ([&] {
                 EulW3<T> F;

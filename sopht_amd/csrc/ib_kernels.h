@@ -15,13 +15,14 @@ struct sopht_spread3d {
   uint64_t* keys;     // (key << 32 | node), sorted, padded
   uint64_t* prev;     // unsorted keys of the previous call, padded
   int* flag;          // [0] out-of-domain, [1] keys changed since the plan was built,
-                      // [2] fill counter of the plan's contributor list
+                      // [2] fill counter of the plan's contributor list, [3] owner count
   // Gather plan, rebuilt only when the keys change: for support slot t = 64 p + s of sorted
   // node p that owns its cell, pcount[t] contributors starting at plist[poff[t]], each
   // (node << 6 | slot) in increasing node order; pcount[t] = 0 for the other slots.
   int* pcount;
   int* poff;
   uint32_t* plist;
+  uint32_t* owners;   // owning slots t of the plan (flag[3] of them), for the list merge
   int64_t plan_nz, plan_ny, plan_nx;  // grid the plan was built for (host copy)
 };
 
@@ -29,6 +30,10 @@ namespace sopht {
 namespace {
 
 constexpr int W4 = 4;  // 2 * interp_kernel_width
+
+// Plain load of a flag written by an earlier launch (no read-modify-write: an atomic "read"
+// from every thread of a launch serialises on the flag's address).
+__device__ __forceinline__ int flag_load(const int* f) { return __atomic_load_n(f, __ATOMIC_RELAXED); }
 
 // numpy floor_divide for doubles (npy_divmod): exact floor of a / b including the
 // cases where a / b rounds onto an integer.
@@ -307,8 +312,11 @@ __global__ void k_spread_keys(const int64_t* idx, int64_t n, int64_t padded, int
 // counter for the rebuild.
 __device__ __forceinline__ void bitonic_block(const uint64_t* prev, uint64_t* keys, int padded,
                                               int* flag, uint64_t* sk) {
-  if (atomicAdd(flag + 1, 0) == 0) return;  // uniform for the workgroup
-  if (threadIdx.x == 0) atomicExch(flag + 2, 0);
+  if (flag_load(flag + 1) == 0) return;  // uniform for the workgroup
+  if (threadIdx.x == 0) {
+    atomicExch(flag + 2, 0);
+    atomicExch(flag + 3, 0);
+  }
   for (int i = threadIdx.x; i < padded; i += blockDim.x) sk[i] = prev[i];
   __syncthreads();
   for (int k = 2; k <= padded; k <<= 1) {
@@ -339,7 +347,10 @@ __global__ void __launch_bounds__(1024) k_bitonic_lds(const uint64_t* prev, uint
 __global__ void k_copy_keys(const uint64_t* src, uint64_t* dst, int64_t m, int* flag) {
   const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (i < m) dst[i] = src[i];
-  if (i == 0) atomicExch(flag + 2, 0);
+  if (i == 0) {
+    atomicExch(flag + 2, 0);
+    atomicExch(flag + 3, 0);
+  }
 }
 
 // One compare-exchange step of a global-memory bitonic sort (for large node counts).
@@ -453,11 +464,46 @@ __device__ __forceinline__ void spread_plan_slot(int64_t nz, int64_t ny, int64_t
   }
 }
 
+// Plan build for moving bodies, in two phases (only when the keys changed).
+// Phase 1, one thread per support slot t = 64 p + s: slot t owns its cell c when sorted node p
+// is the first of c's contributors in key order, i.e. when the first non-empty range among
+// c's candidate base-cell ranges (searched in increasing key order) starts at p. Owners go to
+// the owner list; every other slot gets pcount = 0.
+__device__ __forceinline__ void spread_owner_slot(int64_t nz, int64_t ny, int64_t nx,
+                                                  const uint64_t* keys, int64_t n, int* flag,
+                                                  int* pcount, uint32_t* owners, int64_t gid) {
+  const int64_t p = gid >> 6;
+  const int slot = int(gid & 63);
+  if (p >= n) return;
+  pcount[gid] = 0;
+  const uint64_t kp = keys[p];
+  if (kp == ~uint64_t(0)) return;  // out-of-domain node (sorted to the end)
+  const int64_t cell0 = int64_t(kp >> 32);
+  const int64_t bx = cell0 % nx, by = (cell0 / nx) % ny, bz = cell0 / (nx * ny);
+  const int64_t cz = bz + (slot >> 4), cy = by + ((slot >> 2) & 3), cx = bx + (slot & 3);
+  for (int g = 0; g < 16; ++g) {
+    const int dz = 3 - (g >> 2), dy = 3 - (g & 3);  // increasing key order
+    const int64_t qz = cz - dz, qy = cy - dy;
+    if (qz < 0 || qy < 0 || qz + W4 > nz || qy + W4 > ny) continue;
+    int64_t qx0 = cx - 3, qx1 = cx;  // inclusive base x range
+    if (qx0 < 0) qx0 = 0;
+    if (qx1 + W4 > nx) qx1 = nx - W4;
+    if (qx0 > qx1) continue;
+    const uint64_t c0 = uint64_t((qz * ny + qy) * nx + qx0);
+    const uint64_t c1 = uint64_t((qz * ny + qy) * nx + qx1);
+    const int64_t a = lower_bound_cell(keys, n, c0);
+    if (a < n && (keys[a] >> 32) <= c1) {  // first non-empty range
+      if (a == p) owners[atomicAdd(flag + 3, 1)] = uint32_t(gid);
+      return;
+    }
+  }
+}
+
 template <bool LDS_KEYS>
-__global__ void __launch_bounds__(512) k_spread_plan(int64_t nz, int64_t ny, int64_t nx,
-                                                     const uint64_t* gkeys, int64_t n, int* flag,
-                                                     int* pcount, int* poff, uint32_t* plist) {
-  if (atomicAdd(flag + 1, 0) == 0) return;  // plan still valid (uniform for the launch)
+__global__ void __launch_bounds__(512) k_spread_owner(int64_t nz, int64_t ny, int64_t nx,
+                                                      const uint64_t* gkeys, int64_t n, int* flag,
+                                                      int* pcount, uint32_t* owners) {
+  if (flag_load(flag + 1) == 0) return;  // plan still valid (uniform for the launch)
   extern __shared__ uint64_t skeys[];
   const uint64_t* keys = gkeys;
   if constexpr (LDS_KEYS) {
@@ -465,8 +511,81 @@ __global__ void __launch_bounds__(512) k_spread_plan(int64_t nz, int64_t ny, int
     __syncthreads();
     keys = skeys;
   }
-  spread_plan_slot(nz, ny, nx, keys, n, flag, pcount, poff, plist,
-                   blockIdx.x * int64_t(blockDim.x) + threadIdx.x);
+  spread_owner_slot(nz, ny, nx, keys, n, flag, pcount, owners,
+                    blockIdx.x * int64_t(blockDim.x) + threadIdx.x);
+}
+
+// Phase 2, one wavefront per owner: the contributors of cell c are the nodes whose base cell
+// is c - (dz, dy, dx) for (dz, dy, dx) in [0, 3]^3; lane (dz, dy, dx) holds that base cell's
+// run of the sorted keys (in increasing node order). A prefix sum over the lanes sizes the
+// list (m contributors, one reservation per owner); the runs are staged in the wavefront's
+// LDS window when they fit, and the wavefront merges the 64 runs by node: each output is the
+// minimum head over the lanes (node numbers are unique), and the contribution's support
+// slot in its node's block is the lane number itself.
+constexpr int kMergeWaves = 4, kMergeCap = 2048;  // LDS entries per wavefront
+
+__device__ __forceinline__ void spread_merge_owner(int64_t nz, int64_t ny, int64_t nx,
+                                                   const uint64_t* keys, int64_t n, int* flag,
+                                                   int* pcount, int* poff, uint32_t* plist,
+                                                   int64_t gid, int lane, uint32_t* stage) {
+  const int64_t p = gid >> 6;
+  const int slot = int(gid & 63);
+  const int64_t cell0 = int64_t(keys[p] >> 32);
+  const int64_t cx = cell0 % nx + (slot & 3), cy = (cell0 / nx) % ny + ((slot >> 2) & 3),
+                cz = cell0 / (nx * ny) + (slot >> 4);
+  const int64_t qz = cz - (lane >> 4), qy = cy - ((lane >> 2) & 3), qx = cx - (lane & 3);
+  int64_t pos = 0, hi = 0;
+  if (qz >= 0 && qy >= 0 && qx >= 0 && qz + W4 <= nz && qy + W4 <= ny && qx + W4 <= nx) {
+    const uint64_t qc = uint64_t((qz * ny + qy) * nx + qx);
+    pos = lower_bound_cell(keys, n, qc);
+    hi = lower_bound_cell(keys, n, qc + 1);
+  }
+  const int len = int(hi - pos);
+  int incl = len;  // inclusive prefix sum over the lanes
+  for (int sft = 1; sft < 64; sft <<= 1) {
+    const int v = __shfl_up(incl, sft, 64);
+    if (lane >= sft) incl += v;
+  }
+  const int m = __shfl(incl, 63, 64);
+  int base = 0;
+  if (lane == 0) base = atomicAdd(flag + 2, m);
+  base = __shfl(base, 0, 64);
+  if (lane == 0) {
+    pcount[gid] = m;
+    poff[gid] = base;
+  }
+  const bool staged = m <= kMergeCap;  // uniform
+  const int mine = incl - len;         // this lane's window in the stage
+  if (staged)
+    for (int i = 0; i < len; ++i) stage[mine + i] = uint32_t(keys[pos + i] & 0xffffffffu);
+  int i = 0;  // position within the lane's run
+  for (int k = 0; k < m; ++k) {  // m is uniform over the wavefront
+    uint32_t head = 0xffffffffu;
+    if (i < len) head = staged ? stage[mine + i] : uint32_t(keys[pos + i] & 0xffffffffu);
+    uint32_t mn = head;
+    for (int sft = 1; sft < 64; sft <<= 1) {
+      const uint32_t o = __shfl_xor(mn, sft, 64);
+      mn = o < mn ? o : mn;
+    }
+    if (i < len && head == mn) {
+      plist[base + k] = (mn << 6) | uint32_t(lane);
+      ++i;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(64 * kMergeWaves) k_spread_merge(
+    int64_t nz, int64_t ny, int64_t nx, const uint64_t* keys, int64_t n, int* flag, int* pcount,
+    int* poff, const uint32_t* owners, uint32_t* plist) {
+  __shared__ uint32_t stage[kMergeWaves][kMergeCap];
+  if (flag_load(flag + 1) == 0) return;  // plan still valid (uniform for the launch)
+  const int lane = int(threadIdx.x & 63);
+  const int w = int(threadIdx.x >> 6);
+  const int64_t waves = int64_t(gridDim.x) * kMergeWaves;
+  const int64_t count = flag_load(flag + 3);
+  for (int64_t o = int64_t(blockIdx.x) * kMergeWaves + w; o < count; o += waves)
+    spread_merge_owner(nz, ny, nx, keys, n, flag, pcount, poff, plist, int64_t(owners[o]), lane,
+                       stage[w]);
 }
 
 // Plan replay (every call): each owned cell adds its contributors F_lag[node] w[slot][node]

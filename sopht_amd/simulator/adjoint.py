@@ -44,6 +44,20 @@ class HeaveKinematics:
         self.period = float(period)
         self.n_modes = int(n_modes)
         self.num_params = 2 * self.n_modes
+        self._host = None
+        self._X = torch.empty_like(self.X0)
+        self._V = torch.empty_like(self.X0)
+
+    def prepare(self, theta):
+        """Keep a host copy of ``theta`` for :meth:`state` (one synchronisation per run)."""
+        self._host = (theta, theta.detach().to(dtype=__import__("torch").float64).cpu().numpy())
+
+    def _basis_host(self, t):
+        m = np.arange(1, self.n_modes + 1, dtype=np.float64)
+        om = 2 * math.pi * m / self.period
+        ph = om * float(t)
+        return (np.concatenate([np.cos(ph), np.sin(ph)]),
+                np.concatenate([np.sin(ph) / om, (1 - np.cos(ph)) / om]))
 
     def _basis(self, times):
         import torch
@@ -72,6 +86,17 @@ class HeaveKinematics:
         return X, V
 
     def state(self, theta, t):
+        """Positions and velocities ``(3, n)`` float64 at time t. After :meth:`prepare` with the
+        same ``theta`` the two scalars are formed on the host and the device work is four
+        small kernels (no synchronisation); the returned tensors are reused by the next call."""
+        if self._host is not None and self._host[0] is theta:
+            cv, cy = self._basis_host(t)
+            th = self._host[1]
+            self._X.copy_(self.X0)
+            self._X[1].add_(float(cy @ th))
+            self._V.zero_()
+            self._V[1].fill_(float(cv @ th))
+            return self._X, self._V
         X, V = self.trajectory(theta, [t])
         return X[0], V[0]
 
@@ -106,20 +131,18 @@ class CoupledFlowRollout:
         )
         self._dev = flow.vorticity_field.device
         self._tdt = flow.vorticity_field.dtype
-        ws = C.c_void_p()
-        L.call("sopht_spread3d_create", int(max(self.n, 1)), C.byref(ws))
-        self._ws = ws
         self._lib = L.lib()
         self.dts = None
         self.times = None
         self._shadow = None
+        self._seg_ws = []  # one spreading workspace per step of a segment (see gradient)
         del torch
 
     def __del__(self):
-        ws = getattr(self, "_ws", None)
-        if ws is not None and ws.value:
-            self._lib.sopht_spread3d_destroy(ws)
-            self._ws = None
+        for w in getattr(self, "_seg_ws", []):
+            if w.value:
+                self._lib.sopht_spread3d_destroy(w)
+        self._seg_ws = []
 
     # ---------------------------------------------------------------- forward
     def _restore(self):
@@ -147,6 +170,9 @@ class CoupledFlowRollout:
         Xs = torch.empty((ns, 3, self.n), dtype=torch.float64, device=self._dev) if record else None
         Vs = torch.empty_like(Xs) if record else None
         theta = theta.detach()
+        prep = getattr(self.kin, "prepare", None)
+        if prep is not None:
+            prep(theta)
         t = self.time0
         for s in range(ns):
             if record and s % k == 0:
@@ -220,6 +246,14 @@ class CoupledFlowRollout:
         cc = float(sh.virtual_boundary_damping_coeff)
         st = L.stream_ptr()
         lib = self._lib
+        # step j of a segment spreads with workspace j in the recompute and in the reverse
+        # sweep: the reverse sweep's spreading (same node positions) replays the plan the
+        # recompute built instead of rebuilding it
+        while len(self._seg_ws) < seg:
+            w = C.c_void_p()
+            L.call("sopht_spread3d_create", int(max(n, 1)), C.byref(w))
+            self._seg_ws.append(w)
+        Fst = L.strides3(F[0][0])
         for g in reversed(range(len(self._ck))):
             s0 = g * k
             s1 = min(ns, s0 + k)
@@ -232,8 +266,11 @@ class CoupledFlowRollout:
                 dt = float(self.dts[s])
                 sh.time_step(dt=dt)
                 F[j].zero_()
-                sh.compute_interaction_force_on_eul_and_lag_grid(F[j], U[j], self._X[s], self._V[s])
+                sh.compute_interaction_force_on_lag_grid(U[j], self._X[s], self._V[s])
                 Fl[j].copy_(sh.lag_grid_forcing_field)
+                L.check(lib.sopht_lag_to_eul_spread_3d(
+                    self._tag, 3, self._seg_ws[j], L.comps(F[j]), Fst, nz, ny, nx, L.ptr(Fl[j]),
+                    L.ptr(sh.interp_weights), L.ptr(sh.nearest_eul_grid_index_to_lag_grid), n, st))
                 flow.trajectory_step(W[j], U[j], F[j], W[j + 1], U[j + 1], dt, self.U)
             for j in reversed(range(s1 - s0)):
                 s = s0 + j
@@ -245,7 +282,7 @@ class CoupledFlowRollout:
                 L.check(lib.sopht_vbf_adjoint_3d(
                     self._tag, L.ptr(wbar), pF, L.ptr(U[j]), nz, ny, nx, L.ptr(self._X[s]),
                     L.ptr(Fl[j]), n, dx, shift, kk, cc, dt, L.ptr(fbar[s]), L.ptr(dXb), L.ptr(dVb),
-                    L.ptr(Xb[s]), L.ptr(Vb[s]), L.ptr(wts), L.ptr(idx), L.ptr(lag), self._ws,
+                    L.ptr(Xb[s]), L.ptr(Vb[s]), L.ptr(wts), L.ptr(idx), L.ptr(lag), self._seg_ws[j],
                     L.ptr(ubar), st))
         with torch.enable_grad():
             th = self._theta.detach().clone().requires_grad_(True)
