@@ -308,21 +308,25 @@ int sopht_ns3d_batch_destroy(sopht_ns3d_batch* batch);
 
 /* The flow step of sopht_ns3d_step out of place, for recording trajectories: reads
  * (vorticity_in, velocity_in, forcing), writes (vorticity_out, velocity_out); forcing is left
- * unchanged (the in-place step zeroes it). stream_function receives psi as in the step. */
+ * unchanged (the in-place step zeroes it). stream_function receives psi as in the step.
+ * vorticity1_out (may be NULL) receives omega1 = vorticity + dt/(2 dx rho) curl_h(forcing)
+ * (interior), the state the adjoint step needs. */
 int sopht_ns3d_trajectory_step(sopht_ns3d* plan, const void* vorticity_in, const void* velocity_in,
                                const void* forcing, void* vorticity_out, void* velocity_out,
-                               void* stream_function, double dt, const double* free_stream,
-                               void* stream);
-/* Transpose of one flow step with inputs (vorticity, velocity, forcing) and step dt.
+                               void* stream_function, void* vorticity1_out, double dt,
+                               const double* free_stream, void* stream);
+/* Transpose of one flow step with step dt, given the step's omega1 (vorticity1: the vorticity
+ * after the forcing update, from sopht_ns3d_trajectory_step; the vorticity itself when
+ * forcing is off) and its input velocity.
  * vorticity_adj: in, the adjoint of the vorticity after the step; out, the adjoint of the
  * vorticity before it. velocity_adj: in, the adjoint of the velocity after the step; out, the
  * adjoint of the velocity before it through the advection term (the virtual-boundary
  * interpolation part is added by sopht_vbf_adjoint_3d). The adjoint of the forcing is not
  * stored: sopht_vbf_adjoint_3d evaluates it at the body's support from vorticity_adj.
- * work: 2 x (3, nz, ny, nx) real_t scratch. Fields contiguous. */
-int sopht_ns3d_adjoint_step(sopht_ns3d* plan, const void* vorticity, const void* velocity,
-                            const void* forcing, double dt, void* vorticity_adj,
-                            void* velocity_adj, void* work, void* stream);
+ * work: (3, nz, ny, nx) real_t scratch. Fields contiguous. */
+int sopht_ns3d_adjoint_step(sopht_ns3d* plan, const void* vorticity1, const void* velocity,
+                            double dt, void* vorticity_adj, void* velocity_adj, void* work,
+                            void* stream);
 /* Transpose of the virtual-boundary time step and forcing of one step (cosine kernel):
  * dX += dt dV_prev; U = dx^3 I(u; w(X)); dV = U - V; Fl = k dX + c dV; F = S(Fl; w(X));
  * body force f = -sum Fl. Inputs: vorticity_adj (the flow-step adjoint's output; the forcing

@@ -103,8 +103,8 @@ SIGNATURES = {
     "sopht_ns3d_batch_step": [_vp, _i, _i, _vp],
     "sopht_ns3d_batch_state": [_vp, _pd, _pd, _vp],
     "sopht_ns3d_batch_destroy": [_vp],
-    "sopht_ns3d_trajectory_step": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _pd, _vp],
-    "sopht_ns3d_adjoint_step": [_vp, _vp, _vp, _vp, _d, _vp, _vp, _vp, _vp],
+    "sopht_ns3d_trajectory_step": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _pd, _vp],
+    "sopht_ns3d_adjoint_step": [_vp, _vp, _vp, _d, _vp, _vp, _vp, _vp],
     "sopht_vbf_adjoint_3d": [
         _i, _vp, _d, _vp, _i64, _i64, _i64, _vp, _vp, _i64, _d, _d, _d, _d, _d, _vp,
         _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],

@@ -11,6 +11,7 @@
 // Adjoint (C^T = C and L^T = L on every cell with zero outside the grid; G^T = G):
 //   psib = pC C(M ub') ; t = P^T(wb' + G psib) ; g2 = t + pD L(M t) ; Bb = pB C(M g2) ;
 //   ub = w1 x Bb ; g1 = g2 + Bb x u ; wb = g1 ; Fb = pF C(M g1) (at the body's support only).
+// w1 comes from the recorded trajectory (sopht_ns3d_trajectory_step writes it).
 #include <algorithm>
 #include <cmath>
 
@@ -65,9 +66,9 @@ __device__ __forceinline__ void mx_nbrs(const VecT<T, VEC>& mc, const T* __restr
 template <typename T, int VEC>
 __global__ void __launch_bounds__(kMarchThreads) k_curl_adj(Grid g, const T* __restrict__ a,
                                                             T* __restrict__ out, T p, int tiles_x,
-                                                            int tiles_y, int remap) {
+                                                            int tiles_y, int remap, int chunk) {
   using V = VecT<T, VEC>;
-  const MarchTile m = march_tile<VEC>(g, tiles_x, tiles_y, xcd_item(remap));
+  const MarchTile m = march_tile<VEC>(g, tiles_x, tiles_y, xcd_item(remap), chunk);
   const int nz = int(g.nz), ny = int(g.ny), nx = int(g.nx);
   const int n = nz * ny * nx, sz = ny * nx, sy = nx;
   const bool act = m.x0 < nx && m.y < ny;
@@ -113,9 +114,9 @@ __global__ void __launch_bounds__(kMarchThreads) k_curl_adj(Grid g, const T* __r
 template <typename T, int VEC>
 __global__ void __launch_bounds__(kMarchThreads) k_diff_adj(Grid g, const T* __restrict__ t,
                                                             T* __restrict__ out, T pD, int tiles_x,
-                                                            int tiles_y, int remap) {
+                                                            int tiles_y, int remap, int chunk) {
   using V = VecT<T, VEC>;
-  const MarchTile m = march_tile<VEC>(g, tiles_x, tiles_y, xcd_item(remap));
+  const MarchTile m = march_tile<VEC>(g, tiles_x, tiles_y, xcd_item(remap), chunk);
   const int nz = int(g.nz), ny = int(g.ny), nx = int(g.nx);
   const int n = nz * ny * nx, sz = ny * nx, sy = nx;
   const bool act = m.x0 < nx && m.y < ny;
@@ -184,109 +185,119 @@ __device__ __forceinline__ T pen_factor(const Pen& pen, int axis, int p, int n) 
   return T(1);
 }
 
-// out = P^T (a + b) (b may be null), one thread per cell.
+// ------------------------------------------------------------------ fused sweeps
+
+// P^T along one axis, in place, one thread per line of that axis (c and the two other
+// coordinates): the clamp planes w-1 and n-w gather their slabs with the sine factors, the
+// slab cells become zero. P = P_z P_y P_x is separable, so P^T is the three in any order.
 template <typename T>
-__global__ void __launch_bounds__(256) k_pen_adj(Grid g, const T* __restrict__ a,
-                                                 const T* __restrict__ b, T* __restrict__ out,
-                                                 Pen pen) {
-  const int nz = int(g.nz), ny = int(g.ny), nx = int(g.nx);
+__global__ void __launch_bounds__(256) k_pen_adj_axis(T* __restrict__ f, int axis, int nz, int ny,
+                                                      int nx, Pen pen) {
   const int n = nz * ny * nx;
+  const int len = axis == 0 ? nx : (axis == 1 ? ny : nz);
+  const int lines = 3 * (n / len);
   const int i = int(blockIdx.x) * 256 + int(threadIdx.x);
-  if (i >= 3 * n) return;
-  const int c = i / n, cell = i - c * n;
-  const int z = cell / (ny * nx), y = (cell / nx) % ny, x = cell % nx;
-  const PenRange rz = pen_range(z, nz, pen.w), ry = pen_range(y, ny, pen.w), rx = pen_range(x, nx, pen.w);
-  const T* fa = a + c * n;
-  const T* fb = b ? b + c * n : nullptr;
-  T acc = T(0);
-  for (int pz = rz.lo; pz <= rz.hi; ++pz) {
-    const T sz_ = pen_factor<T>(pen, 2, pz, nz);
-    for (int py = ry.lo; py <= ry.hi; ++py) {
-      const T syz = sz_ * pen_factor<T>(pen, 1, py, ny);
-      for (int px = rx.lo; px <= rx.hi; ++px) {
-        const int j = (pz * ny + py) * nx + px;
-        const T v = fb ? fa[j] + fb[j] : fa[j];
-        acc += (syz * pen_factor<T>(pen, 0, px, nx)) * v;
-      }
-    }
+  if (i >= lines) return;
+  // line base and stride: lines enumerate (c, other coordinates) with x fastest where x is
+  // not the axis, so neighbouring threads touch neighbouring cells
+  int base, stride;
+  const int c = i / (n / len), r = i % (n / len);
+  if (axis == 0) {          // rows (c, z, y)
+    base = c * n + r * nx;
+    stride = 1;
+  } else if (axis == 1) {   // (c, z, x)
+    base = c * n + (r / nx) * ny * nx + r % nx;
+    stride = nx;
+  } else {                  // (c, y, x)
+    base = c * n + r;
+    stride = ny * nx;
   }
-  out[i] = acc;
+  const int w = pen.w;
+  T front = T(0), back = T(0);
+  for (int p = 0; p < w; ++p) {
+    front += T(pen.f[axis][p]) * f[base + p * stride];
+    back += T(pen.f[axis][w + p]) * f[base + (len - w + p) * stride];
+  }
+  for (int p = 0; p < w - 1; ++p) {
+    f[base + p * stride] = T(0);
+    f[base + (len - 1 - p) * stride] = T(0);
+  }
+  f[base + (w - 1) * stride] = front;
+  f[base + (len - w) * stride] = back;
 }
 
-// ------------------------------------------------------------------ cross-product transpose
-
-// w1 = w + pF M C F (the forward's omega1), then ub = w1 x Bb and g1 = g2 + Bb x u.
-// One thread per VEC cells of a row.
+// Bb = pB C(M g2) on every cell, then ub = w1 x Bb and g1 = g2 + Bb x u, with w1 the
+// forward's omega1 = omega + pF M C F.
 template <typename T, int VEC>
-__global__ void __launch_bounds__(256) k_cross_adj(Grid g, const T* __restrict__ Bb,
-                                                   const T* __restrict__ g2,
-                                                   const T* __restrict__ w,
-                                                   const T* __restrict__ F,
-                                                   const T* __restrict__ u, T* __restrict__ ub,
-                                                   T* __restrict__ g1, T pF, int with_forcing) {
+__global__ void __launch_bounds__(kMarchThreads) k_curl_cross_adj(
+    Grid g, const T* __restrict__ g2, const T* __restrict__ w1, const T* __restrict__ u,
+    T* __restrict__ ub, T* __restrict__ g1, T pB, int tiles_x, int tiles_y, int remap, int chunk) {
   using V = VecT<T, VEC>;
+  const MarchTile m = march_tile<VEC>(g, tiles_x, tiles_y, xcd_item(remap), chunk);
   const int nz = int(g.nz), ny = int(g.ny), nx = int(g.nx);
   const int n = nz * ny * nx, sz = ny * nx, sy = nx;
-  const int i0 = (int(blockIdx.x) * 256 + int(threadIdx.x)) * VEC;
-  if (i0 >= n) return;
-  V a[3], B[3], G2[3], U[3];
-  for (int c = 0; c < 3; ++c) {
-    a[c] = vload<T, VEC>(w + c * n, i0);
-    B[c] = vload<T, VEC>(Bb + c * n, i0);
-    G2[c] = vload<T, VEC>(g2 + c * n, i0);
-    U[c] = vload<T, VEC>(u + c * n, i0);
-  }
-  if (with_forcing) {
-    const int z = i0 / sz, y = (i0 / sy) % ny;
-    const T* F0 = F;
-    const T* F1 = F + n;
-    const T* F2 = F + 2 * n;
-    for (int k = 0; k < VEC; ++k) {
-      const int x = (i0 + k) % nx;
-      if (z > 0 && z < nz - 1 && y > 0 && y < ny - 1 && x > 0 && x < nx - 1) {
-        const int j = i0 + k;
-        const T c0 = F2[j + sy] - F2[j - sy] - F1[j + sz] + F1[j - sz];
-        const T c1 = F0[j + sz] - F0[j - sz] - F2[j + 1] + F2[j - 1];
-        const T c2 = F1[j + 1] - F1[j - 1] - F0[j + sy] + F0[j - sy];
-        a[0].v[k] = a[0].v[k] + pF * c0;
-        a[1].v[k] = a[1].v[k] + pF * c1;
-        a[2].v[k] = a[2].v[k] + pF * c2;
+  const bool act = m.x0 < nx && m.y < ny;
+  const int y = act ? m.y : 0, x0 = act ? m.x0 : 0;
+  const T* f0 = g2;
+  const T* f1 = g2 + n;
+  const T* f2 = g2 + 2 * n;
+  V p0m = mrow<T, VEC>(f0, m.z0 - 1, y, x0, nz, ny, nx, sz, sy, act);
+  V p1m = mrow<T, VEC>(f1, m.z0 - 1, y, x0, nz, ny, nx, sz, sy, act);
+  V p0c = mrow<T, VEC>(f0, m.z0, y, x0, nz, ny, nx, sz, sy, act);
+  V p1c = mrow<T, VEC>(f1, m.z0, y, x0, nz, ny, nx, sz, sy, act);
+  for (int z = m.z0; z < m.z1; ++z) {
+    const V p0p = mrow<T, VEC>(f0, z + 1, y, x0, nz, ny, nx, sz, sy, act);
+    const V p1p = mrow<T, VEC>(f1, z + 1, y, x0, nz, ny, nx, sz, sy, act);
+    const V q2c = mrow<T, VEC>(f2, z, y, x0, nz, ny, nx, sz, sy, act);
+    const V q2p = mrow<T, VEC>(f2, z, y + 1, x0, nz, ny, nx, sz, sy, act);
+    const V q2m = mrow<T, VEC>(f2, z, y - 1, x0, nz, ny, nx, sz, sy, act);
+    const V q0p = mrow<T, VEC>(f0, z, y + 1, x0, nz, ny, nx, sz, sy, act);
+    const V q0m = mrow<T, VEC>(f0, z, y - 1, x0, nz, ny, nx, sz, sy, act);
+    T l1, r1, l2, r2;
+    mx_nbrs<T, VEC>(p1c, f1, z, y, x0, nz, ny, nx, sz, sy, m.tx, act, l1, r1);
+    mx_nbrs<T, VEC>(q2c, f2, z, y, x0, nz, ny, nx, sz, sy, m.tx, act, l2, r2);
+    const int i = z * sz + y * sy + x0;
+    if (act) {
+      V A[3], G2[3], U[3], o[3], q[3];
+      for (int c = 0; c < 3; ++c) {
+        A[c] = vload<T, VEC>(w1 + c * n, i);
+        G2[c] = vload<T, VEC>(g2 + c * n, i);
+        U[c] = vload<T, VEC>(u + c * n, i);
+      }
+      for (int k = 0; k < VEC; ++k) {
+        const T f1r = k < VEC - 1 ? p1c.v[k + 1] : r1, f1l = k > 0 ? p1c.v[k - 1] : l1;
+        const T f2r = k < VEC - 1 ? q2c.v[k + 1] : r2, f2l = k > 0 ? q2c.v[k - 1] : l2;
+        const T b0 = pB * ((q2p.v[k] - q2m.v[k]) - (p1p.v[k] - p1m.v[k]));
+        const T b1 = pB * ((p0p.v[k] - p0m.v[k]) - (f2r - f2l));
+        const T b2 = pB * ((f1r - f1l) - (q0p.v[k] - q0m.v[k]));
+        const T a0 = A[0].v[k], a1 = A[1].v[k], a2 = A[2].v[k];
+        const T u0 = U[0].v[k], u1 = U[1].v[k], u2 = U[2].v[k];
+        o[0].v[k] = a1 * b2 - a2 * b1;
+        o[1].v[k] = a2 * b0 - a0 * b2;
+        o[2].v[k] = a0 * b1 - a1 * b0;
+        q[0].v[k] = G2[0].v[k] + (b1 * u2 - b2 * u1);
+        q[1].v[k] = G2[1].v[k] + (b2 * u0 - b0 * u2);
+        q[2].v[k] = G2[2].v[k] + (b0 * u1 - b1 * u0);
+      }
+      for (int c = 0; c < 3; ++c) {
+        vstore<T, VEC>(ub + c * n, i, o[c]);
+        vstore<T, VEC>(g1 + c * n, i, q[c]);
       }
     }
-  }
-  V o[3], q[3];
-  for (int k = 0; k < VEC; ++k) {
-    const T a0 = a[0].v[k], a1 = a[1].v[k], a2 = a[2].v[k];
-    const T b0 = B[0].v[k], b1 = B[1].v[k], b2 = B[2].v[k];
-    const T u0 = U[0].v[k], u1 = U[1].v[k], u2 = U[2].v[k];
-    o[0].v[k] = a1 * b2 - a2 * b1;
-    o[1].v[k] = a2 * b0 - a0 * b2;
-    o[2].v[k] = a0 * b1 - a1 * b0;
-    q[0].v[k] = G2[0].v[k] + (b1 * u2 - b2 * u1);
-    q[1].v[k] = G2[1].v[k] + (b2 * u0 - b0 * u2);
-    q[2].v[k] = G2[2].v[k] + (b0 * u1 - b1 * u0);
-  }
-  for (int c = 0; c < 3; ++c) {
-    vstore<T, VEC>(ub + c * n, i0, o[c]);
-    vstore<T, VEC>(g1 + c * n, i0, q[c]);
+    p0m = p0c;
+    p1m = p1c;
+    p0c = p0p;
+    p1c = p1p;
   }
 }
 
 template <typename T>
-int launch_march_adj(int which, const MarchLaunch& l, Grid g, const T* in, T* out, T p,
-                     hipStream_t s) {
-  constexpr int VV = 16 / sizeof(T);
-  if (which == 0) {
-    if (l.vec == 1)
-      k_curl_adj<T, 1><<<l.blocks, kMarchThreads, 0, s>>>(g, in, out, p, l.tiles_x, l.tiles_y, l.remap);
-    else
-      k_curl_adj<T, VV><<<l.blocks, kMarchThreads, 0, s>>>(g, in, out, p, l.tiles_x, l.tiles_y, l.remap);
-  } else {
-    if (l.vec == 1)
-      k_diff_adj<T, 1><<<l.blocks, kMarchThreads, 0, s>>>(g, in, out, p, l.tiles_x, l.tiles_y, l.remap);
-    else
-      k_diff_adj<T, VV><<<l.blocks, kMarchThreads, 0, s>>>(g, in, out, p, l.tiles_x, l.tiles_y, l.remap);
-  }
+int launch_curl_adj(const MarchLaunch& l, Grid g, const T* in, T* out, T p, hipStream_t s) {
+  if (l.vec == 1)
+    k_curl_adj<T, 1><<<l.blocks, kMarchThreads, 0, s>>>(g, in, out, p, l.tiles_x, l.tiles_y, l.remap, l.chunk);
+  else
+    k_curl_adj<T, 16 / sizeof(T)><<<l.blocks, kMarchThreads, 0, s>>>(g, in, out, p, l.tiles_x,
+                                                                    l.tiles_y, l.remap, l.chunk);
   SOPHT_CHECK_LAUNCH();
   return SOPHT_OK;
 }
@@ -299,49 +310,56 @@ inline int adj_vec(int64_t nx) {
 }
 
 template <typename T>
-int adjoint_step_impl(sopht_ns3d* P, const void* vort, const void* vel, const void* forcing,
-                      double dt, void* vort_adj, void* vel_adj, void* work, hipStream_t s) {
+int adjoint_step_impl(sopht_ns3d* P, const void* vort1, const void* vel, double dt,
+                      void* vort_adj, void* vel_adj, void* work, hipStream_t s) {
   Grid g{P->nz, P->ny, P->nx};
   const int64_t n = g.nz * g.ny * g.nx;
   const T dx = T(P->dx);
   // the forward's prefactors (flow_step_3d.hip step_impl)
-  const T pF = T(dt / (2.0 * double(dx) * P->rho));
   const T pB = T(dt / (2.0 * double(dx)));
   const T pD = T(P->nu * dt / double(dx) / double(dx));
   const T pC = T(0.5 / double(dx));
   T* wb = static_cast<T*>(vort_adj);
   T* ub = static_cast<T*>(vel_adj);
   T* W0 = static_cast<T*>(work);
-  T* W1 = W0 + 3 * n;
-  const MarchLaunch ml = march_launch(g.nz, g.ny, g.nx, adj_vec<T>(g.nx));
+  const MarchLaunch ml = march_launch(g.nz, g.ny, g.nx, adj_vec<T>(g.nx), true);
   int rc;
   // psib = pC C(M ub)
-  if ((rc = launch_march_adj<T>(0, ml, g, ub, W0, pC, s))) return rc;
-  // G psib
+  if ((rc = launch_curl_adj<T>(ml, g, ub, W0, pC, s))) return rc;
+  // wb += G psib (the Poisson's last pass accumulates)
   {
-    void* const sol[3] = {W1, W1 + n, W1 + 2 * n};
+    void* const sol[3] = {wb, wb + n, wb + 2 * n};
     const void* const rhs[3] = {W0, W0 + n, W0 + 2 * n};
     const int64_t st[3] = {g.ny * g.nx, g.nx, 1};
-    if ((rc = sopht_poisson3d_vector_solve(P->poisson, sol, st, rhs, st, s))) return rc;
+    if ((rc = poisson_vector_solve_accumulate(P->poisson, sol, st, rhs, st, s))) return rc;
   }
-  // t = P^T (wb + G psib)
-  k_pen_adj<T><<<unsigned(ceil_div(3 * n, 256)), 256, 0, s>>>(g, wb, W1, W0, make_pen(P));
-  SOPHT_CHECK_LAUNCH();
+  // t = P^T wb, in place, axis by axis
+  const Pen pen = make_pen(P);
+  if (pen.w > 0) {
+    const int dims[3] = {int(g.nx), int(g.ny), int(g.nz)};
+    for (int axis = 0; axis < 3; ++axis) {
+      const int lines = int(3 * n / dims[axis]);
+      k_pen_adj_axis<T><<<unsigned(ceil_div(lines, 256)), 256, 0, s>>>(wb, axis, int(g.nz), int(g.ny),
+                                                                       int(g.nx), pen);
+      SOPHT_CHECK_LAUNCH();
+    }
+  }
   // g2 = t + pD L(M t)
-  if ((rc = launch_march_adj<T>(1, ml, g, W0, W1, pD, s))) return rc;
-  // Bb = pB C(M g2)
-  if ((rc = launch_march_adj<T>(0, ml, g, W1, W0, pB, s))) return rc;
-  // ub = w1 x Bb ; wb = g2 + Bb x u
-  const int wf = P->with_forcing && forcing != nullptr;
-  const int v = (g.nx % (16 / int(sizeof(T))) == 0) ? 16 / int(sizeof(T)) : 1;
-  const unsigned nb = unsigned(ceil_div(n / v, 256));
-  const T* wn = static_cast<const T*>(vort);
-  const T* Fn = static_cast<const T*>(forcing);
+  const T* w1 = static_cast<const T*>(vort1);
   const T* un = static_cast<const T*>(vel);
-  if (v == 1)
-    k_cross_adj<T, 1><<<nb, 256, 0, s>>>(g, W0, W1, wn, Fn, un, ub, wb, pF, wf);
+  constexpr int VV = 16 / sizeof(T);
+  if (ml.vec == 1)
+    k_diff_adj<T, 1><<<ml.blocks, kMarchThreads, 0, s>>>(g, wb, W0, pD, ml.tiles_x, ml.tiles_y, ml.remap, ml.chunk);
   else
-    k_cross_adj<T, 16 / sizeof(T)><<<nb, 256, 0, s>>>(g, W0, W1, wn, Fn, un, ub, wb, pF, wf);
+    k_diff_adj<T, VV><<<ml.blocks, kMarchThreads, 0, s>>>(g, wb, W0, pD, ml.tiles_x, ml.tiles_y, ml.remap, ml.chunk);
+  SOPHT_CHECK_LAUNCH();
+  // Bb = pB C(M g2) ; ub = w1 x Bb ; wb = g2 + Bb x u
+  if (ml.vec == 1)
+    k_curl_cross_adj<T, 1><<<ml.blocks, kMarchThreads, 0, s>>>(g, W0, w1, un, ub, wb, pB, ml.tiles_x,
+                                                               ml.tiles_y, ml.remap, ml.chunk);
+  else
+    k_curl_cross_adj<T, VV><<<ml.blocks, kMarchThreads, 0, s>>>(g, W0, w1, un, ub, wb, pB, ml.tiles_x,
+                                                                ml.tiles_y, ml.remap, ml.chunk);
   SOPHT_CHECK_LAUNCH();
   return SOPHT_OK;
 }
@@ -459,15 +477,14 @@ using namespace sopht;
 
 extern "C" {
 
-int sopht_ns3d_adjoint_step(sopht_ns3d* P, const void* vort, const void* vel, const void* forcing,
-                            double dt, void* vort_adj, void* vel_adj, void* work, void* stream) {
+int sopht_ns3d_adjoint_step(sopht_ns3d* P, const void* vort1, const void* vel, double dt,
+                            void* vort_adj, void* vel_adj, void* work, void* stream) {
   SOPHT_REQUIRE(P != nullptr, "plan is null");
-  SOPHT_REQUIRE(vort && vel && vort_adj && vel_adj && work, "null field pointer");
-  SOPHT_REQUIRE(!P->with_forcing || forcing, "forcing field required when with_forcing");
+  SOPHT_REQUIRE(vort1 && vel && vort_adj && vel_adj && work, "null field pointer");
   hipStream_t s = static_cast<hipStream_t>(stream);
   return P->dtype == SOPHT_F32
-             ? adjoint_step_impl<float>(P, vort, vel, forcing, dt, vort_adj, vel_adj, work, s)
-             : adjoint_step_impl<double>(P, vort, vel, forcing, dt, vort_adj, vel_adj, work, s);
+             ? adjoint_step_impl<float>(P, vort1, vel, dt, vort_adj, vel_adj, work, s)
+             : adjoint_step_impl<double>(P, vort1, vel, dt, vort_adj, vel_adj, work, s);
 }
 
 int sopht_vbf_adjoint_3d(int dtype, const void* vort_adj, double forcing_prefactor,

@@ -154,6 +154,9 @@ namespace sopht {
 int poisson_vector_solve_timed(sopht_poisson3d* P, void* const sol[3], const int64_t* ss,
                                const void* const rhs[3], const int64_t* rs, hipStream_t s,
                                StageTimer* timer, int first_stage);
+// As the vector solve, adding the solution into sol (sol += G rhs).
+int poisson_vector_solve_accumulate(sopht_poisson3d* P, void* const sol[3], const int64_t* ss,
+                                    const void* const rhs[3], const int64_t* rs, hipStream_t s);
 // A plan whose slab holds `comps` components (one Poisson plan per grid size of a batch).
 int poisson_create(int dtype, int64_t nz, int64_t ny, int64_t nx, double x_range, int comps,
                    void* stream, sopht_poisson3d** out);

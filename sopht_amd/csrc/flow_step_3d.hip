@@ -39,7 +39,7 @@ namespace {
 template <typename T>
 int step_impl(sopht_ns3d* P, const void* vort_in, void* vort_out, const void* vel_in,
               void* vel_out, void* psi, void* forcing, int zero_forcing, double dt,
-              const double* U, hipStream_t s) {
+              const double* U, hipStream_t s, void* vort1_out = nullptr) {
   Grid g{P->nz, P->ny, P->nx};
   const T* w_in = static_cast<const T*>(vort_in);
   T* w = static_cast<T*>(vort_out);
@@ -63,11 +63,12 @@ int step_impl(sopht_ns3d* P, const void* vort_in, void* vort_out, const void* ve
     const int tiles_x = int(ceil_div(g.nx, ST::TX)), tiles_y = int(ceil_div(g.ny, ST::TY));
     const int64_t items = int64_t(tiles_x) * tiles_y * ceil_div(g.nz, zchunk);
     k_s1<T><<<unsigned(items), ST::NT, 0, s>>>(g, w_in, u_in, F, ps, pF, pB, wf, zchunk, tiles_x,
-                                              tiles_y, items % 8 == 0 ? 1 : 0);
+                                              tiles_y, items % 8 == 0 ? 1 : 0,
+                                              static_cast<T*>(vort1_out));
     SOPHT_CHECK_LAUNCH();
   }
   tm.mark(0, s);
-  const MarchLaunch ml = march_launch(g.nz, g.ny, g.nx, march_vec<T>(g.nx, P->width));
+  const MarchLaunch ml = march_launch(g.nz, g.ny, g.nx, march_vec<T>(g.nx, P->width), true);
   launch_sc<T>(ml, g, ps, w, F, pD, make_pen(P), wf && zero_forcing, s);
   SOPHT_CHECK_LAUNCH();
   tm.mark(1, s);
@@ -107,7 +108,7 @@ int recover_impl(sopht_ns3d* P, void* vort, void* vel, void* psi, const double* 
   const T U0 = P->with_free_stream ? T(U[0]) : T(0);
   const T U1 = P->with_free_stream ? T(U[1]) : T(0);
   const T U2 = P->with_free_stream ? T(U[2]) : T(0);
-  const MarchLaunch ml = march_launch(g.nz, g.ny, g.nx, march_vec<T>(g.nx, P->width));
+  const MarchLaunch ml = march_launch(g.nz, g.ny, g.nx, march_vec<T>(g.nx, P->width), true);
   launch_s2<T>(ml, g, ps, static_cast<T*>(vel), T(0.5 / P->dx), U0, U1, U2, s);
   SOPHT_CHECK_LAUNCH();
   return SOPHT_OK;
@@ -217,7 +218,7 @@ int sopht_ns3d_step(sopht_ns3d* P, void* vort, void* vel, void* psi, void* forci
 
 int sopht_ns3d_trajectory_step(sopht_ns3d* P, const void* vort_in, const void* vel_in,
                                const void* forcing, void* vort_out, void* vel_out, void* psi,
-                               double dt, const double* U, void* stream) {
+                               void* vort1_out, double dt, const double* U, void* stream) {
   SOPHT_REQUIRE(P != nullptr, "plan is null");
   SOPHT_REQUIRE(vort_in && vel_in && vort_out && vel_out && psi, "null field pointer");
   SOPHT_REQUIRE(vort_in != vort_out && vel_in != vel_out, "trajectory step is out of place");
@@ -225,9 +226,10 @@ int sopht_ns3d_trajectory_step(sopht_ns3d* P, const void* vort_in, const void* v
   SOPHT_REQUIRE(!P->with_free_stream || U, "free stream velocity required");
   hipStream_t s = static_cast<hipStream_t>(stream);
   void* F = const_cast<void*>(forcing);  // read only: zero_forcing is 0
+  SOPHT_REQUIRE(vort1_out != vort_in && vort1_out != vort_out, "omega1 needs its own buffer");
   return P->dtype == SOPHT_F32
-             ? step_impl<float>(P, vort_in, vort_out, vel_in, vel_out, psi, F, 0, dt, U, s)
-             : step_impl<double>(P, vort_in, vort_out, vel_in, vel_out, psi, F, 0, dt, U, s);
+             ? step_impl<float>(P, vort_in, vort_out, vel_in, vel_out, psi, F, 0, dt, U, s, vort1_out)
+             : step_impl<double>(P, vort_in, vort_out, vel_in, vel_out, psi, F, 0, dt, U, s, vort1_out);
 }
 
 int sopht_ns3d_recover_velocity(sopht_ns3d* P, void* vort, void* vel, void* psi, const double* U,

@@ -347,7 +347,7 @@ __global__ void __launch_bounds__((ColTile<T, L>::NT)) __attribute__((amdgpu_wav
 // half spectra A, B (kx <= nx) of rows ra, rb become Z[k] = A[k] + i B[k],
 // Z[L-k] = conj(A[k]) + i conj(B[k]); imaginary parts at kx = 0 and nx are dropped
 // (irfft semantics).
-template <typename T, int L>
+template <typename T, int L, bool ACC>
 __global__ void __launch_bounds__((RowTile<T, L>::NT)) k5_xc2r(SlabGeom g, const Cplx<T>* H, CompPtrsW out,
                                                              const Cplx<T>* tw) {
   using RT = RowTile<T, L>;
@@ -392,8 +392,13 @@ __global__ void __launch_bounds__((RowTile<T, L>::NT)) k5_xc2r(SlabGeom g, const
   static_for<0, E / 2>([&](auto e_) {
     constexpr int e = decltype(e_)::value;
     const int64_t x = t + TT * e;
-    if (pa) pa[x * out.sx] = v[e].re;
-    if (pb) pb[x * out.sx] = v[e].im;
+    if constexpr (ACC) {  // solution += result (the adjoint adds G psi_bar to omega_bar)
+      if (pa) pa[x * out.sx] += v[e].re;
+      if (pb) pb[x * out.sx] += v[e].im;
+    } else {
+      if (pa) pa[x * out.sx] = v[e].re;
+      if (pb) pb[x * out.sx] = v[e].im;
+    }
   });
 }
 
@@ -503,13 +508,16 @@ int launch_k3(int64_t L, SlabGeom g, Cplx<T>* H, const T* ghat, const Cplx<T>* t
 
 template <typename T>
 int launch_k5(int64_t L, SlabGeom g, const Cplx<T>* H, CompPtrsW out, const Cplx<T>* tw,
-              hipStream_t s) {
+              hipStream_t s, bool acc) {
   const int64_t rows = int64_t(g.ncomp) * g.nz * g.ny;
   const int64_t pairs = (rows + 1) / 2;
 #define X(n)                                                                          \
   if (L == n) {                                                                       \
     using RT = RowTile<T, n>;                                                         \
-    k5_xc2r<T, n><<<ceil_div(pairs, RT::NTEAM), RT::NT, RT::lds, s>>>(g, H, out, tw); \
+    if (acc)                                                                          \
+      k5_xc2r<T, n, true><<<ceil_div(pairs, RT::NTEAM), RT::NT, RT::lds, s>>>(g, H, out, tw); \
+    else                                                                              \
+      k5_xc2r<T, n, false><<<ceil_div(pairs, RT::NTEAM), RT::NT, RT::lds, s>>>(g, H, out, tw); \
     SOPHT_CHECK_LAUNCH();                                                             \
     return SOPHT_OK;                                                                  \
   }
@@ -530,7 +538,7 @@ template <typename T>
 int solve_impl(sopht_poisson3d* P, int ncomp, void* const sol[3], const int64_t* ss,
                const void* const rhs[3], const int64_t* rs, hipStream_t s,
                StageTimer* timer = nullptr, int first_stage = 0, void* const* sol_tab = nullptr,
-               const void* const* rhs_tab = nullptr) {
+               const void* const* rhs_tab = nullptr, bool accumulate = false) {
   if (ncomp > P->slab_comps) {
     set_last_error("more components than the Poisson plan's slab holds", __FILE__, __LINE__);
     return SOPHT_ERR_INVALID;
@@ -562,7 +570,7 @@ int solve_impl(sopht_poisson3d* P, int ncomp, void* const sol[3], const int64_t*
   mark(2);
   if ((rc = launch_ycols<T, true>(2 * P->ny, g, H, twy, s))) return rc;
   mark(3);
-  rc = launch_k5<T>(2 * P->nx, g, H, out, twx, s);
+  rc = launch_k5<T>(2 * P->nx, g, H, out, twx, s, accumulate);
   mark(4);
   return rc;
 }
@@ -642,6 +650,13 @@ int poisson_vector_solve_timed(sopht_poisson3d* P, void* const sol[3], const int
                                StageTimer* timer, int first_stage) {
   return P->dtype == SOPHT_F32 ? solve_impl<float>(P, 3, sol, ss, rhs, rs, s, timer, first_stage)
                                : solve_impl<double>(P, 3, sol, ss, rhs, rs, s, timer, first_stage);
+}
+
+int poisson_vector_solve_accumulate(sopht_poisson3d* P, void* const sol[3], const int64_t* ss,
+                                    const void* const rhs[3], const int64_t* rs, hipStream_t s) {
+  return P->dtype == SOPHT_F32
+             ? solve_impl<float>(P, 3, sol, ss, rhs, rs, s, nullptr, 0, nullptr, nullptr, true)
+             : solve_impl<double>(P, 3, sol, ss, rhs, rs, s, nullptr, 0, nullptr, nullptr, true);
 }
 
 int poisson_table_solve(sopht_poisson3d* P, int ncomp, void* const* sol_tab, const int64_t* ss,

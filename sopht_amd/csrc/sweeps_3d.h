@@ -43,7 +43,7 @@ struct S1Tile {
 template <typename T>
 __device__ __forceinline__ void s1_body(Grid g, const T* w, const T* u, const T* F, T* w2out, T pF,
                                         T pB, int with_forcing, int zchunk, int tiles_x,
-                                        int tiles_y, int64_t item) {
+                                        int tiles_y, int64_t item, T* w1out = nullptr) {
   using S = S1Tile<T>;
   constexpr int TX = S::TX, TY = S::TY, GX = S::GX, GP = S::GP;
   __shared__ T bs[3][3][GP];
@@ -133,6 +133,11 @@ __device__ __forceinline__ void s1_body(Grid g, const T* w, const T* u, const T*
       w2out[j] = o0;
       w2out[n + j] = o1;
       w2out[2 * n + j] = o2;
+      if (w1out) {  // omega1 as well (trajectories recorded for the adjoint)
+        w1out[j] = a1prev[0];
+        w1out[n + j] = a1prev[1];
+        w1out[2 * n + j] = a1prev[2];
+      }
     }
     for (int c = 0; c < 3; ++c) {
       bm2[c] = bm1[c];
@@ -153,8 +158,9 @@ __device__ __forceinline__ int64_t xcd_item(int remap) {
 template <typename T>
 __global__ void __launch_bounds__(S1Tile<T>::NT) __attribute__((amdgpu_waves_per_eu(6)))
 k_s1(Grid g, const T* w, const T* u, const T* F, T* w2out, T pF, T pB, int with_forcing,
-     int zchunk, int tiles_x, int tiles_y, int remap) {
-  s1_body<T>(g, w, u, F, w2out, pF, pB, with_forcing, zchunk, tiles_x, tiles_y, xcd_item(remap));
+     int zchunk, int tiles_x, int tiles_y, int remap, T* w1out) {
+  s1_body<T>(g, w, u, F, w2out, pF, pB, with_forcing, zchunk, tiles_x, tiles_y, xcd_item(remap),
+             w1out);
 }
 
 struct Pen {
@@ -205,28 +211,34 @@ struct MarchTile {
 
 template <int VEC>
 __device__ __forceinline__ MarchTile march_tile(const Grid& g, int tiles_x, int tiles_y,
-                                                int64_t item) {
+                                                int64_t item, int chunk = kMarchChunk) {
   using G = MarchGeom<VEC>;
   MarchTile m;
   m.tx = int(threadIdx.x % G::MXT);
   m.x0 = int(item % tiles_x) * MX + m.tx * VEC;
   m.y = int((item / tiles_x) % tiles_y) * G::MYV + int(threadIdx.x / G::MXT);
-  m.z0 = int(item / (int64_t(tiles_x) * tiles_y)) * kMarchChunk;
-  m.z1 = m.z0 + kMarchChunk < int(g.nz) ? m.z0 + kMarchChunk : int(g.nz);
+  m.z0 = int(item / (int64_t(tiles_x) * tiles_y)) * chunk;
+  m.z1 = m.z0 + chunk < int(g.nz) ? m.z0 + chunk : int(g.nz);
   return m;
 }
 
 struct MarchLaunch {
   unsigned blocks;
-  int tiles_x, tiles_y, remap, vec;
+  int tiles_x, tiles_y, remap, vec, chunk;
 };
 
-inline MarchLaunch march_launch(int64_t nz, int64_t ny, int64_t nx, int vec) {
+// chunk = z planes per work item. The default keeps kMarchChunk; fill = true halves it (down
+// to 2) while the launch has fewer than about 1024 workgroups, so small grids still put four
+// wavefronts on every SIMD (the extra z-window planes of a shorter chunk come from L2).
+inline MarchLaunch march_launch(int64_t nz, int64_t ny, int64_t nx, int vec, bool fill = false) {
   MarchLaunch l;
   l.vec = vec;
   l.tiles_x = int(ceil_div(nx, MX));
   l.tiles_y = int(ceil_div(ny, kMarchThreads / (MX / vec)));
-  const int64_t items = int64_t(l.tiles_x) * l.tiles_y * ceil_div(nz, kMarchChunk);
+  l.chunk = kMarchChunk;
+  if (fill)
+    while (l.chunk > 2 && int64_t(l.tiles_x) * l.tiles_y * ceil_div(nz, l.chunk) < 1024) l.chunk /= 2;
+  const int64_t items = int64_t(l.tiles_x) * l.tiles_y * ceil_div(nz, l.chunk);
   l.blocks = unsigned(items);
   l.remap = items % 8 == 0 ? 1 : 0;
   return l;
@@ -260,9 +272,9 @@ template <typename T, int VEC>
 __device__ __forceinline__ void sc_body(Grid g, const T* __restrict__ w2, T* __restrict__ w,
                                         T* __restrict__ F, T pD, const Pen& pen,
                                         int zero_forcing, int tiles_x, int tiles_y,
-                                        int64_t item) {
+                                        int64_t item, int chunk = kMarchChunk) {
   using V = VecT<T, VEC>;
-  const MarchTile m = march_tile<VEC>(g, tiles_x, tiles_y, item);
+  const MarchTile m = march_tile<VEC>(g, tiles_x, tiles_y, item, chunk);
   const int nz = int(g.nz), ny = int(g.ny), nx = int(g.nx);
   const int n = nz * ny * nx, sz = ny * nx, sy = nx;
   const bool act = m.x0 < nx && m.y < ny;  // whole vectors (nx % VEC == 0)
@@ -365,17 +377,17 @@ template <typename T, int VEC>
 __global__ void __launch_bounds__(kMarchThreads) k_sc(Grid g, const T* __restrict__ w2,
                                                       T* __restrict__ w, T* __restrict__ F, T pD,
                                                       Pen pen, int zero_forcing, int tiles_x,
-                                                      int tiles_y, int remap) {
-  sc_body<T, VEC>(g, w2, w, F, pD, pen, zero_forcing, tiles_x, tiles_y, xcd_item(remap));
+                                                      int tiles_y, int remap, int chunk) {
+  sc_body<T, VEC>(g, w2, w, F, pD, pen, zero_forcing, tiles_x, tiles_y, xcd_item(remap), chunk);
 }
 
 // S2: u = p curl_h(psi) (interior), 0 on the border, + U_inf.
 template <typename T, int VEC>
 __device__ __forceinline__ void s2_body(Grid g, const T* __restrict__ psi, T* __restrict__ u, T p,
                                         T U0, T U1, T U2, int tiles_x, int tiles_y,
-                                        int64_t item) {
+                                        int64_t item, int chunk = kMarchChunk) {
   using V = VecT<T, VEC>;
-  const MarchTile m = march_tile<VEC>(g, tiles_x, tiles_y, item);
+  const MarchTile m = march_tile<VEC>(g, tiles_x, tiles_y, item, chunk);
   const int nz = int(g.nz), ny = int(g.ny), nx = int(g.nx);
   const int n = nz * ny * nx, sz = ny * nx, sy = nx;
   const bool act = m.x0 < nx && m.y < ny;
@@ -437,8 +449,8 @@ __device__ __forceinline__ void s2_body(Grid g, const T* __restrict__ psi, T* __
 template <typename T, int VEC>
 __global__ void __launch_bounds__(kMarchThreads) k_s2(Grid g, const T* __restrict__ psi,
                                                       T* __restrict__ u, T p, T U0, T U1, T U2,
-                                                      int tiles_x, int tiles_y, int remap) {
-  s2_body<T, VEC>(g, psi, u, p, U0, U1, U2, tiles_x, tiles_y, xcd_item(remap));
+                                                      int tiles_x, int tiles_y, int remap, int chunk) {
+  s2_body<T, VEC>(g, psi, u, p, U0, U1, U2, tiles_x, tiles_y, xcd_item(remap), chunk);
 }
 
 // vector width for a row length and penalty width: whole 16-byte vectors per row, and the x
@@ -453,20 +465,22 @@ template <typename T>
 void launch_sc(const MarchLaunch& l, Grid g, const T* w2, T* w, T* F, T pD, const Pen& pen, int wf,
                hipStream_t s) {
   if (l.vec == 1)
-    k_sc<T, 1><<<l.blocks, kMarchThreads, 0, s>>>(g, w2, w, F, pD, pen, wf, l.tiles_x, l.tiles_y, l.remap);
+    k_sc<T, 1><<<l.blocks, kMarchThreads, 0, s>>>(g, w2, w, F, pD, pen, wf, l.tiles_x, l.tiles_y, l.remap,
+                                                  l.chunk);
   else
     k_sc<T, 16 / sizeof(T)><<<l.blocks, kMarchThreads, 0, s>>>(g, w2, w, F, pD, pen, wf, l.tiles_x,
-                                                                l.tiles_y, l.remap);
+                                                                l.tiles_y, l.remap, l.chunk);
 }
 
 template <typename T>
 void launch_s2(const MarchLaunch& l, Grid g, const T* psi, T* u, T p, T U0, T U1, T U2,
                hipStream_t s) {
   if (l.vec == 1)
-    k_s2<T, 1><<<l.blocks, kMarchThreads, 0, s>>>(g, psi, u, p, U0, U1, U2, l.tiles_x, l.tiles_y, l.remap);
+    k_s2<T, 1><<<l.blocks, kMarchThreads, 0, s>>>(g, psi, u, p, U0, U1, U2, l.tiles_x, l.tiles_y, l.remap,
+                                                  l.chunk);
   else
     k_s2<T, 16 / sizeof(T)><<<l.blocks, kMarchThreads, 0, s>>>(g, psi, u, p, U0, U1, U2, l.tiles_x,
-                                                                l.tiles_y, l.remap);
+                                                                l.tiles_y, l.remap, l.chunk);
 }
 
 // max over cells of |u0| + |u1| + |u2| (sum in component order, as numpy's axis-0 sum)

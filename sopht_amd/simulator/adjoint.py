@@ -226,13 +226,16 @@ class CoupledFlowRollout:
         fbar = fbar.detach().to(device=dev, dtype=tdt).contiguous()
         sh = self._shadow_interactor()
         seg = min(k, ns)
-        W = torch.empty((seg + 1, *shape), dtype=tdt, device=dev)
-        U = torch.empty_like(W)
-        F = torch.empty((seg, *shape), dtype=tdt, device=dev)
+        # per step of a segment: omega1 (the adjoint step's input) and u; omega ping-pongs and
+        # F is scratch (the adjoint needs only omega1 and u)
+        W = torch.empty((2, *shape), dtype=tdt, device=dev)
+        W1 = torch.empty((seg, *shape), dtype=tdt, device=dev)
+        U = torch.empty((seg + 1, *shape), dtype=tdt, device=dev)
+        F = torch.empty(shape, dtype=tdt, device=dev)
         Fl = torch.empty((seg, 3, n), dtype=tdt, device=dev)
         wbar = torch.zeros(shape, dtype=tdt, device=dev)
         ubar = torch.zeros_like(wbar)
-        work = torch.empty((2, *shape), dtype=tdt, device=dev)
+        work = torch.empty(shape, dtype=tdt, device=dev)
         dXb = torch.zeros((3, n), dtype=tdt, device=dev)
         dVb = torch.zeros_like(dXb)
         Xb = torch.empty((ns, 3, n), dtype=tdt, device=dev)
@@ -253,7 +256,8 @@ class CoupledFlowRollout:
             w = C.c_void_p()
             L.call("sopht_spread3d_create", int(max(n, 1)), C.byref(w))
             self._seg_ws.append(w)
-        Fst = L.strides3(F[0][0])
+        Fst = L.strides3(F[0])
+        Fp = L.comps(F)
         for g in reversed(range(len(self._ck))):
             s0 = g * k
             s1 = min(ns, s0 + k)
@@ -265,18 +269,19 @@ class CoupledFlowRollout:
             for j, s in enumerate(range(s0, s1)):
                 dt = float(self.dts[s])
                 sh.time_step(dt=dt)
-                F[j].zero_()
+                F.zero_()
                 sh.compute_interaction_force_on_lag_grid(U[j], self._X[s], self._V[s])
                 Fl[j].copy_(sh.lag_grid_forcing_field)
                 L.check(lib.sopht_lag_to_eul_spread_3d(
-                    self._tag, 3, self._seg_ws[j], L.comps(F[j]), Fst, nz, ny, nx, L.ptr(Fl[j]),
+                    self._tag, 3, self._seg_ws[j], Fp, Fst, nz, ny, nx, L.ptr(Fl[j]),
                     L.ptr(sh.interp_weights), L.ptr(sh.nearest_eul_grid_index_to_lag_grid), n, st))
-                flow.trajectory_step(W[j], U[j], F[j], W[j + 1], U[j + 1], dt, self.U)
+                flow.trajectory_step(W[j % 2], U[j], F, W[(j + 1) % 2], U[j + 1], dt, self.U,
+                                     vorticity1_out=W1[j])
             for j in reversed(range(s1 - s0)):
                 s = s0 + j
                 dt = float(self.dts[s])
                 L.check(lib.sopht_ns3d_adjoint_step(
-                    flow._plan, L.ptr(W[j]), L.ptr(U[j]), L.ptr(F[j]), dt, L.ptr(wbar), L.ptr(ubar),
+                    flow._plan, L.ptr(W1[j]), L.ptr(U[j]), dt, L.ptr(wbar), L.ptr(ubar),
                     L.ptr(work), st))
                 pF = dt / (2.0 * dx * float(flow.flow_density))
                 L.check(lib.sopht_vbf_adjoint_3d(

@@ -180,15 +180,16 @@ class UnboundedNavierStokesFlowSimulator3D(FlowSimulator):
         self._step(dt, free_stream_velocity, self.eul_grid_forcing_field)
 
     def trajectory_step(self, vorticity_in, velocity_in, forcing, vorticity_out, velocity_out, dt,
-                        free_stream_velocity=_zeros_3):
+                        free_stream_velocity=_zeros_3, vorticity1_out=None):
         """The step of ``time_step`` out of place, for recording trajectories (no counterpart in
         the reference): reads the state (vorticity_in, velocity_in, forcing), writes the next
-        state into (vorticity_out, velocity_out) and leaves forcing unchanged. The simulator's
+        state into (vorticity_out, velocity_out) and leaves forcing unchanged; optionally
+        ``vorticity1_out`` receives omega1 = omega + dt/(2 dx rho) curl_h(F). The simulator's
         own fields and time are untouched except ``stream_func_field`` (scratch)."""
         L.call(
             "sopht_ns3d_trajectory_step", self._plan, L.ptr(vorticity_in), L.ptr(velocity_in),
             L.ptr(forcing), L.ptr(vorticity_out), L.ptr(velocity_out), L.ptr(self.stream_func_field),
-            float(dt), self._fs(free_stream_velocity), L.stream_ptr(),
+            L.ptr(vorticity1_out), float(dt), self._fs(free_stream_velocity), L.stream_ptr(),
         )
 
     def recover_velocity(self, free_stream_velocity=_zeros_3):

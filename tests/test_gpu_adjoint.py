@@ -55,8 +55,17 @@ def test_trajectory_step_matches_in_place_step(dev, precision, shape, rng):
     U1 = torch.empty_like(w0)
     F = F0.clone()
     U = np.array([1.0, -0.5, 0.25])
-    g.trajectory_step(w0, u0, F, W1, U1, 1e-3, U)
+    w1 = torch.empty_like(w0)
+    g.trajectory_step(w0, u0, F, W1, U1, 1e-3, U, vorticity1_out=w1)
     assert torch.equal(F, F0)
+    # omega1 = omega + dt/(2 dx rho) curl_h F on the interior (the oracle's forcing update)
+    o = OF.UnboundedNavierStokesFlowSimulator3D(shape, 1.0, 1e-2, real_t=np.float64, with_forcing=True)
+    ref = to_host(w0).astype(np.float64)
+    from oracle import stencils as OS
+
+    OS.update_vorticity_from_velocity_forcing(ref, to_host(F0).astype(np.float64),
+                                              np.float64(1e-3 / (2 * float(g.dx))))
+    assert rel_l2(to_host(w1), ref) <= (1e-15 if real_t == np.float64 else 1e-6)
     g.vorticity_field.copy_(w0)
     g.velocity_field.copy_(u0)
     g.eul_grid_forcing_field.copy_(F0)
@@ -67,22 +76,30 @@ def test_trajectory_step_matches_in_place_step(dev, precision, shape, rng):
 
 @pytest.mark.parametrize("shape", [(12, 8, 16), (8, 12, 6), (16, 16, 16), (24, 16, 12)])
 @pytest.mark.parametrize("rho", [1.0, 0.5])
-def test_flow_step_adjoint_matches_oracle(dev, shape, rho, rng):
+@pytest.mark.parametrize("width", [2, 1, 3])
+def test_flow_step_adjoint_matches_oracle(dev, shape, rho, width, rng):
     import torch
 
     real_t = np.float64
     nu, dt = 1e-2, 2e-3
     o = OF.UnboundedNavierStokesFlowSimulator3D(shape, 1.0, nu, real_t=real_t, with_forcing=True,
-                                                with_free_stream_flow=True, flow_density=rho)
-    g = gpu_sim(shape, real_t, nu=nu, rho=rho)
+                                                with_free_stream_flow=True, flow_density=rho,
+                                                penalty_zone_width=width)
+    g = sps().UnboundedNavierStokesFlowSimulator3D(
+        grid_size=shape, x_range=1.0, kinematic_viscosity=nu, real_t=real_t, with_forcing=True,
+        with_free_stream_flow=True, flow_density=rho, penalty_zone_width=width)
     w, u, F, wb, ub = (rng.standard_normal((3,) + shape) for _ in range(5))
     ow, ou, oF = OA.flow_step_adjoint(o, dt, w, u, F, wb, ub)
-    d = [to_dev(a, dev) for a in (w, u, F, wb, ub)]
-    work = torch.empty((2, 3) + shape, dtype=torch.float64, device=dev)
-    L().call("sopht_ns3d_adjoint_step", g._plan, L().ptr(d[0]), L().ptr(d[1]), L().ptr(d[2]),
-             dt, L().ptr(d[3]), L().ptr(d[4]), L().ptr(work), L().stream_ptr())
-    assert rel_l2(to_host(d[3]), ow) <= 1e-12, rel_l2(to_host(d[3]), ow)
-    assert rel_l2(to_host(d[4]), ou) <= 1e-12, rel_l2(to_host(d[4]), ou)
+    w1 = w.copy()  # omega1, the adjoint step's input
+    from oracle import stencils as OS
+
+    OS.update_vorticity_from_velocity_forcing(w1, F, OA.step_prefactors(o, dt)[0])
+    d = [to_dev(a, dev) for a in (w1, u, wb, ub)]
+    work = torch.empty((3,) + shape, dtype=torch.float64, device=dev)
+    L().call("sopht_ns3d_adjoint_step", g._plan, L().ptr(d[0]), L().ptr(d[1]), dt, L().ptr(d[2]),
+             L().ptr(d[3]), L().ptr(work), L().stream_ptr())
+    assert rel_l2(to_host(d[2]), ow) <= 1e-12, rel_l2(to_host(d[2]), ow)
+    assert rel_l2(to_host(d[3]), ou) <= 1e-12, rel_l2(to_host(d[3]), ou)
 
 
 def test_vbf_adjoint_matches_oracle(dev, rng):
