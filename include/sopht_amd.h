@@ -253,6 +253,12 @@ int sopht_ns3d_recover_velocity(sopht_ns3d* plan, void* vorticity, void* velocit
 int sopht_ns3d_max_abs_velocity_sum(sopht_ns3d* plan, const void* velocity, void* out,
                                     void* stream);
 
+/* The same maximum for the velocity field that the last sopht_ns3d_step /
+ * sopht_ns3d_recover_velocity of this plan wrote, from the per-workgroup maxima its velocity
+ * sweep kept (no pass over the field). SOPHT_ERR_INVALID if the plan's last velocity sweep
+ * wrote another field. The caller must know that the field is unchanged since. */
+int sopht_ns3d_last_velocity_max(sopht_ns3d* plan, const void* velocity, void* out, void* stream);
+
 /* Per-kernel timing of sopht_ns3d_step with HIP events on the launch stream (no
  * counterpart in the reference, which has no profiling hooks). enable_timing creates the
  * events; stage_times harvests (synchronising on the last recorded step), writes the

@@ -96,6 +96,7 @@ SIGNATURES = {
     "sopht_ns3d_step": [_vp, _vp, _vp, _vp, _vp, _vp, _d, _pd, _vp],
     "sopht_ns3d_recover_velocity": [_vp, _vp, _vp, _vp, _pd, _vp],
     "sopht_ns3d_max_abs_velocity_sum": [_vp, _vp, _vp, _vp],
+    "sopht_ns3d_last_velocity_max": [_vp, _vp, _vp, _vp],
     "sopht_ns3d_enable_timing": [_vp, _i],
     "sopht_ns3d_stage_times": [_vp, _pd, _i, _pi64, _i],
     "sopht_ns3d_stage_name": [_i],

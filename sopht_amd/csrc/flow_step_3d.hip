@@ -81,8 +81,11 @@ int step_impl(sopht_ns3d* P, const void* vort_in, void* vort_out, const void* ve
   const T U0 = P->with_free_stream ? T(U[0]) : T(0);
   const T U1 = P->with_free_stream ? T(U[1]) : T(0);
   const T U2 = P->with_free_stream ? T(U[2]) : T(0);
-  launch_s2<T>(ml, g, ps, u, pC, U0, U1, U2, s);
+  const bool fused_max = int64_t(ml.blocks) <= P->s2max_cap;
+  launch_s2<T>(ml, g, ps, u, pC, U0, U1, U2, s, fused_max ? static_cast<T*>(P->s2max) : nullptr);
   SOPHT_CHECK_LAUNCH();
+  P->s2_partials = fused_max ? int64_t(ml.blocks) : 0;
+  P->s2_velocity = u;
   tm.mark(7, s);
   tm.end();
   return SOPHT_OK;
@@ -109,8 +112,12 @@ int recover_impl(sopht_ns3d* P, void* vort, void* vel, void* psi, const double* 
   const T U1 = P->with_free_stream ? T(U[1]) : T(0);
   const T U2 = P->with_free_stream ? T(U[2]) : T(0);
   const MarchLaunch ml = march_launch(g.nz, g.ny, g.nx, march_vec<T>(g.nx, P->width), true);
-  launch_s2<T>(ml, g, ps, static_cast<T*>(vel), T(0.5 / P->dx), U0, U1, U2, s);
+  const bool fused_max = int64_t(ml.blocks) <= P->s2max_cap;
+  launch_s2<T>(ml, g, ps, static_cast<T*>(vel), T(0.5 / P->dx), U0, U1, U2, s,
+               fused_max ? static_cast<T*>(P->s2max) : nullptr);
   SOPHT_CHECK_LAUNCH();
+  P->s2_partials = fused_max ? int64_t(ml.blocks) : 0;
+  P->s2_velocity = vel;
   return SOPHT_OK;
 }
 
@@ -158,7 +165,11 @@ int sopht_ns3d_create(int dtype, int64_t nz, int64_t ny, int64_t nx, double x_ra
     return rc;
   }
   P->partial_cap = 4096;
-  if (hipMalloc(&P->partial, P->partial_cap * 8) != hipSuccess) {
+  P->s2max_cap = int64_t(march_launch(nz, ny, nx, 1, true).blocks);  // VEC 1 has the most
+  P->s2_partials = 0;
+  P->s2_velocity = nullptr;
+  if (hipMalloc(&P->partial, P->partial_cap * 8) != hipSuccess ||
+      hipMalloc(&P->s2max, P->s2max_cap * 8) != hipSuccess) {
     sopht_ns3d_destroy(P);
     set_last_error("hipMalloc of the reduction scratch failed", __FILE__, __LINE__);
     return SOPHT_ERR_HIP;
@@ -199,6 +210,7 @@ int sopht_ns3d_destroy(sopht_ns3d* P) {
   P->timer.disable();
   sopht_poisson3d_destroy(P->poisson);
   if (P->partial) (void)hipFree(P->partial);
+  if (P->s2max) (void)hipFree(P->s2max);
   delete P;
   return SOPHT_OK;
 }
@@ -257,6 +269,20 @@ int sopht_ns3d_max_abs_velocity_sum(sopht_ns3d* P, const void* vel, void* out, v
     SOPHT_CHECK_LAUNCH();
     k_max_final<double><<<1, 256, 0, s>>>(static_cast<const double*>(P->partial), nb, static_cast<double*>(out));
   }
+  SOPHT_CHECK_LAUNCH();
+  return SOPHT_OK;
+}
+
+int sopht_ns3d_last_velocity_max(sopht_ns3d* P, const void* vel, void* out, void* stream) {
+  SOPHT_REQUIRE(P != nullptr, "plan is null");
+  SOPHT_REQUIRE(P->s2_partials > 0 && vel == P->s2_velocity,
+                "no fused maxima for this velocity field (the last sweep wrote another one)");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int nb = int(P->s2_partials);
+  if (P->dtype == SOPHT_F32)
+    k_max_final<float><<<1, 256, 0, s>>>(static_cast<const float*>(P->s2max), nb, static_cast<float*>(out));
+  else
+    k_max_final<double><<<1, 256, 0, s>>>(static_cast<const double*>(P->s2max), nb, static_cast<double*>(out));
   SOPHT_CHECK_LAUNCH();
   return SOPHT_OK;
 }

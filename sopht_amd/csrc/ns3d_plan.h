@@ -14,6 +14,10 @@ struct sopht_ns3d {
   double sine[6 * 8];  // x, y, z: width front factors then width back factors
   void* partial;       // per-block partial maxima for the CFL reduction
   int64_t partial_cap;
+  void* s2max;         // per-workgroup maxima of sum_i |u_i| written by the last S2 sweep
+  int64_t s2max_cap;
+  int64_t s2_partials; // how many (0: none)
+  const void* s2_velocity;  // the velocity field that S2 wrote
   sopht::StageTimer timer;
 };
 

@@ -385,9 +385,11 @@ __global__ void __launch_bounds__(kMarchThreads) k_sc(Grid g, const T* __restric
 template <typename T, int VEC>
 __device__ __forceinline__ void s2_body(Grid g, const T* __restrict__ psi, T* __restrict__ u, T p,
                                         T U0, T U1, T U2, int tiles_x, int tiles_y,
-                                        int64_t item, int chunk = kMarchChunk) {
+                                        int64_t item, int chunk = kMarchChunk,
+                                        T* umax = nullptr) {
   using V = VecT<T, VEC>;
   const MarchTile m = march_tile<VEC>(g, tiles_x, tiles_y, item, chunk);
+  T vmax = T(0);  // max over this thread's cells of |u0| + |u1| + |u2| (when umax is given)
   const int nz = int(g.nz), ny = int(g.ny), nx = int(g.nx);
   const int n = nz * ny * nx, sz = ny * nx, sy = nx;
   const bool act = m.x0 < nx && m.y < ny;
@@ -436,6 +438,9 @@ __device__ __forceinline__ void s2_body(Grid g, const T* __restrict__ psi, T* __
       o[0].v[k] = c0 + U[0];
       o[1].v[k] = c1 + U[1];
       o[2].v[k] = c2 + U[2];
+      // passive_transport_flow_simulators.py:151: sum over components of |u_i|, numpy's order
+      const T sabs = (fabs(o[0].v[k]) + fabs(o[1].v[k])) + fabs(o[2].v[k]);
+      vmax = sabs > vmax ? sabs : vmax;
     }
     if (act)
       for (int c = 0; c < 3; ++c) vstore<T, VEC>(u + c * n, i, o[c]);
@@ -444,13 +449,24 @@ __device__ __forceinline__ void s2_body(Grid g, const T* __restrict__ psi, T* __
     p0c = p0p;
     p1c = p1p;
   }
+  if (umax) {  // one partial maximum per workgroup (inactive lanes hold 0)
+    __shared__ T red[kMarchThreads];
+    red[threadIdx.x] = act ? vmax : T(0);
+    __syncthreads();
+    for (int k = kMarchThreads / 2; k > 0; k >>= 1) {
+      if (int(threadIdx.x) < k) red[threadIdx.x] = red[threadIdx.x + k] > red[threadIdx.x] ? red[threadIdx.x + k] : red[threadIdx.x];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) umax[blockIdx.x] = red[0];
+  }
 }
 
 template <typename T, int VEC>
 __global__ void __launch_bounds__(kMarchThreads) k_s2(Grid g, const T* __restrict__ psi,
                                                       T* __restrict__ u, T p, T U0, T U1, T U2,
-                                                      int tiles_x, int tiles_y, int remap, int chunk) {
-  s2_body<T, VEC>(g, psi, u, p, U0, U1, U2, tiles_x, tiles_y, xcd_item(remap), chunk);
+                                                      int tiles_x, int tiles_y, int remap, int chunk,
+                                                      T* umax) {
+  s2_body<T, VEC>(g, psi, u, p, U0, U1, U2, tiles_x, tiles_y, xcd_item(remap), chunk, umax);
 }
 
 // vector width for a row length and penalty width: whole 16-byte vectors per row, and the x
@@ -474,13 +490,13 @@ void launch_sc(const MarchLaunch& l, Grid g, const T* w2, T* w, T* F, T pD, cons
 
 template <typename T>
 void launch_s2(const MarchLaunch& l, Grid g, const T* psi, T* u, T p, T U0, T U1, T U2,
-               hipStream_t s) {
+               hipStream_t s, T* umax = nullptr) {
   if (l.vec == 1)
     k_s2<T, 1><<<l.blocks, kMarchThreads, 0, s>>>(g, psi, u, p, U0, U1, U2, l.tiles_x, l.tiles_y, l.remap,
-                                                  l.chunk);
+                                                  l.chunk, umax);
   else
     k_s2<T, 16 / sizeof(T)><<<l.blocks, kMarchThreads, 0, s>>>(g, psi, u, p, U0, U1, U2, l.tiles_x,
-                                                                l.tiles_y, l.remap, l.chunk);
+                                                                l.tiles_y, l.remap, l.chunk, umax);
 }
 
 // max over cells of |u0| + |u1| + |u2| (sum in component order, as numpy's axis-0 sum)

@@ -170,6 +170,7 @@ class UnboundedNavierStokesFlowSimulator3D(FlowSimulator):
             L.ptr(self.stream_func_field), L.ptr(forcing), L.ptr(self.buffer_vector_field),
             float(dt), self._fs(free_stream_velocity), L.stream_ptr(),
         )
+        self._velocity_version = self.velocity_field._version
 
     # :449-485
     def _navier_stokes_time_step(self, dt, free_stream_velocity=_zeros_3):
@@ -199,15 +200,24 @@ class UnboundedNavierStokesFlowSimulator3D(FlowSimulator):
             L.ptr(self.velocity_field), L.ptr(self.stream_func_field), self._fs(free_stream_velocity),
             L.stream_ptr(),
         )
+        self._velocity_version = self.velocity_field._version
 
     # :501-512
     def compute_stable_timestep(self, dt_prefac=1.0):
         return dt_prefac * self._stable_timestep_from_max(self.max_abs_velocity_sum())
 
     def max_abs_velocity_sum(self):
-        """``max over cells of sum_i |u_i|`` (one device reduction and one readback)."""
-        L.call("sopht_ns3d_max_abs_velocity_sum", self._plan, L.ptr(self.velocity_field),
-               L.ptr(self._max_out), L.stream_ptr())
+        """``max over cells of sum_i |u_i|`` (one readback). When the velocity field is the one
+        the last step wrote (torch's version counter of the tensor is unchanged), the maxima
+        its velocity sweep kept per workgroup are reduced instead of re-reading the field."""
+        lib = self._lib
+        rc = -1
+        if getattr(self, "_velocity_version", None) == self.velocity_field._version:
+            rc = lib.sopht_ns3d_last_velocity_max(self._plan, L.ptr(self.velocity_field),
+                                                  L.ptr(self._max_out), L.stream_ptr())
+        if rc != 0:
+            L.call("sopht_ns3d_max_abs_velocity_sum", self._plan, L.ptr(self.velocity_field),
+                   L.ptr(self._max_out), L.stream_ptr())
         return self.real_t(self._max_out.item())
 
     def _stable_timestep_from_max(self, vmax):

@@ -167,3 +167,35 @@ def test_coupled_step_fp32_vs_fp64_oracle(dev):
     eu = rel_l2(to_host(g.velocity_field), o.velocity_field)
     print(f"fp32 vs fp64 oracle after 3 steps: vorticity rel-L2 {ew:.3e}, velocity rel-L2 {eu:.3e}")
     assert ew < 1e-5 and eu < 1e-5
+
+
+@pytest.mark.parametrize("precision", ["single", "double"])
+@pytest.mark.parametrize("shape", [(16, 16, 16), (8, 12, 6)])
+def test_fused_velocity_max_matches_full_reduction(dev, precision, shape, rng):
+    """The CFL maximum from the velocity sweep's per-workgroup maxima equals the full
+    reduction over the field bit for bit; once the field changes through torch the full
+    reduction is used (passive_transport_flow_simulators.py:139-155)."""
+    import torch
+
+    from sopht_amd import _lib as L
+
+    real_t = PREC[precision]
+    g = sps().UnboundedNavierStokesFlowSimulator3D(
+        grid_size=shape, x_range=1.0, kinematic_viscosity=1e-2, real_t=real_t, with_forcing=True,
+        with_free_stream_flow=True)
+    g.vorticity_field.copy_(to_dev(rng.standard_normal((3,) + shape).astype(real_t), dev))
+    g.eul_grid_forcing_field.copy_(to_dev(rng.standard_normal((3,) + shape).astype(real_t), dev))
+    U = np.array([0.3, -0.2, 0.1])
+    g.recover_velocity(U)
+    g.time_step(dt=1e-3, free_stream_velocity=U)
+
+    def full():
+        out = torch.zeros((), dtype=g.velocity_field.dtype, device=dev)
+        L.call("sopht_ns3d_max_abs_velocity_sum", g._plan, L.ptr(g.velocity_field), L.ptr(out),
+               L.stream_ptr())
+        return out.item()
+
+    ref = np.max(np.sum(np.abs(to_host(g.velocity_field)), axis=0))
+    assert g.max_abs_velocity_sum() == real_t(full()) == ref
+    g.velocity_field.mul_(2.0)  # a change the step did not make
+    assert g.max_abs_velocity_sum() == real_t(full()) == real_t(2.0) * ref
