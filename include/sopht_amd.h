@@ -221,6 +221,17 @@ int sopht_vbf_lag_force_3d(int dtype, int kernel_type, const void* const velocit
 int sopht_vbf_time_step_3d(int dtype, void* position_mismatch, const void* velocity_mismatch,
                            int64_t num_lag_nodes, double dt, void* stream);
 
+/* gen_laplacian_filter_kernel_3d (scalar field) —
+ * sopht/numeric/eulerian_grid_ops/stencil_ops_3d/laplacian_filter_3d.py:13-194.
+ * filter_type 0 = "multiplicative", 1 = "convolution". Buffers as the reference's
+ * (filter_flux_buffer, field_buffer); their contents afterwards are those the reference leaves.
+ * A vector field is its three components in turn (:176-189). */
+int sopht_laplacian_filter_3d(int dtype, int filter_order, int filter_type, void* field,
+                              const int64_t* field_strides, int64_t nz, int64_t ny, int64_t nx,
+                              void* filter_flux_buffer, const int64_t* flux_strides,
+                              void* field_buffer, const int64_t* buffer_strides,
+                              int flux_boundary_width, void* stream);
+
 /* ---------------------------------------------------------------- the fused flow step */
 
 typedef struct sopht_ns3d sopht_ns3d;
@@ -232,6 +243,12 @@ int sopht_ns3d_create(int dtype, int64_t nz, int64_t ny, int64_t nx, double x_ra
                       int with_forcing, int with_free_stream_flow, void* stream,
                       sopht_ns3d** plan);
 int sopht_ns3d_destroy(sopht_ns3d* plan);
+
+/* Vorticity filtering of the step (filter_vorticity=True with filter_setting_dict,
+ * navier_stokes_flow_simulators.py:274-292,396-408, applied at :473 after diffusion and before
+ * penalisation). order < 0 turns it off (the default); type 0 multiplicative, 1 convolution.
+ * The step then uses the components 0 and 1 of its buffer argument as the filter buffers. */
+int sopht_ns3d_set_filter(sopht_ns3d* plan, int filter_order, int filter_type);
 
 /* _navier_stokes_with_forcing_time_step / _navier_stokes_time_step — :449-498.
  * Fields are contiguous (3, nz, ny, nx). forcing may be NULL when with_forcing is 0 and is

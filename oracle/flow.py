@@ -55,6 +55,7 @@ class UnboundedNavierStokesFlowSimulator3D:
         with_free_stream_flow=False,
         flow_density=1.0,
         penalty_zone_width=2,
+        filter_setting_dict=None,
     ):
         self.grid_size = tuple(grid_size)
         self.x_range = x_range
@@ -66,6 +67,8 @@ class UnboundedNavierStokesFlowSimulator3D:
         self.with_free_stream_flow = with_free_stream_flow
         self.flow_density = flow_density
         self.penalty_zone_width = penalty_zone_width
+        # navier_stokes_flow_simulators.py:274-292,396-408 (None: filter off)
+        self.filter_setting_dict = filter_setting_dict
         self.position_field, self.dx = position_field(self.grid_size, x_range, real_t)
         shape = (3, *self.grid_size)
         self.vorticity_field = np.zeros(shape, dtype=real_t)
@@ -93,7 +96,8 @@ class UnboundedNavierStokesFlowSimulator3D:
             stencils.add_fixed_val(self.velocity_field, self.velocity_field, np.asarray(free_stream_velocity))
 
     def navier_stokes_time_step(self, dt, free_stream_velocity=(0.0, 0.0, 0.0)):
-        """``_navier_stokes_time_step`` (``:449-485``), filter off."""
+        """``_navier_stokes_time_step`` (``:449-485``); the vorticity filter (``:473``) when
+        ``filter_setting_dict`` is given."""
         rt = self.real_t
         b = self.buffer_vector_field
         stencils.elementwise_cross_product(b, self.velocity_field, self.vorticity_field)
@@ -103,6 +107,11 @@ class UnboundedNavierStokesFlowSimulator3D:
             self.buffer_scalar_field,
             rt(self.kinematic_viscosity * dt / self.dx / self.dx),
         )
+        if self.filter_setting_dict is not None:  # :473
+            stencils.laplacian_filter_vector(
+                self.vorticity_field, self.filter_setting_dict["order"], self.filter_setting_dict["type"],
+                self.buffer_vector_field[0], self.buffer_vector_field[1],
+            )
         self.recover_velocity(free_stream_velocity)
 
     def time_step(self, dt, free_stream_velocity=(0.0, 0.0, 0.0)):

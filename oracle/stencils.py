@@ -322,3 +322,44 @@ def vorticity_stretching_timestep_ssprk3(
     vorticity_stretching_flux(flux_field, m, velocity_field, dt_by_2_dx * 0.5)
     m[...] = m + flux_field
     vorticity_field[...] = (1.0 / 3.0) * vorticity_field + (2.0 / 3.0) * m
+
+
+def laplacian_filter_1d(filter_flux, field, axis):
+    """One 1D Laplacian filter pass (``laplacian_filter_3d.py:60-80``): on ``[1:-1]^3``,
+    ``flux = 0.25 * (-f[+1] - f[-1] + 2 f)`` along ``axis`` (0 = z, 1 = y, 2 = x); the rest of
+    ``filter_flux`` is left as it is."""
+    c = field[I, I, I]
+    filter_flux[I, I, I] = 0.25 * ((-_d(field, axis, 1) - _d(field, axis, -1)) + 2 * c)
+
+
+def laplacian_filter(scalar_field, filter_order, filter_type, filter_flux_buffer, field_buffer,
+                     boundary_width=1):
+    """``gen_laplacian_filter_kernel_3d`` on a scalar field (``laplacian_filter_3d.py:98-166``),
+    in place. The flux buffer's boundary of ``boundary_width`` is zeroed first; each pass writes
+    the interior and the field buffer becomes a full copy of the flux.
+    multiplicative: ``order`` times x, y, z passes, then ``field -= flux``;
+    convolution: per axis x, y, z: ``order`` passes from a copy of the field, then ``field -= flux``."""
+    set_fixed_val_at_boundaries(filter_flux_buffer, boundary_width, 0)
+    axes = (2, 1, 0)  # x, y, z
+    if filter_type == "multiplicative":
+        field_buffer[...] = scalar_field
+        for _ in range(filter_order):
+            for a in axes:
+                laplacian_filter_1d(filter_flux_buffer, field_buffer, a)
+                field_buffer[...] = filter_flux_buffer
+        scalar_field[...] = scalar_field - filter_flux_buffer
+    elif filter_type == "convolution":
+        for a in axes:
+            field_buffer[...] = scalar_field
+            for _ in range(filter_order):
+                laplacian_filter_1d(filter_flux_buffer, field_buffer, a)
+                field_buffer[...] = filter_flux_buffer
+            scalar_field[...] = scalar_field - filter_flux_buffer
+    else:
+        raise ValueError("Invalid filter type")
+
+
+def laplacian_filter_vector(vector_field, filter_order, filter_type, filter_flux_buffer, field_buffer):
+    """Vector form (``laplacian_filter_3d.py:176-189``): components x, y, z in turn."""
+    for i in range(3):
+        laplacian_filter(vector_field[i], filter_order, filter_type, filter_flux_buffer, field_buffer)

@@ -97,6 +97,8 @@ SIGNATURES = {
     "sopht_ns3d_recover_velocity": [_vp, _vp, _vp, _vp, _pd, _vp],
     "sopht_ns3d_max_abs_velocity_sum": [_vp, _vp, _vp, _vp],
     "sopht_ns3d_last_velocity_max": [_vp, _vp, _vp, _vp],
+    "sopht_ns3d_set_filter": [_vp, _i, _i],
+    "sopht_laplacian_filter_3d": [_i, _i, _i, _vp, _pi64, _i64, _i64, _i64, _vp, _pi64, _vp, _pi64, _i, _vp],
     "sopht_ns3d_enable_timing": [_vp, _i],
     "sopht_ns3d_stage_times": [_vp, _pd, _i, _pi64, _i],
     "sopht_ns3d_stage_name": [_i],

@@ -154,6 +154,11 @@ namespace sopht {
 int poisson_vector_solve_timed(sopht_poisson3d* P, void* const sol[3], const int64_t* ss,
                                const void* const rhs[3], const int64_t* rs, hipStream_t s,
                                StageTimer* timer, int first_stage);
+// The Laplacian filter of one scalar field (gen_laplacian_filter_kernel_3d), type 0
+// multiplicative, 1 convolution, with the reference's buffers and kernel sequence.
+int laplacian_filter_field(int dtype, int order, int type, void* field, const int64_t* fs,
+                           int64_t nz, int64_t ny, int64_t nx, void* flux, const int64_t* xs,
+                           void* fbuf, const int64_t* bs, int bw, hipStream_t s);
 // As the vector solve, adding the solution into sol (sol += G rhs).
 int poisson_vector_solve_accumulate(sopht_poisson3d* P, void* const sol[3], const int64_t* ss,
                                     const void* const rhs[3], const int64_t* rs, hipStream_t s);

@@ -39,7 +39,7 @@ namespace {
 template <typename T>
 int step_impl(sopht_ns3d* P, const void* vort_in, void* vort_out, const void* vel_in,
               void* vel_out, void* psi, void* forcing, int zero_forcing, double dt,
-              const double* U, hipStream_t s, void* vort1_out = nullptr) {
+              const double* U, hipStream_t s, void* vort1_out = nullptr, void* buffer = nullptr) {
   Grid g{P->nz, P->ny, P->nx};
   const T* w_in = static_cast<const T*>(vort_in);
   T* w = static_cast<T*>(vort_out);
@@ -69,8 +69,30 @@ int step_impl(sopht_ns3d* P, const void* vort_in, void* vort_out, const void* ve
   }
   tm.mark(0, s);
   const MarchLaunch ml = march_launch(g.nz, g.ny, g.nx, march_vec<T>(g.nx, P->width), true);
-  launch_sc<T>(ml, g, ps, w, F, pD, make_pen(P), wf && zero_forcing, s);
-  SOPHT_CHECK_LAUNCH();
+  if (P->filter_order >= 0) {
+    // diffusion alone, the filter (buffer components 0 and 1, as the reference), then the
+    // penalisation (navier_stokes_flow_simulators.py:466-474)
+    Pen none = make_pen(P);
+    none.w = 0;
+    launch_sc<T>(ml, g, ps, w, F, pD, none, wf && zero_forcing, s);
+    SOPHT_CHECK_LAUNCH();
+    const int64_t n = g.nz * g.ny * g.nx;
+    const int64_t st[3] = {g.ny * g.nx, g.nx, 1};
+    T* bf = static_cast<T*>(buffer);
+    for (int c = 0; c < 3; ++c) {
+      int rc = laplacian_filter_field(P->dtype, P->filter_order, P->filter_type, w + c * n, st, g.nz,
+                                      g.ny, g.nx, bf, st, bf + n, st, 1, s);
+      if (rc) return rc;
+    }
+    for (int c = 0; c < 3; ++c) {
+      int rc = sopht_penalise_field_boundary_3d(P->dtype, w + c * n, st, g.nz, g.ny, g.nx, P->width,
+                                                P->sine, s);
+      if (rc) return rc;
+    }
+  } else {
+    launch_sc<T>(ml, g, ps, w, F, pD, make_pen(P), wf && zero_forcing, s);
+    SOPHT_CHECK_LAUNCH();
+  }
   tm.mark(1, s);
   const int64_t n = g.nz * g.ny * g.nx;
   void* const sol[3] = {ps, ps + n, ps + 2 * n};
@@ -168,6 +190,8 @@ int sopht_ns3d_create(int dtype, int64_t nz, int64_t ny, int64_t nx, double x_ra
   P->s2max_cap = int64_t(march_launch(nz, ny, nx, 1, true).blocks);  // VEC 1 has the most
   P->s2_partials = 0;
   P->s2_velocity = nullptr;
+  P->filter_order = -1;
+  P->filter_type = 0;
   if (hipMalloc(&P->partial, P->partial_cap * 8) != hipSuccess ||
       hipMalloc(&P->s2max, P->s2max_cap * 8) != hipSuccess) {
     sopht_ns3d_destroy(P);
@@ -222,10 +246,10 @@ int sopht_ns3d_step(sopht_ns3d* P, void* vort, void* vel, void* psi, void* forci
   SOPHT_REQUIRE(!P->with_forcing || forcing, "forcing field required when with_forcing");
   SOPHT_REQUIRE(!P->with_free_stream || U, "free stream velocity required");
   hipStream_t s = static_cast<hipStream_t>(stream);
-  (void)buffer;  // u x omega1 never leaves the S1 sweep
+  // buffer: u x omega1 never leaves the S1 sweep; the vorticity filter uses it
   return P->dtype == SOPHT_F32
-             ? step_impl<float>(P, vort, vort, vel, vel, psi, forcing, 1, dt, U, s)
-             : step_impl<double>(P, vort, vort, vel, vel, psi, forcing, 1, dt, U, s);
+             ? step_impl<float>(P, vort, vort, vel, vel, psi, forcing, 1, dt, U, s, nullptr, buffer)
+             : step_impl<double>(P, vort, vort, vel, vel, psi, forcing, 1, dt, U, s, nullptr, buffer);
 }
 
 int sopht_ns3d_trajectory_step(sopht_ns3d* P, const void* vort_in, const void* vel_in,
@@ -234,6 +258,7 @@ int sopht_ns3d_trajectory_step(sopht_ns3d* P, const void* vort_in, const void* v
   SOPHT_REQUIRE(P != nullptr, "plan is null");
   SOPHT_REQUIRE(vort_in && vel_in && vort_out && vel_out && psi, "null field pointer");
   SOPHT_REQUIRE(vort_in != vort_out && vel_in != vel_out, "trajectory step is out of place");
+  SOPHT_REQUIRE(P->filter_order < 0, "trajectories (and the adjoint) do not cover the vorticity filter");
   SOPHT_REQUIRE(!P->with_forcing || forcing, "forcing field required when with_forcing");
   SOPHT_REQUIRE(!P->with_free_stream || U, "free stream velocity required");
   hipStream_t s = static_cast<hipStream_t>(stream);
@@ -270,6 +295,14 @@ int sopht_ns3d_max_abs_velocity_sum(sopht_ns3d* P, const void* vel, void* out, v
     k_max_final<double><<<1, 256, 0, s>>>(static_cast<const double*>(P->partial), nb, static_cast<double*>(out));
   }
   SOPHT_CHECK_LAUNCH();
+  return SOPHT_OK;
+}
+
+int sopht_ns3d_set_filter(sopht_ns3d* P, int order, int type) {
+  SOPHT_REQUIRE(P != nullptr, "plan is null");
+  SOPHT_REQUIRE(type == 0 || type == 1, "Invalid filter type");
+  P->filter_order = order < 0 ? -1 : order;
+  P->filter_type = type;
   return SOPHT_OK;
 }
 

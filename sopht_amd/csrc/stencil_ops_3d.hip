@@ -331,7 +331,71 @@ int stretching_impl(void* const flux[3], const int64_t* fs, const void* const w[
       s);
 }
 
+
+// gen_laplacian_filter_kernel_3d on one scalar field (laplacian_filter_3d.py:13-194), with the
+// reference's sequence of kernels: the flux buffer's boundary of width bw is zeroed; each 1D
+// pass writes flux = 0.25 (-f[+1] - f[-1] + 2 f) on [1:-1]^3 (ghost width 1 on every axis,
+// :60-80); the field buffer becomes a full copy of the flux after every pass.
+// type 0 multiplicative (:98-122): order x (x, y, z) passes, then field -= flux.
+// type 1 convolution (:124-166): per axis x, y, z: order passes from a copy of the field, then
+// field -= flux.
+template <typename T>
+int laplacian_filter_impl(int order, int type, void* field, const int64_t* fs, int64_t nz,
+                          int64_t ny, int64_t nx, void* flux, const int64_t* xs, void* fbuf,
+                          const int64_t* bs, int bw, hipStream_t s) {
+  View3<T> f = make_view<T>(field, fs), F = make_view<T>(flux, xs), B = make_view<T>(fbuf, bs);
+  int rc = launch_boundary_slabs(
+      nz, ny, nx, bw, [=] __device__(int64_t z, int64_t y, int64_t x) { F(z, y, x) = T(0); }, s);
+  if (rc) return rc;
+  auto copy = [&](View3<T> dst, View3<T> src) {
+    return launch_box(0, 0, 0, nz, ny, nx,
+                      [=] __device__(int64_t z, int64_t y, int64_t x) { dst(z, y, x) = src(z, y, x); }, s);
+  };
+  auto pass = [&](int axis) {  // F = filter_axis(B) on the interior
+    if (nz <= 2 || ny <= 2 || nx <= 2) return int(SOPHT_OK);
+    const int64_t dz = axis == 2, dy = axis == 1, dx = axis == 0;  // axis 0 = x
+    return launch_box(1, 1, 1, nz - 2, ny - 2, nx - 2,
+                      [=] __device__(int64_t z, int64_t y, int64_t x) {
+                        F(z, y, x) = T(0.25) * ((-B(z + dz, y + dy, x + dx) - B(z - dz, y - dy, x - dx)) +
+                                                T(2) * B(z, y, x));
+                      },
+                      s);
+  };
+  auto subtract = [&]() {  // elementwise_saxpby(field, 1.0 * field + (-1.0) * flux)
+    return launch_box(0, 0, 0, nz, ny, nx,
+                      [=] __device__(int64_t z, int64_t y, int64_t x) {
+                        f(z, y, x) = T(1) * f(z, y, x) + T(-1) * F(z, y, x);
+                      },
+                      s);
+  };
+  if (type == 0) {
+    if ((rc = copy(B, f))) return rc;
+    for (int k = 0; k < order; ++k)
+      for (int axis = 0; axis < 3; ++axis) {
+        if ((rc = pass(axis))) return rc;
+        if ((rc = copy(B, F))) return rc;
+      }
+    return subtract();
+  }
+  for (int axis = 0; axis < 3; ++axis) {
+    if ((rc = copy(B, f))) return rc;
+    for (int k = 0; k < order; ++k) {
+      if ((rc = pass(axis))) return rc;
+      if ((rc = copy(B, F))) return rc;
+    }
+    if ((rc = subtract())) return rc;
+  }
+  return SOPHT_OK;
+}
 }  // namespace
+
+int laplacian_filter_field(int dtype, int order, int type, void* field, const int64_t* fs,
+                           int64_t nz, int64_t ny, int64_t nx, void* flux, const int64_t* xs,
+                           void* fbuf, const int64_t* bs, int bw, hipStream_t s) {
+  if (dtype == SOPHT_F32)
+    return laplacian_filter_impl<float>(order, type, field, fs, nz, ny, nx, flux, xs, fbuf, bs, bw, s);
+  return laplacian_filter_impl<double>(order, type, field, fs, nz, ny, nx, flux, xs, fbuf, bs, bw, s);
+}
 }  // namespace sopht
 
 using namespace sopht;
@@ -549,6 +613,22 @@ int sopht_vorticity_stretching_flux_3d(int dtype, void* const flux[3], const int
                                        int64_t ny, int64_t nx, double p, void* stream) {
   REQUIRE_GRID(nz, ny, nx);
   DISPATCH(dtype, stretching_impl<T>(flux, fs, w, ws, u, us, nz, ny, nx, T(p), as_stream(stream)));
+}
+
+int sopht_laplacian_filter_3d(int dtype, int filter_order, int filter_type, void* field,
+                              const int64_t* field_strides, int64_t nz, int64_t ny, int64_t nx,
+                              void* filter_flux_buffer, const int64_t* flux_strides,
+                              void* field_buffer, const int64_t* buffer_strides,
+                              int flux_boundary_width, void* stream) {
+  REQUIRE_GRID(nz, ny, nx);
+  SOPHT_REQUIRE(filter_order >= 0, "Invalid filter order, must be a non-negative integer");
+  SOPHT_REQUIRE(flux_boundary_width > 0,
+                "Invalid value for filter flux buffer boundary zone, must be a positive integer");
+  SOPHT_REQUIRE(filter_type == 0 || filter_type == 1, "Invalid filter type");
+  SOPHT_REQUIRE(dtype == SOPHT_F32 || dtype == SOPHT_F64, "dtype must be SOPHT_F32 or SOPHT_F64");
+  return laplacian_filter_field(dtype, filter_order, filter_type, field, field_strides, nz, ny, nx,
+                                filter_flux_buffer, flux_strides, field_buffer, buffer_strides,
+                                flux_boundary_width, as_stream(stream));
 }
 
 }  // extern "C"

@@ -558,3 +558,53 @@ def gen_vorticity_stretching_timestep_ssprk3_pyst_kernel_3d(
         saxpby(vorticity_field, vorticity_field, m, 1.0 / 3.0, 2.0 / 3.0)
 
     return vorticity_stretching_timestep_ssprk3_pyst_kernel_3d
+
+
+def gen_laplacian_filter_kernel_3d(
+    filter_order: int,
+    filter_flux_buffer,
+    field_buffer,
+    real_t: type,
+    num_threads: bool | int = False,
+    fixed_grid_size: tuple[int, int, int] | bool = False,
+    field_type: str = "scalar",
+    filter_type: str = "multiplicative",
+    filter_flux_buffer_boundary_width: int = 1,
+) -> Callable:
+    """``laplacian_filter_3d.py:13-194``: 1D Laplacian filters along x, y, z (Jeanmart and
+    Winckelmans 2007), ``"multiplicative"`` or ``"convolution"``, applied in place. The closure
+    keeps the caller's two scalar buffers (so it is not re-entrant across users of the same
+    buffers, as in the reference)."""
+    if not isinstance(filter_order, int) or filter_order < 0:
+        msg = "Invalid filter order, must be a non-negative integer"
+        raise ValueError(msg)
+    if not isinstance(filter_flux_buffer_boundary_width, int) or filter_flux_buffer_boundary_width <= 0:
+        msg = "Invalid value for filter flux buffer boundary zone, must be a positive integer"
+        raise ValueError(msg)
+    types = {"multiplicative": 0, "convolution": 1}
+    if filter_type not in types:
+        msg = "Invalid filter type"
+        raise ValueError(msg)
+    if field_type not in ("scalar", "vector"):
+        msg = "Invalid field type"
+        raise ValueError(msg)
+    tag = L.dtype_tag(real_t)
+    ftype = types[filter_type]
+    width = int(filter_flux_buffer_boundary_width)
+
+    def scalar_field_filter_kernel_3d(scalar_field) -> None:
+        nz, ny, nx = _shape3(scalar_field)
+        L.call(
+            "sopht_laplacian_filter_3d", tag, int(filter_order), ftype, L.ptr(scalar_field),
+            L.strides3(scalar_field), nz, ny, nx, L.ptr(filter_flux_buffer), L.strides3(filter_flux_buffer),
+            L.ptr(field_buffer), L.strides3(field_buffer), width, L.stream_ptr(),
+        )
+
+    if field_type == "scalar":
+        return scalar_field_filter_kernel_3d
+
+    def vector_field_filter_kernel_3d(vector_field) -> None:
+        for i in range(3):
+            scalar_field_filter_kernel_3d(scalar_field=vector_field[i])
+
+    return vector_field_filter_kernel_3d

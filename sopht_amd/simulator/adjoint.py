@@ -111,6 +111,8 @@ class CoupledFlowRollout:
 
         if not flow.with_forcing:
             raise ValueError("the rollout needs a simulator built with forcing")
+        if getattr(flow, "filter_vorticity", False):
+            raise NotImplementedError("the adjoint does not cover the vorticity filter")
         if interactor._ktype != 0:
             raise NotImplementedError("the adjoint supports the cosine interpolation kernel")
         self.flow = flow

@@ -52,6 +52,9 @@ class BatchedFlowSimulation:
             if not flow.with_forcing and inter is not None:
                 msg = "a member with an interactor needs with_forcing=True"
                 raise ValueError(msg)
+            if getattr(flow, "filter_vorticity", False):
+                msg = "the batched step does not cover the vorticity filter"
+                raise NotImplementedError(msg)
             q = arr[i]
             nz, ny, nx = (int(v) for v in flow.grid_size)
             q.nz, q.ny, q.nx = nz, ny, nx

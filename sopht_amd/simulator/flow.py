@@ -98,9 +98,15 @@ class UnboundedNavierStokesFlowSimulator3D(FlowSimulator):
         self.with_free_stream_flow = with_free_stream_flow
         self.penalty_zone_width = kwargs.get("penalty_zone_width", 2)
         self.filter_vorticity = filter_vorticity
+        self.filter_setting_dict = None
         if filter_vorticity:
-            msg = "filter_vorticity=True (Laplacian filter) is not built in this backend yet"
-            raise NotImplementedError(msg)
+            # :274-292: default order 2, multiplicative; custom settings via filter_setting_dict
+            default_filter_setting_dict = {"order": 2, "type": "multiplicative"}
+            self.filter_setting_dict = kwargs.get("filter_setting_dict", default_filter_setting_dict)
+            logger.info(
+                "Vorticity filtering is turned on: order = %s, type = %s",
+                self.filter_setting_dict["order"], self.filter_setting_dict["type"],
+            )
         self.flow_density = flow_density
         supported = ["greens_function_convolution", "fast_diagonalisation"]
         self.poisson_solver_type = poisson_solver_type
@@ -145,6 +151,17 @@ class UnboundedNavierStokesFlowSimulator3D(FlowSimulator):
         )
         self._plan = plan
         self._lib = L.lib()
+        # :396-408 filter applied in the step after diffusion (buffer components 0 and 1)
+        if self.filter_vorticity and self.filter_setting_dict is not None:
+            order = self.filter_setting_dict["order"]
+            ftype = self.filter_setting_dict["type"]
+            if not isinstance(order, int) or order < 0:
+                msg = "Invalid filter order, must be a non-negative integer"
+                raise ValueError(msg)
+            if ftype not in ("multiplicative", "convolution"):
+                msg = "Invalid filter type"
+                raise ValueError(msg)
+            L.call("sopht_ns3d_set_filter", plan, int(order), 0 if ftype == "multiplicative" else 1)
         # standalone drop-ins used outside the fused step
         self._compute_divergence = spne.gen_divergence_pyst_kernel_3d(real_t=self.real_t)
 

@@ -78,3 +78,18 @@ def test_reference_validation_errors():
         spne.gen_penalise_field_boundary_pyst_kernel_3d(-1, 0.1, None, None, None, np.float32)
     with pytest.raises(ValueError):
         spne.gen_set_fixed_val_at_boundaries_pyst_kernel_3d(np.float32, width=0)
+
+
+def test_laplacian_filter_factory_validation():
+    """gen_laplacian_filter_kernel_3d's factory-time ValueErrors (laplacian_filter_3d.py:40-50,
+    :168-174) need no device."""
+    import numpy as np
+
+    from sopht_amd.numeric.eulerian_grid_ops import gen_laplacian_filter_kernel_3d
+
+    for kw in ({"filter_order": -1}, {"filter_order": 1.5}, {"filter_type": "gaussian"},
+               {"filter_flux_buffer_boundary_width": 0}):
+        args = dict(filter_order=1, filter_flux_buffer=None, field_buffer=None, real_t=np.float32)
+        args.update(kw)
+        with pytest.raises(ValueError):
+            gen_laplacian_filter_kernel_3d(**args)

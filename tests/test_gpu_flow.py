@@ -199,3 +199,32 @@ def test_fused_velocity_max_matches_full_reduction(dev, precision, shape, rng):
     assert g.max_abs_velocity_sum() == real_t(full()) == ref
     g.velocity_field.mul_(2.0)  # a change the step did not make
     assert g.max_abs_velocity_sum() == real_t(full()) == real_t(2.0) * ref
+
+
+@pytest.mark.parametrize("filter_type", ["multiplicative", "convolution"])
+@pytest.mark.parametrize("filter_order", [1, 2])
+@pytest.mark.parametrize("precision", ["single", "double"])
+def test_one_step_with_vorticity_filter(dev, filter_type, filter_order, precision, rng):
+    """filter_vorticity=True (navier_stokes_flow_simulators.py:274-292,396-408,473): the fused
+    step with the Laplacian filter between diffusion and penalisation against the oracle."""
+    real_t = PREC[precision]
+    shape = (16, 12, 16)
+    setting = {"order": filter_order, "type": filter_type}
+    kw = dict(kinematic_viscosity=1e-2, time=0.0, with_forcing=True, with_free_stream_flow=True)
+    o = OF.UnboundedNavierStokesFlowSimulator3D(shape, 1.0, real_t=np.float64, filter_setting_dict=setting, **kw)
+    g = sps().UnboundedNavierStokesFlowSimulator3D(grid_size=shape, x_range=1.0, real_t=real_t,
+                                                  filter_vorticity=True, filter_setting_dict=setting, **kw)
+    w0, u0, F0 = (rng.random((3,) + shape).astype(real_t) for _ in range(3))
+    o.vorticity_field[...] = w0
+    o.velocity_field[...] = u0
+    o.eul_grid_forcing_field[...] = F0
+    g.vorticity_field.copy_(to_dev(w0, dev))
+    g.velocity_field.copy_(to_dev(u0, dev))
+    g.eul_grid_forcing_field.copy_(to_dev(F0, dev))
+    U = np.array([1.0, 0.5, -0.25])
+    o.time_step(dt=1e-3, free_stream_velocity=U)
+    g.time_step(dt=1e-3, free_stream_velocity=U)
+    bound = 1e-10 if real_t == np.float64 else 2e-5
+    for name in ("vorticity_field", "velocity_field", "stream_func_field"):
+        a, b = to_host(getattr(g, name)), getattr(o, name)
+        assert rel_l2(a, b) <= bound, (name, rel_l2(a, b))

@@ -197,3 +197,28 @@ def test_vorticity_stretching(dev, real_t, rng):
     mid = to_dev(np.zeros_like(w), dev)
     spne().gen_vorticity_stretching_timestep_ssprk3_pyst_kernel_3d(real_t, mid)(wd, to_dev(u, dev), to_dev(np.zeros_like(w), dev), real_t(0.1))
     np.testing.assert_allclose(to_host(wd), r3, atol=10 * tol(real_t), rtol=0)
+
+
+@pytest.mark.parametrize("field_type", ["scalar", "vector"])
+@pytest.mark.parametrize("filter_type", ["multiplicative", "convolution"])
+@pytest.mark.parametrize("filter_order", [0, 1, 2])
+@pytest.mark.parametrize("shape", SHAPES)
+def test_laplacian_filter(dev, real_t, field_type, filter_type, filter_order, shape, rng):
+    """gen_laplacian_filter_kernel_3d (laplacian_filter_3d.py:13-194) against the oracle on a
+    random field (the reference's test_laplacian_filter_random_field), including the two
+    buffers' final contents; order 0 subtracts whatever the flux buffer's interior holds."""
+    full = (3, *shape) if field_type == "vector" else shape
+    f = rng.random(full).astype(real_t)
+    flux0 = rng.random(shape).astype(real_t)
+    fb0 = rng.random(shape).astype(real_t)
+    ref, rflux, rfb = f.copy(), flux0.copy(), fb0.copy()
+    comps = [ref] if field_type == "scalar" else [ref[i] for i in range(3)]
+    for c in comps:
+        S.laplacian_filter(c, filter_order, filter_type, rflux, rfb)
+    g, gflux, gfb = to_dev(f, dev), to_dev(flux0, dev), to_dev(fb0, dev)
+    spne().gen_laplacian_filter_kernel_3d(
+        filter_order=filter_order, filter_flux_buffer=gflux, field_buffer=gfb, real_t=real_t,
+        field_type=field_type, filter_type=filter_type)(g)
+    np.testing.assert_allclose(to_host(g), ref, atol=tol(real_t), rtol=0)
+    np.testing.assert_allclose(to_host(gflux), rflux, atol=tol(real_t), rtol=0)
+    np.testing.assert_allclose(to_host(gfb), rfb, atol=tol(real_t), rtol=0)

@@ -178,3 +178,50 @@ def test_vorticity_stretching_linear_velocity():
     for c in range(3):
         np.testing.assert_allclose(flux[c][1:-1, 1:-1, 1:-1], np.dot(g[c], [1.0, 2.0, 3.0]), atol=1e-12)
         assert np.all(flux[c][0] == 0)
+
+
+# ---------------------------------------------------------------- Laplacian filter (A7)
+# Pins of the oracle's laplacian_filter (laplacian_filter_3d.py:13-194):
+# * a constant field is unchanged (the reference's test_laplacian_filter_constant_field,
+#   test_laplacian_filter_3d.py:97-131);
+# * a separable cosine mode cos(a x) cos(b y) cos(c z) is an eigenfunction of each 1D pass
+#   away from the borders, with factor s_i = (1 - cos(k_i h)) / 2: far from the border the
+#   multiplicative filter of order k leaves (1 - (s_x s_y s_z)^k) f and the convolution
+#   filter (1 - s_x^k)(1 - s_y^k)(1 - s_z^k) f (a sign, coefficient, axis or order mistake
+#   changes these numbers);
+# * border cells are never changed (the flux is zero there).
+
+
+def _mode(n, kx, ky, kz):
+    i = np.arange(n)
+    ax, ay, az = 2 * np.pi * kx / n, 2 * np.pi * ky / n, 2 * np.pi * kz / n
+    f = np.cos(az * i)[:, None, None] * np.cos(ay * i)[None, :, None] * np.cos(ax * i)[None, None, :]
+    s = [(1 - np.cos(a)) / 2 for a in (ax, ay, az)]
+    return f, s
+
+
+@pytest.mark.parametrize("filter_type", ["multiplicative", "convolution"])
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_laplacian_filter_cosine_mode_closed_form(filter_type, order):
+    n = 32
+    f, (sx, sy, sz) = _mode(n, 3, 5, 2)
+    g = f.copy()
+    S.laplacian_filter(g, order, filter_type, np.zeros_like(f), np.zeros_like(f))
+    if filter_type == "multiplicative":
+        factor = 1 - (sx * sy * sz) ** order
+    else:
+        factor = (1 - sx**order) * (1 - sy**order) * (1 - sz**order)
+    m = 3 * order + 1  # the border's influence travels one cell per pass
+    c = (slice(m, -m),) * 3
+    np.testing.assert_allclose(g[c], factor * f[c], rtol=0, atol=1e-12)
+    for sl in S._boundary_slabs(1):
+        np.testing.assert_array_equal(g[sl], f[sl])
+
+
+@pytest.mark.parametrize("filter_type", ["multiplicative", "convolution"])
+@pytest.mark.parametrize("order", [0, 1, 2])
+def test_laplacian_filter_constant_field_unchanged(filter_type, order):
+    f = 2 * np.ones((16, 16, 16))
+    g = f.copy()
+    S.laplacian_filter(g, order, filter_type, np.zeros_like(f), np.zeros_like(f))
+    np.testing.assert_allclose(g, f, atol=1e-14)
