@@ -14,6 +14,7 @@ from .flow import (
     compute_advection_diffusion_stable_timestep,
     create_unbounded_flow_simulator_3d,
 )
+from .passive_transport import PassiveTransportFlowSimulator
 from .immersed_body import (
     FibonacciEllipsoidForcingGrid,
     FibonacciSphereForcingGrid,
@@ -37,6 +38,7 @@ __all__ = [
     "FlowSimulator",
     "ImmersedBodyFlowInteraction",
     "ImmersedBodyForcingGrid",
+    "PassiveTransportFlowSimulator",
     "RigidBody",
     "RigidBodyFlowInteraction",
     "SphereForcingGrid",

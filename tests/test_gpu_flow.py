@@ -11,7 +11,7 @@ oracle on the same seeded inputs.
 import numpy as np
 import pytest
 
-from helpers import PREC, rel_l2, rel_linf, to_dev, to_host
+from helpers import PREC, rel_l2, rel_linf, to_dev, to_host, tol
 from oracle import flow as OF
 from sopht_amd.utils import synthetic
 
@@ -228,3 +228,48 @@ def test_one_step_with_vorticity_filter(dev, filter_type, filter_order, precisio
     for name in ("vorticity_field", "velocity_field", "stream_func_field"):
         a, b = to_host(getattr(g, name)), getattr(o, name)
         assert rel_l2(a, b) <= bound, (name, rel_l2(a, b))
+
+
+@pytest.mark.parametrize("grid_size_x", [4, 8, 16])
+@pytest.mark.parametrize("precision", ["single", "double"])
+@pytest.mark.parametrize("field_type", ["scalar", "vector"])
+def test_passive_transport_time_step(dev, grid_size_x, precision, field_type, rng):
+    """PassiveTransportFlowSimulator.time_step against the manual ENO3 advection + diffusion
+    sequence of the oracle (test_passive_transport_flow_simulators.py:8-88)."""
+    from oracle import stencils as OS
+
+    real_t = PREC[precision]
+    nu, dt, init_time = 1e-2, 2.0, 1.0
+    grid = (grid_size_x,) * 3
+    dx = real_t(1.0 / grid_size_x)
+    sim = sps().PassiveTransportFlowSimulator(
+        kinematic_viscosity=nu, grid_dim=3, grid_size=grid, x_range=1.0, real_t=real_t,
+        time=init_time, field_type=field_type)
+    u = rng.random(tuple(sim.velocity_field.shape)).astype(real_t)
+    f = rng.random(tuple(sim.primary_field.shape)).astype(real_t)
+    sim.velocity_field.copy_(to_dev(u, dev))
+    sim.primary_field.copy_(to_dev(f, dev))
+    sim.time_step(dt=dt)
+    ref = f.copy()
+    buf = np.zeros(grid, dtype=real_t)
+    OS.advection_timestep_euler_forward_conservative_eno3(ref, buf, u, real_t(dt / dx))
+    OS.diffusion_timestep_euler_forward(ref, buf, real_t(nu * dt / dx / dx))
+    assert sim.time == init_time + dt
+    # dt/dx up to 32 and nu dt/dx^2 up to 5.1 amplify rounding (values grow to ~50): the
+    # reference's tolerance 1e3 eps, relative and absolute
+    np.testing.assert_allclose(to_host(sim.primary_field), ref, rtol=tol(real_t), atol=tol(real_t))
+
+
+@pytest.mark.parametrize("grid_size_x", [4, 8])
+@pytest.mark.parametrize("precision", ["single", "double"])
+def test_passive_transport_stable_timestep(dev, grid_size_x, precision):
+    """test_passive_transport_flow_simulators.py:91-124: velocity 2 everywhere, cfl 0.2."""
+    real_t = PREC[precision]
+    nu, cfl, dt_prefac = 1e-2, 0.2, 0.5
+    dx = real_t(1.0 / grid_size_x)
+    sim = sps().PassiveTransportFlowSimulator(kinematic_viscosity=nu, cfl=cfl, grid_dim=3,
+                                             grid_size=(grid_size_x,) * 3, x_range=1.0, real_t=real_t)
+    sim.velocity_field.fill_(2.0)
+    tol = 10 * np.finfo(real_t).eps
+    ref = dt_prefac * min(cfl * dx / (3 * 2.0 + tol), 0.9 * dx**2 / (2 * 3) / nu)
+    assert sim.compute_stable_timestep(dt_prefac=dt_prefac) == ref
