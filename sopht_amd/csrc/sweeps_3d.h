@@ -69,8 +69,9 @@ __device__ __forceinline__ void s1_body(Grid g, const T* w, const T* u, const T*
   }
   const bool ev = tid < S::NEDGE && x0 + ex >= 0 && x0 + ex < g.nx && y0 + ey >= 0 && y0 + ey < g.ny;
 
-  // omega1 and B = u x omega1 at (x0 + xr, y0 + yr, q)
-  auto produce = [&](int xr, int yr, int q, T b[3], T a[3]) {
+  // omega1 and B = u x omega1 at (x0 + xr, y0 + yr, q); cf: curl_h F already formed (tile
+  // cells), or null (edge-ring cells read F's neighbours)
+  auto produce = [&](int xr, int yr, int q, T b[3], T a[3], const T* cf) {
     const int x = x0 + xr, y = y0 + yr;
     const int j = q * sz + y * sy + x;
     a[0] = w[j];
@@ -78,12 +79,19 @@ __device__ __forceinline__ void s1_body(Grid g, const T* w, const T* u, const T*
     a[2] = w[2 * n + j];
     const T u0 = u[j], u1 = u[n + j], u2 = u[2 * n + j];
     if (with_forcing && g.interior(q, y, x)) {
-      const T* F0 = F;
-      const T* F1 = F + n;
-      const T* F2 = F + 2 * n;
-      const T c0 = F2[j + sy] - F2[j - sy] - F1[j + sz] + F1[j - sz];
-      const T c1 = F0[j + sz] - F0[j - sz] - F2[j + 1] + F2[j - 1];
-      const T c2 = F1[j + 1] - F1[j - 1] - F0[j + sy] + F0[j - sy];
+      T c0, c1, c2;
+      if (cf) {
+        c0 = cf[0];
+        c1 = cf[1];
+        c2 = cf[2];
+      } else {
+        const T* F0 = F;
+        const T* F1 = F + n;
+        const T* F2 = F + 2 * n;
+        c0 = F2[j + sy] - F2[j - sy] - F1[j + sz] + F1[j - sz];
+        c1 = F0[j + sz] - F0[j - sz] - F2[j + 1] + F2[j - 1];
+        c2 = F1[j + 1] - F1[j - 1] - F0[j + sy] + F0[j - sy];
+      }
       a[0] = a[0] + pF * c0;
       a[1] = a[1] + pF * c1;
       a[2] = a[2] + pF * c2;
@@ -96,19 +104,52 @@ __device__ __forceinline__ void s1_body(Grid g, const T* w, const T* u, const T*
 
   T bm2[3] = {T(0), T(0), T(0)}, bm1[3] = {T(0), T(0), T(0)}, a1prev[3] = {T(0), T(0), T(0)};
   const int gt = (ty + 1) * GX + tx + 1;
+  // curl_h F of the tile cells: F0 and F1 of the lane's own column ride along z in registers,
+  // the x neighbours of F1 and F2 come from the adjacent lanes of the row (a tile row is one
+  // wavefront), the y neighbours are row loads
+  static_assert(TX == 64, "a tile row must be one wavefront");
+  const int xt = x0 + tx, yt = y0 + ty;
+  const int colj = yt * sy + xt;
+  auto fown = [&](const T* f, int q) -> T {
+    return (with_forcing && tv && q >= 0 && q < g.nz) ? f[q * sz + colj] : T(0);
+  };
+  T f0m = fown(F, z0 - 2), f0c = fown(F, z0 - 1), f1m = fown(F + n, z0 - 2), f1c = fown(F + n, z0 - 1);
   for (int i = z0 - 1; i <= z1; ++i) {
     T b0[3] = {T(0), T(0), T(0)}, a1[3] = {T(0), T(0), T(0)};
+    const T f0p = fown(F, i + 1), f1p = fown(F + n, i + 1);
+    const T f2c = fown(F + 2 * n, i);
+    T f2r = __shfl_down(f2c, 1, 64), f2l = __shfl_up(f2c, 1, 64);
+    T f1r = __shfl_down(f1c, 1, 64), f1l = __shfl_up(f1c, 1, 64);
+    T cf[3] = {T(0), T(0), T(0)};
+    if (with_forcing && tv && i >= 0 && i < g.nz && g.interior(i, yt, xt)) {
+      const int j = i * sz + colj;  // interior: every neighbour below is inside the grid
+      if (tx == 0) {
+        f2l = F[2 * n + j - 1];
+        f1l = F[n + j - 1];
+      }
+      if (tx == TX - 1) {
+        f2r = F[2 * n + j + 1];
+        f1r = F[n + j + 1];
+      }
+      cf[0] = F[2 * n + j + sy] - F[2 * n + j - sy] - f1p + f1m;
+      cf[1] = f0p - f0m - f2r + f2l;
+      cf[2] = f1r - f1l - F[j + sy] + F[j - sy];
+    }
+    f0m = f0c;
+    f0c = f0p;
+    f1m = f1c;
+    f1c = f1p;
     if (i >= 0 && i < g.nz) {
       T(*Bq)[GP] = bs[i % 3];
       if (tv) {
-        produce(tx, ty, i, b0, a1);
+        produce(tx, ty, i, b0, a1, cf);
         Bq[0][gt] = b0[0];
         Bq[1][gt] = b0[1];
         Bq[2][gt] = b0[2];
       }
       if (ev) {
         T eb[3], ea[3];
-        produce(ex, ey, i, eb, ea);
+        produce(ex, ey, i, eb, ea, nullptr);
         const int ge = (ey + 1) * GX + ex + 1;
         Bq[0][ge] = eb[0];
         Bq[1][ge] = eb[1];
