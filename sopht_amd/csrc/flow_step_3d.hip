@@ -57,14 +57,18 @@ int step_impl(sopht_ns3d* P, const void* vort_in, void* vort_out, const void* ve
   const T pC = T(0.5 / double(dx));
   StageTimer& tm = P->timer;
   tm.begin(s);
+  using ST = S1Tile<T>;
+  const int s1_tx = int(ceil_div(g.nx, ST::TX)), s1_ty = int(ceil_div(g.ny, ST::TY));
+  const bool flags_ok = int64_t(s1_tx) * s1_ty * ceil_div(g.nz, kS1Chunk) <= P->s1flags_cap;
+  const int* fflags = flags_ok ? P->s1flags : nullptr;
   {
-    using ST = S1Tile<T>;
     const int zchunk = kS1Chunk;
-    const int tiles_x = int(ceil_div(g.nx, ST::TX)), tiles_y = int(ceil_div(g.ny, ST::TY));
+    const int tiles_x = s1_tx, tiles_y = s1_ty;
     const int64_t items = int64_t(tiles_x) * tiles_y * ceil_div(g.nz, zchunk);
     k_s1<T><<<unsigned(items), ST::NT, 0, s>>>(g, w_in, u_in, F, ps, pF, pB, wf, zchunk, tiles_x,
                                               tiles_y, items % 8 == 0 ? 1 : 0,
-                                              static_cast<T*>(vort1_out));
+                                              static_cast<T*>(vort1_out),
+                                              items <= P->s1flags_cap ? P->s1flags : nullptr);
     SOPHT_CHECK_LAUNCH();
   }
   tm.mark(0, s);
@@ -74,7 +78,7 @@ int step_impl(sopht_ns3d* P, const void* vort_in, void* vort_out, const void* ve
     // penalisation (navier_stokes_flow_simulators.py:466-474)
     Pen none = make_pen(P);
     none.w = 0;
-    launch_sc<T>(ml, g, ps, w, F, pD, none, wf && zero_forcing, s);
+    launch_sc<T>(ml, g, ps, w, F, pD, none, wf && zero_forcing, s, fflags, s1_tx, s1_ty);
     SOPHT_CHECK_LAUNCH();
     const int64_t n = g.nz * g.ny * g.nx;
     const int64_t st[3] = {g.ny * g.nx, g.nx, 1};
@@ -90,7 +94,7 @@ int step_impl(sopht_ns3d* P, const void* vort_in, void* vort_out, const void* ve
       if (rc) return rc;
     }
   } else {
-    launch_sc<T>(ml, g, ps, w, F, pD, make_pen(P), wf && zero_forcing, s);
+    launch_sc<T>(ml, g, ps, w, F, pD, make_pen(P), wf && zero_forcing, s, fflags, s1_tx, s1_ty);
     SOPHT_CHECK_LAUNCH();
   }
   tm.mark(1, s);
@@ -192,8 +196,14 @@ int sopht_ns3d_create(int dtype, int64_t nz, int64_t ny, int64_t nx, double x_ra
   P->s2_velocity = nullptr;
   P->filter_order = -1;
   P->filter_type = 0;
+  {
+    const int64_t i32 = ceil_div(nx, S1Tile<float>::TX) * ceil_div(ny, S1Tile<float>::TY) * ceil_div(nz, kS1Chunk);
+    const int64_t i64 = ceil_div(nx, S1Tile<double>::TX) * ceil_div(ny, S1Tile<double>::TY) * ceil_div(nz, kS1Chunk);
+    P->s1flags_cap = dtype == SOPHT_F32 ? i32 : i64;
+  }
   if (hipMalloc(&P->partial, P->partial_cap * 8) != hipSuccess ||
-      hipMalloc(&P->s2max, P->s2max_cap * 8) != hipSuccess) {
+      hipMalloc(&P->s2max, P->s2max_cap * 8) != hipSuccess ||
+      hipMalloc(&P->s1flags, P->s1flags_cap * sizeof(int)) != hipSuccess) {
     sopht_ns3d_destroy(P);
     set_last_error("hipMalloc of the reduction scratch failed", __FILE__, __LINE__);
     return SOPHT_ERR_HIP;
@@ -235,6 +245,7 @@ int sopht_ns3d_destroy(sopht_ns3d* P) {
   sopht_poisson3d_destroy(P->poisson);
   if (P->partial) (void)hipFree(P->partial);
   if (P->s2max) (void)hipFree(P->s2max);
+  if (P->s1flags) (void)hipFree(P->s1flags);
   delete P;
   return SOPHT_OK;
 }

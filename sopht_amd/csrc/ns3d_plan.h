@@ -18,6 +18,8 @@ struct sopht_ns3d {
   int64_t s2max_cap;
   int64_t s2_partials; // how many (0: none)
   const void* s2_velocity;  // the velocity field that S2 wrote
+  int* s1flags;        // per S1 tile: did it see a bit of F set (SC zeroes F only there)
+  int64_t s1flags_cap;
   int filter_order;    // vorticity filter: order (< 0: off) and type (0 multiplicative, 1 convolution)
   int filter_type;
   sopht::StageTimer timer;

@@ -40,10 +40,15 @@ struct S1Tile {
   static constexpr int NEDGE = 2 * (TX + TY);                    // edge-ring items (no corners)
 };
 
+// true when any bit of v is set (-0.0 and NaN count)
+__device__ __forceinline__ bool nonzero_bits(float v) { return __float_as_uint(v) != 0u; }
+__device__ __forceinline__ bool nonzero_bits(double v) { return __double_as_longlong(v) != 0ll; }
+
 template <typename T>
 __device__ __forceinline__ void s1_body(Grid g, const T* w, const T* u, const T* F, T* w2out, T pF,
                                         T pB, int with_forcing, int zchunk, int tiles_x,
-                                        int tiles_y, int64_t item, T* w1out = nullptr) {
+                                        int tiles_y, int64_t item, T* w1out = nullptr,
+                                        int* fflags = nullptr) {
   using S = S1Tile<T>;
   constexpr int TX = S::TX, TY = S::TY, GX = S::GX, GP = S::GP;
   __shared__ T bs[3][3][GP];
@@ -114,10 +119,12 @@ __device__ __forceinline__ void s1_body(Grid g, const T* w, const T* u, const T*
     return (with_forcing && tv && q >= 0 && q < g.nz) ? f[q * sz + colj] : T(0);
   };
   T f0m = fown(F, z0 - 2), f0c = fown(F, z0 - 1), f1m = fown(F + n, z0 - 2), f1c = fown(F + n, z0 - 1);
+  bool fset = false;  // some bit of F set in this lane's column (the tile's SC zeroes F then)
   for (int i = z0 - 1; i <= z1; ++i) {
     T b0[3] = {T(0), T(0), T(0)}, a1[3] = {T(0), T(0), T(0)};
     const T f0p = fown(F, i + 1), f1p = fown(F + n, i + 1);
     const T f2c = fown(F + 2 * n, i);
+    fset = fset || nonzero_bits(f0p) || nonzero_bits(f1p) || nonzero_bits(f2c);
     T f2r = __shfl_down(f2c, 1, 64), f2l = __shfl_up(f2c, 1, 64);
     T f1r = __shfl_down(f1c, 1, 64), f1l = __shfl_up(f1c, 1, 64);
     T cf[3] = {T(0), T(0), T(0)};
@@ -186,6 +193,10 @@ __device__ __forceinline__ void s1_body(Grid g, const T* w, const T* u, const T*
       a1prev[c] = a1[c];
     }
   }
+  if (fflags) {  // per tile (logical item index): does the tile's F need zeroing
+    const int any = __syncthreads_or(fset ? 1 : 0);
+    if (tid == 0) fflags[item] = any;
+  }
 }
 
 // Workgroups are dealt to the 8 XCDs in turn; with the remap, XCD k takes a contiguous run of
@@ -199,9 +210,9 @@ __device__ __forceinline__ int64_t xcd_item(int remap) {
 template <typename T>
 __global__ void __launch_bounds__(S1Tile<T>::NT) __attribute__((amdgpu_waves_per_eu(6)))
 k_s1(Grid g, const T* w, const T* u, const T* F, T* w2out, T pF, T pB, int with_forcing,
-     int zchunk, int tiles_x, int tiles_y, int remap, T* w1out) {
+     int zchunk, int tiles_x, int tiles_y, int remap, T* w1out, int* fflags) {
   s1_body<T>(g, w, u, F, w2out, pF, pB, with_forcing, zchunk, tiles_x, tiles_y, xcd_item(remap),
-             w1out);
+             w1out, fflags);
 }
 
 struct Pen {
@@ -313,9 +324,16 @@ template <typename T, int VEC>
 __device__ __forceinline__ void sc_body(Grid g, const T* __restrict__ w2, T* __restrict__ w,
                                         T* __restrict__ F, T pD, const Pen& pen,
                                         int zero_forcing, int tiles_x, int tiles_y,
-                                        int64_t item, int chunk = kMarchChunk) {
+                                        int64_t item, int chunk = kMarchChunk,
+                                        const int* fflags = nullptr, int s1_tiles_x = 0,
+                                        int s1_tiles_y = 0) {
   using V = VecT<T, VEC>;
   const MarchTile m = march_tile<VEC>(g, tiles_x, tiles_y, item, chunk);
+  if (zero_forcing && fflags && m.x0 < int(g.nx) && m.y < int(g.ny)) {
+    // the S1 tile holding these cells saw no bit of F set: F is zero there already
+    const int t = ((m.z0 / kS1Chunk) * s1_tiles_y + m.y / S1Tile<T>::TY) * s1_tiles_x + m.x0 / S1Tile<T>::TX;
+    if (fflags[t] == 0) zero_forcing = 0;
+  }
   const int nz = int(g.nz), ny = int(g.ny), nx = int(g.nx);
   const int n = nz * ny * nx, sz = ny * nx, sy = nx;
   const bool act = m.x0 < nx && m.y < ny;  // whole vectors (nx % VEC == 0)
@@ -418,8 +436,11 @@ template <typename T, int VEC>
 __global__ void __launch_bounds__(kMarchThreads) k_sc(Grid g, const T* __restrict__ w2,
                                                       T* __restrict__ w, T* __restrict__ F, T pD,
                                                       Pen pen, int zero_forcing, int tiles_x,
-                                                      int tiles_y, int remap, int chunk) {
-  sc_body<T, VEC>(g, w2, w, F, pD, pen, zero_forcing, tiles_x, tiles_y, xcd_item(remap), chunk);
+                                                      int tiles_y, int remap, int chunk,
+                                                      const int* fflags, int s1_tiles_x,
+                                                      int s1_tiles_y) {
+  sc_body<T, VEC>(g, w2, w, F, pD, pen, zero_forcing, tiles_x, tiles_y, xcd_item(remap), chunk,
+                  fflags, s1_tiles_x, s1_tiles_y);
 }
 
 // S2: u = p curl_h(psi) (interior), 0 on the border, + U_inf.
@@ -520,13 +541,14 @@ inline int march_vec(int64_t nx, int width) {
 
 template <typename T>
 void launch_sc(const MarchLaunch& l, Grid g, const T* w2, T* w, T* F, T pD, const Pen& pen, int wf,
-               hipStream_t s) {
+               hipStream_t s, const int* fflags = nullptr, int s1_tiles_x = 0, int s1_tiles_y = 0) {
   if (l.vec == 1)
     k_sc<T, 1><<<l.blocks, kMarchThreads, 0, s>>>(g, w2, w, F, pD, pen, wf, l.tiles_x, l.tiles_y, l.remap,
-                                                  l.chunk);
+                                                  l.chunk, fflags, s1_tiles_x, s1_tiles_y);
   else
     k_sc<T, 16 / sizeof(T)><<<l.blocks, kMarchThreads, 0, s>>>(g, w2, w, F, pD, pen, wf, l.tiles_x,
-                                                                l.tiles_y, l.remap, l.chunk);
+                                                                l.tiles_y, l.remap, l.chunk, fflags,
+                                                                s1_tiles_x, s1_tiles_y);
 }
 
 template <typename T>
