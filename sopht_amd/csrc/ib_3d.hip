@@ -153,7 +153,7 @@ int sopht_lag_to_eul_spread_3d(int dtype, int ncomp, sopht_spread3d* ws, void* c
     SOPHT_CHECK_LAUNCH();
     for (int64_t k = 2; k <= padded; k <<= 1)
       for (int64_t j = k >> 1; j > 0; j >>= 1) {
-        k_bitonic_step<<<blocks_for(padded, 256), 256, 0, s>>>(ws->keys, padded, j, k);
+        k_bitonic_step<<<blocks_for(padded, 256), 256, 0, s>>>(ws->keys, padded, j, k, ws->flag);
         SOPHT_CHECK_LAUNCH();
       }
   }
@@ -166,16 +166,21 @@ int sopht_lag_to_eul_spread_3d(int dtype, int ncomp, sopht_spread3d* ws, void* c
     k_spread_owner<false><<<blocks_for(n * 64, 512), 512, 0, s>>>(nz, ny, nx, ws->keys, n, ws->flag,
                                                                   ws->pcount, ws->owners);
   SOPHT_CHECK_LAUNCH();
-  k_spread_merge<<<std::min(2048, blocks_for(n * 64, 64 * kMergeWaves)), 64 * kMergeWaves, 0, s>>>(
-      nz, ny, nx, ws->keys, n, ws->flag, ws->pcount, ws->poff, ws->owners, ws->plist);
+  const int mblocks = std::min(2048, blocks_for(n * 64, 64 * kMergeWaves));
+  if (n <= 8192)  // sorted keys in LDS (64 KiB) beside the merge stage
+    k_spread_merge<true><<<mblocks, 64 * kMergeWaves, n * sizeof(uint64_t), s>>>(
+        nz, ny, nx, ws->keys, n, ws->flag, ws->pcount, ws->poff, ws->owners, ws->plist);
+  else
+    k_spread_merge<false><<<mblocks, 64 * kMergeWaves, 0, s>>>(
+        nz, ny, nx, ws->keys, n, ws->flag, ws->pcount, ws->poff, ws->owners, ws->plist);
   SOPHT_CHECK_LAUNCH();
   IB_DISPATCH(dtype, ([&] {
                 EulW3<T> F;
                 for (int c = 0; c < 3; ++c) F.p[c] = c < ncomp ? static_cast<T*>(eul[c]) : nullptr;
                 F.sz = es[0]; F.sy = es[1]; F.sx = es[2];
-                k_spread_replay<T><<<blocks_for(n * 64, 256), 256, 0, s>>>(
+                k_spread_replay_owners<T><<<std::min(2048, blocks_for(n * 64, 256)), 256, 0, s>>>(
                     ncomp, F, nz, ny, nx, static_cast<const T*>(lag), static_cast<const T*>(weights),
-                    ws->keys, n, ws->pcount, ws->poff, ws->plist, ws->flag);
+                    ws->keys, n, ws->pcount, ws->poff, ws->plist, ws->owners, ws->flag);
               }()));
   SOPHT_CHECK_LAUNCH();
   return SOPHT_OK;

@@ -344,7 +344,10 @@ __global__ void __launch_bounds__(1024) k_bitonic_lds(const uint64_t* prev, uint
   bitonic_block(prev, keys, padded, flag, sk);
 }
 
+// Large-node path of the key sort (only when the keys changed, as bitonic_block): copy, then
+// global compare-exchange steps.
 __global__ void k_copy_keys(const uint64_t* src, uint64_t* dst, int64_t m, int* flag) {
+  if (flag_load(flag + 1) == 0) return;  // plan still valid: keys stay sorted
   const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (i < m) dst[i] = src[i];
   if (i == 0) {
@@ -354,7 +357,8 @@ __global__ void k_copy_keys(const uint64_t* src, uint64_t* dst, int64_t m, int* 
 }
 
 // One compare-exchange step of a global-memory bitonic sort (for large node counts).
-__global__ void k_bitonic_step(uint64_t* keys, int64_t padded, int64_t j, int64_t k) {
+__global__ void k_bitonic_step(uint64_t* keys, int64_t padded, int64_t j, int64_t k, const int* flag) {
+  if (flag_load(flag + 1) == 0) return;
   const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (i >= padded) return;
   const int64_t p = i ^ j;
@@ -574,11 +578,19 @@ __device__ __forceinline__ void spread_merge_owner(int64_t nz, int64_t ny, int64
   }
 }
 
+template <bool LDS_KEYS>
 __global__ void __launch_bounds__(64 * kMergeWaves) k_spread_merge(
-    int64_t nz, int64_t ny, int64_t nx, const uint64_t* keys, int64_t n, int* flag, int* pcount,
+    int64_t nz, int64_t ny, int64_t nx, const uint64_t* gkeys, int64_t n, int* flag, int* pcount,
     int* poff, const uint32_t* owners, uint32_t* plist) {
   __shared__ uint32_t stage[kMergeWaves][kMergeCap];
+  extern __shared__ uint64_t skeys[];
   if (flag_load(flag + 1) == 0) return;  // plan still valid (uniform for the launch)
+  const uint64_t* keys = gkeys;
+  if constexpr (LDS_KEYS) {  // the binary searches then run on LDS
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) skeys[i] = gkeys[i];
+    __syncthreads();
+    keys = skeys;
+  }
   const int lane = int(threadIdx.x & 63);
   const int w = int(threadIdx.x >> 6);
   const int64_t waves = int64_t(gridDim.x) * kMergeWaves;
@@ -588,9 +600,9 @@ __global__ void __launch_bounds__(64 * kMergeWaves) k_spread_merge(
                        stage[w]);
 }
 
-// Plan replay (every call): each owned cell adds its contributors F_lag[node] w[slot][node]
-// in increasing node order, every product and sum rounded separately, exactly as the
-// reference's serial `F[block] += F_n * w_n`. Clears the keys-changed flag for the next call.
+// Plan replay for the batched step (every call; one thread per support slot): each owned cell
+// adds its contributors F_lag[node] w[slot][node] in increasing node order, every product and
+// sum rounded separately, exactly as the reference's serial `F[block] += F_n * w_n`.
 template <typename T>
 __device__ __forceinline__ void spread_replay_slot(int ncomp, const EulW3<T>& F, int64_t nz,
                                                    int64_t ny, int64_t nx, const T* lag,
@@ -617,16 +629,47 @@ __device__ __forceinline__ void spread_replay_slot(int ncomp, const EulW3<T>& F,
   for (int c = 0; c < ncomp; ++c) F.p[c][cidx] = acc[c];
 }
 
+// Plan replay, one wavefront per owned cell (the owner list of the plan build): the lanes load
+// up to 64 contributions at once and form their products F_lag[node] w[slot][node]; the
+// products are then added to the cell's value one after another in increasing node order
+// (every lane keeps the same running sum), each sum rounded separately, exactly as the
+// reference's serial `F[block] += F_n * w_n`. Clears the keys-changed flag for the next call.
 template <typename T>
-__global__ void __launch_bounds__(256) k_spread_replay(int ncomp, EulW3<T> F, int64_t nz,
-                                                       int64_t ny, int64_t nx, const T* lag,
-                                                       const T* weights, const uint64_t* keys,
-                                                       int64_t n, const int* pcount,
-                                                       const int* poff, const uint32_t* plist,
-                                                       int* flag) {
-  const int64_t gid = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-  if (gid == 0) atomicExch(flag + 1, 0);
-  if (gid < 64 * n) spread_replay_slot<T>(ncomp, F, nz, ny, nx, lag, weights, keys, n, pcount, poff, plist, gid);
+__global__ void __launch_bounds__(256) k_spread_replay_owners(
+    int ncomp, EulW3<T> F, int64_t nz, int64_t ny, int64_t nx, const T* lag, const T* weights,
+    const uint64_t* keys, int64_t n, const int* pcount, const int* poff, const uint32_t* plist,
+    const uint32_t* owners, int* flag) {
+#pragma clang fp contract(off)
+  const int lane = int(threadIdx.x & 63);
+  const int64_t waves = int64_t(gridDim.x) * (blockDim.x >> 6);
+  const int64_t count = flag_load(flag + 3);
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicExch(flag + 1, 0);
+  for (int64_t o = int64_t(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6); o < count; o += waves) {
+    const int64_t gid = int64_t(owners[o]);
+    const int m = pcount[gid];
+    const int base = poff[gid];
+    const int64_t cell0 = int64_t(keys[gid >> 6] >> 32);
+    const int slot = int(gid & 63);
+    const int64_t cx = cell0 % nx + (slot & 3), cy = (cell0 / nx) % ny + ((slot >> 2) & 3),
+                  cz = cell0 / (nx * ny) + (slot >> 4);
+    const int64_t cidx = cz * F.sz + cy * F.sy + cx * F.sx;
+    T acc[3] = {T(0), T(0), T(0)};
+    for (int c = 0; c < ncomp; ++c) acc[c] = F.p[c][cidx];
+    for (int k0 = 0; k0 < m; k0 += 64) {
+      T p[3] = {T(0), T(0), T(0)};
+      if (k0 + lane < m) {
+        const uint32_t e = plist[base + k0 + lane];
+        const int64_t node = int64_t(e >> 6);
+        const T w = weights[int64_t(e & 63u) * n + node];
+        for (int c = 0; c < ncomp; ++c) p[c] = lag[c * n + node] * w;
+      }
+      const int cnt = m - k0 < 64 ? m - k0 : 64;
+      for (int j = 0; j < cnt; ++j)
+        for (int c = 0; c < ncomp; ++c) acc[c] = acc[c] + __shfl(p[c], j, 64);
+    }
+    if (lane == 0)
+      for (int c = 0; c < ncomp; ++c) F.p[c][cidx] = acc[c];
+  }
 }
 
 inline int blocks_for(int64_t n, int t) { return int((n + t - 1) / t); }
