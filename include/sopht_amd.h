@@ -165,6 +165,32 @@ int sopht_poisson3d_destroy(sopht_poisson3d* plan);
 /* Bytes of device scratch the plan holds (for capacity planning). */
 int64_t sopht_poisson3d_scratch_bytes(const sopht_poisson3d* plan);
 
+/* ---------------------------------------------------------------- Neumann Poisson (fast diagonalisation) */
+
+typedef struct sopht_fastdiag3d sopht_fastdiag3d;
+
+/* FastDiagPoissonSolver3D.__init__ — .../poisson_solver_3d/FastDiagPoissonSolver3D.py:15-145,
+ * bc_type "homogenous_neumann_along_xyz". The per-axis eigenbasis of the Neumann
+ * second-difference matrix is the cell-centred cosine basis, built in closed form (double),
+ * orthonormal, eigenvalues in decreasing order. 2 <= n <= 2048 per axis. */
+int sopht_fastdiag3d_create(int dtype, int64_t nz, int64_t ny, int64_t nx, double dx, void* stream,
+                            sopht_fastdiag3d** plan);
+/* solve — :147-189: -lap(solution) = rhs with the constant mode removed. Both fields
+ * C-contiguous (nz, ny, nx); rhs is read only; solution may alias rhs. */
+int sopht_fastdiag3d_solve(sopht_fastdiag3d* plan, void* solution, const int64_t* solution_strides,
+                           const void* rhs, const int64_t* rhs_strides, void* stream);
+/* vector_field_solve — :191-202. Components C-contiguous; components spaced nz*ny*nx apart
+ * (a (3, nz, ny, nx) tensor) are solved in one batched sequence. */
+int sopht_fastdiag3d_vector_solve(sopht_fastdiag3d* plan, void* const solution[3],
+                                  const int64_t* solution_strides, const void* const rhs[3],
+                                  const int64_t* rhs_strides, void* stream);
+/* Host copy of one axis' eigenbasis (axis 0 z, 1 y, 2 x): eig_vals (n), eig_vecs row-major
+ * (n, n) with column p the eigenvector of eig_vals[p] (the reference's eig_vecs_*). Either
+ * output may be null. */
+int sopht_fastdiag3d_eigenbasis(const sopht_fastdiag3d* plan, int axis, double* eig_vals,
+                                double* eig_vecs);
+int sopht_fastdiag3d_destroy(sopht_fastdiag3d* plan);
+
 /* ---------------------------------------------------------------- Eulerian <-> Lagrangian */
 
 /* local_eulerian_grid_support_of_lagrangian_grid_kernel_3d —
@@ -242,6 +268,15 @@ int sopht_ns3d_create(int dtype, int64_t nz, int64_t ny, int64_t nx, double x_ra
                       double kinematic_viscosity, double flow_density, int penalty_zone_width,
                       int with_forcing, int with_free_stream_flow, void* stream,
                       sopht_ns3d** plan);
+/* As sopht_ns3d_create with the simulator's poisson_solver_type (:296-358): 0
+ * "greens_function_convolution", 1 "fast_diagonalisation" (FastDiagPoissonSolver3D with the
+ * plan's dx, homogeneous Neumann; any grid of 3 or more cells per axis). Only the chosen
+ * solver's tables are built. The adjoint step needs type 0. */
+int sopht_ns3d_create_with_poisson_solver(int dtype, int64_t nz, int64_t ny, int64_t nx,
+                                          double x_range, double kinematic_viscosity,
+                                          double flow_density, int penalty_zone_width,
+                                          int with_forcing, int with_free_stream_flow,
+                                          int poisson_solver_type, void* stream, sopht_ns3d** plan);
 int sopht_ns3d_destroy(sopht_ns3d* plan);
 
 /* Vorticity filtering of the step (filter_vorticity=True with filter_setting_dict,

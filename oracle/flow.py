@@ -15,7 +15,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from . import ib, poisson, stencils
+from . import fastdiag, ib, poisson, stencils
 
 
 def position_field(grid_size, x_range, real_t):
@@ -41,7 +41,7 @@ def compute_advection_diffusion_stable_timestep(velocity_field, dx, cfl, nu, rea
 
 
 class UnboundedNavierStokesFlowSimulator3D:
-    """Oracle of the reference simulator with the greens-function Poisson solver."""
+    """Oracle of the reference simulator (greens-function or fast-diagonalisation Poisson)."""
 
     def __init__(
         self,
@@ -56,6 +56,7 @@ class UnboundedNavierStokesFlowSimulator3D:
         flow_density=1.0,
         penalty_zone_width=2,
         filter_setting_dict=None,
+        poisson_solver_type="greens_function_convolution",
     ):
         self.grid_size = tuple(grid_size)
         self.x_range = x_range
@@ -78,7 +79,11 @@ class UnboundedNavierStokesFlowSimulator3D:
         self.stream_func_field = np.zeros(shape, dtype=real_t)
         if with_forcing:
             self.eul_grid_forcing_field = np.zeros(shape, dtype=real_t)
-        self.poisson = poisson.UnboundedPoissonSolver3D(*self.grid_size, x_range=x_range, real_t=real_t)
+        # navier_stokes_flow_simulators.py:337-358
+        if poisson_solver_type == "fast_diagonalisation":
+            self.poisson = fastdiag.FastDiagPoissonSolver3D(*self.grid_size, dx=self.dx, real_t=real_t)
+        else:
+            self.poisson = poisson.UnboundedPoissonSolver3D(*self.grid_size, x_range=x_range, real_t=real_t)
 
     # -- pieces of ``_navier_stokes_time_step`` (:449-485)
     def penalise(self, field):

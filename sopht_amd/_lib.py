@@ -80,6 +80,11 @@ SIGNATURES = {
     "sopht_poisson3d_vector_solve": [_vp, _p3, _pi64, _p3, _pi64, _vp],
     "sopht_poisson3d_destroy": [_vp],
     "sopht_poisson3d_scratch_bytes": [_vp],
+    "sopht_fastdiag3d_create": [_i, _i64, _i64, _i64, _d, _vp, C.POINTER(_vp)],
+    "sopht_fastdiag3d_solve": [_vp, _vp, _pi64, _vp, _pi64, _vp],
+    "sopht_fastdiag3d_vector_solve": [_vp, _p3, _pi64, _p3, _pi64, _vp],
+    "sopht_fastdiag3d_eigenbasis": [_vp, _i, _pd, _pd],
+    "sopht_fastdiag3d_destroy": [_vp],
     "sopht_lag_support_3d": [_i, _vp, _vp, _vp, _i64, _d, _d, _vp],
     "sopht_lag_interp_weights_3d": [_i, _i, _vp, _vp, _i64, _d, _vp],
     "sopht_eul_to_lag_interp_3d": [_i, _i, _vp, _p3, _pi64, _i64, _i64, _i64, _vp, _vp, _i64, _d, _vp],
@@ -92,6 +97,8 @@ SIGNATURES = {
         _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "sopht_vbf_time_step_3d": [_i, _vp, _vp, _i64, _d, _vp],
     "sopht_ns3d_create": [_i, _i64, _i64, _i64, _d, _d, _d, _i, _i, _i, _vp, C.POINTER(_vp)],
+    "sopht_ns3d_create_with_poisson_solver": [
+        _i, _i64, _i64, _i64, _d, _d, _d, _i, _i, _i, _i, _vp, C.POINTER(_vp)],
     "sopht_ns3d_destroy": [_vp],
     "sopht_ns3d_step": [_vp, _vp, _vp, _vp, _vp, _vp, _d, _pd, _vp],
     "sopht_ns3d_recover_velocity": [_vp, _vp, _vp, _vp, _pd, _vp],

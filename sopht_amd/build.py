@@ -37,6 +37,11 @@ HIP_FLAGS = [
 ]
 
 
+# rocBLAS runs the dense eigenbasis GEMMs of the fast-diagonalisation Poisson solver.
+ROCM_LIB = "/opt/rocm/lib"
+LINK_LIBS = ["-L", ROCM_LIB, f"-Wl,-rpath,{ROCM_LIB}", "-lrocblas"]
+
+
 def _compile(src: Path) -> Path:
     obj = OBJ / (src.stem + ".o")
     cmd = [HIPCC, *HIP_FLAGS, "-c", str(src), "-o", str(obj)]
@@ -53,7 +58,8 @@ def build(verbose: bool = True) -> Path:
     with concurrent.futures.ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(_compile, sources))
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)]
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs),
+           *LINK_LIBS]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")

@@ -482,6 +482,7 @@ int sopht_ns3d_adjoint_step(sopht_ns3d* P, const void* vort1, const void* vel, d
   SOPHT_REQUIRE(P != nullptr, "plan is null");
   SOPHT_REQUIRE(vort1 && vel && vort_adj && vel_adj && work, "null field pointer");
   SOPHT_REQUIRE(P->filter_order < 0, "the adjoint does not cover the vorticity filter");
+  SOPHT_REQUIRE(P->poisson != nullptr, "the adjoint covers the greens-function Poisson solver only");
   hipStream_t s = static_cast<hipStream_t>(stream);
   return P->dtype == SOPHT_F32
              ? adjoint_step_impl<float>(P, vort1, vel, dt, vort_adj, vel_adj, work, s)

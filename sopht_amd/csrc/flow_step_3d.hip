@@ -102,7 +102,8 @@ int step_impl(sopht_ns3d* P, const void* vort_in, void* vort_out, const void* ve
   void* const sol[3] = {ps, ps + n, ps + 2 * n};
   const void* const rhs[3] = {w, w + n, w + 2 * n};
   const int64_t st[3] = {g.ny * g.nx, g.nx, 1};
-  int rc = poisson_vector_solve_timed(P->poisson, sol, st, rhs, st, s, &tm, 2);
+  int rc = P->fastdiag ? fastdiag_vector_solve_timed(P->fastdiag, sol, st, rhs, st, s, &tm, 2)
+                       : poisson_vector_solve_timed(P->poisson, sol, st, rhs, st, s, &tm, 2);
   if (rc) return rc;
   const T U0 = P->with_free_stream ? T(U[0]) : T(0);
   const T U1 = P->with_free_stream ? T(U[1]) : T(0);
@@ -132,7 +133,8 @@ int recover_impl(sopht_ns3d* P, void* vort, void* vel, void* psi, const double* 
   }
   void* const sol[3] = {ps, ps + n, ps + 2 * n};
   const void* const rhs[3] = {w, w + n, w + 2 * n};
-  int rc = sopht_poisson3d_vector_solve(P->poisson, sol, st, rhs, st, s);
+  int rc = P->fastdiag ? fastdiag_vector_solve_timed(P->fastdiag, sol, st, rhs, st, s, nullptr, 0)
+                       : sopht_poisson3d_vector_solve(P->poisson, sol, st, rhs, st, s);
   if (rc) return rc;
   const T U0 = P->with_free_stream ? T(U[0]) : T(0);
   const T U1 = P->with_free_stream ? T(U[1]) : T(0);
@@ -157,8 +159,18 @@ extern "C" {
 int sopht_ns3d_create(int dtype, int64_t nz, int64_t ny, int64_t nx, double x_range, double nu,
                       double rho, int width, int with_forcing, int with_free_stream,
                       void* stream, sopht_ns3d** out) {
+  return sopht_ns3d_create_with_poisson_solver(dtype, nz, ny, nx, x_range, nu, rho, width,
+                                               with_forcing, with_free_stream, 0, stream, out);
+}
+
+int sopht_ns3d_create_with_poisson_solver(int dtype, int64_t nz, int64_t ny, int64_t nx,
+                                          double x_range, double nu, double rho, int width,
+                                          int with_forcing, int with_free_stream,
+                                          int poisson_solver_type, void* stream, sopht_ns3d** out) {
   SOPHT_REQUIRE(out != nullptr, "plan pointer is null");
   *out = nullptr;
+  SOPHT_REQUIRE(poisson_solver_type == 0 || poisson_solver_type == 1,
+                "poisson_solver_type must be 0 (greens function) or 1 (fast diagonalisation)");
   SOPHT_REQUIRE(dtype == SOPHT_F32 || dtype == SOPHT_F64, "dtype must be SOPHT_F32 or SOPHT_F64");
   SOPHT_REQUIRE(nz >= 3 && ny >= 3 && nx >= 3, "grid must be at least 3 cells per axis");
   SOPHT_REQUIRE(width >= 0 && width <= 8, "penalty_zone_width must be in [0, 8]");
@@ -185,7 +197,9 @@ int sopht_ns3d_create(int dtype, int64_t nz, int64_t ny, int64_t nx, double x_ra
       P->sine[a * 2 * width + i] = std::sin(pre * (double(i) * P->dx));
       P->sine[a * 2 * width + width + i] = std::sin(pre * (double(width - 1 - i) * P->dx));
     }
-  int rc = sopht_poisson3d_create(dtype, nz, ny, nx, x_range, stream, &P->poisson);
+  int rc = poisson_solver_type == 1
+               ? sopht_fastdiag3d_create(dtype, nz, ny, nx, P->dx, stream, &P->fastdiag)
+               : sopht_poisson3d_create(dtype, nz, ny, nx, x_range, stream, &P->poisson);
   if (rc) {
     delete P;
     return rc;
@@ -243,6 +257,7 @@ int sopht_ns3d_destroy(sopht_ns3d* P) {
   if (!P) return SOPHT_OK;
   P->timer.disable();
   sopht_poisson3d_destroy(P->poisson);
+  sopht_fastdiag3d_destroy(P->fastdiag);
   if (P->partial) (void)hipFree(P->partial);
   if (P->s2max) (void)hipFree(P->s2max);
   if (P->s1flags) (void)hipFree(P->s1flags);

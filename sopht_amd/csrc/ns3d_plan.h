@@ -10,7 +10,8 @@ struct sopht_ns3d {
   int64_t nz, ny, nx;
   double x_range, dx, nu, rho;
   int width, with_forcing, with_free_stream;
-  sopht_poisson3d* poisson;
+  sopht_poisson3d* poisson;     // free-space solver (the default); null when fastdiag is set
+  sopht_fastdiag3d* fastdiag;   // Neumann fast-diagonalisation solver, or null
   double sine[6 * 8];  // x, y, z: width front factors then width back factors
   void* partial;       // per-block partial maxima for the CFL reduction
   int64_t partial_cap;

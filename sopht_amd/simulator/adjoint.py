@@ -113,6 +113,8 @@ class CoupledFlowRollout:
             raise ValueError("the rollout needs a simulator built with forcing")
         if getattr(flow, "filter_vorticity", False):
             raise NotImplementedError("the adjoint does not cover the vorticity filter")
+        if getattr(flow, "poisson_solver_type", "greens_function_convolution") != "greens_function_convolution":
+            raise NotImplementedError("the adjoint covers the greens-function Poisson solver only")
         if interactor._ktype != 0:
             raise NotImplementedError("the adjoint supports the cosine interpolation kernel")
         self.flow = flow

@@ -55,6 +55,9 @@ class BatchedFlowSimulation:
             if getattr(flow, "filter_vorticity", False):
                 msg = "the batched step does not cover the vorticity filter"
                 raise NotImplementedError(msg)
+            if getattr(flow, "poisson_solver_type", "greens_function_convolution") != "greens_function_convolution":
+                msg = "the batched step covers the greens-function Poisson solver only"
+                raise NotImplementedError(msg)
             q = arr[i]
             nz, ny, nx = (int(v) for v in flow.grid_size)
             q.nz, q.ny, q.nx = nz, ny, nx

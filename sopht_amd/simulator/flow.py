@@ -113,9 +113,6 @@ class UnboundedNavierStokesFlowSimulator3D(FlowSimulator):
         if poisson_solver_type not in supported:
             msg = "Invalid Poisson solver type given"
             raise ValueError(msg)
-        if poisson_solver_type == "fast_diagonalisation":
-            msg = "poisson_solver_type='fast_diagonalisation' is outside this backend"
-            raise NotImplementedError(msg)
         super().__init__(
             grid_dim=3, grid_size=grid_size, x_range=x_range, real_t=real_t,
             num_threads=num_threads, time=time,
@@ -143,11 +140,13 @@ class UnboundedNavierStokesFlowSimulator3D(FlowSimulator):
 
         nz, ny, nx = self.grid_size
         plan = C.c_void_p()
+        # :337-358: the free-space solver, or the Neumann fast-diagonalisation one
+        solver_type = 1 if self.poisson_solver_type == "fast_diagonalisation" else 0
         L.call(
-            "sopht_ns3d_create", L.dtype_tag(self.real_t), nz, ny, nx, float(self.x_range),
-            float(self.kinematic_viscosity), float(self.flow_density), int(self.penalty_zone_width),
-            int(bool(self.with_forcing)), int(bool(self.with_free_stream_flow)), L.stream_ptr(),
-            C.byref(plan),
+            "sopht_ns3d_create_with_poisson_solver", L.dtype_tag(self.real_t), nz, ny, nx,
+            float(self.x_range), float(self.kinematic_viscosity), float(self.flow_density),
+            int(self.penalty_zone_width), int(bool(self.with_forcing)),
+            int(bool(self.with_free_stream_flow)), solver_type, L.stream_ptr(), C.byref(plan),
         )
         self._plan = plan
         self._lib = L.lib()

@@ -55,6 +55,13 @@ def test_error_paths_need_no_gpu():
     assert lib.sopht_poisson3d_create(7, 16, 16, 16, 1.0, None, ctypes.byref(plan)) == _lib.SOPHT_ERR_INVALID
     st = (ctypes.c_int64 * 3)(1, 1, 1)
     assert lib.sopht_penalise_field_boundary_3d(0, None, st, 8, 8, 8, -1, None, None) == _lib.SOPHT_ERR_INVALID
+    fd = ctypes.c_void_p()
+    assert lib.sopht_fastdiag3d_create(0, 1, 8, 8, 0.1, None, ctypes.byref(fd)) == _lib.SOPHT_ERR_INVALID
+    assert lib.sopht_fastdiag3d_create(0, 8, 8, 8, -0.1, None, ctypes.byref(fd)) == _lib.SOPHT_ERR_INVALID
+    assert lib.sopht_fastdiag3d_create(3, 8, 8, 8, 0.1, None, ctypes.byref(fd)) == _lib.SOPHT_ERR_INVALID
+    ns = ctypes.c_void_p()
+    assert lib.sopht_ns3d_create_with_poisson_solver(
+        0, 8, 8, 8, 1.0, 1e-2, 1.0, 2, 0, 0, 2, None, ctypes.byref(ns)) == _lib.SOPHT_ERR_INVALID
 
 
 def test_product_does_not_import_oracle():
@@ -93,3 +100,13 @@ def test_laplacian_filter_factory_validation():
         args.update(kw)
         with pytest.raises(ValueError):
             gen_laplacian_filter_kernel_3d(**args)
+
+
+def test_fast_diag_bc_type_validation():
+    """FastDiagPoissonSolver3D defines homogenous_neumann_along_xyz only (FastDiagPoissonSolver3D.py:22)."""
+    import numpy as np
+
+    from sopht_amd.numeric.eulerian_grid_ops import FastDiagPoissonSolver3D
+
+    with pytest.raises(ValueError):
+        FastDiagPoissonSolver3D(8, 8, 8, dx=0.1, real_t=np.float32, bc_type="dirichlet")
