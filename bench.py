@@ -18,6 +18,12 @@ Prints one JSON line with the throughput in cell-updates/s, the per-kernel roofl
 the dominant kernel (its average duration from HIP events recorded on its own stream
 during the timed steps) and a CPU baseline: the numpy/scipy restatement of the same loop
 timed on this host on a bounded sample of the same workload.
+
+Secondary objects on the same line: "batched" (64 small coupled instances in one batched
+step against the same instances stepped one after another), "adjoint" (forward rollout
+against forward with checkpoints plus the adjoint sweep, 128^3, 200 steps) and
+"fast_diagonalisation" (the Neumann fast-diagonalisation vector solve at 256^3 FP32 against
+the FP32 MFMA peak, the flow step with that solver, and the numpy oracle of the solve).
 """
 
 from __future__ import annotations
@@ -36,7 +42,8 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 WORKLOAD = "coupled IB flow step, 256^3 fp32, sphere r=0.08 (3000 Fibonacci points), nu=5e-4, U=(1,0,0), CFL 0.1"
-# frozen algorithmic traffic of the whole step in words per cell (survey section 8(d))
+# frozen algorithmic traffic of the whole step in words per cell: the minimal reads and
+# writes of the step's fields (vorticity, velocity, forcing, stream function) per cell
 STEP_WORDS_PER_CELL = 76
 
 
@@ -285,6 +292,83 @@ def adjoint_vs_forward(n=128, steps=200, checkpoint=20, seed=5):
     }
 
 
+FD_WORKLOAD = "Neumann fast-diagonalisation vector Poisson solve, 256^3 fp32 (3 components)"
+FP32_MFMA_PEAK_TFLOPS = 157.3
+
+
+def fastdiag_solve(n=256, reps=20, cpu=True):
+    """The fast-diagonalisation (Neumann) vector solve: six dense per-axis transforms of
+    2 n^4 flops per component each (rocBLAS GEMMs on the MFMA units) and the spectral scaling,
+    timed with HIP events on the launch stream; the 256^3 flow step with this solver; and the
+    oracle's numpy solve of the same field on the host as the CPU baseline."""
+    import torch
+
+    from sopht_amd import simulator as sps
+    from sopht_amd.numeric.eulerian_grid_ops import FastDiagPoissonSolver3D
+
+    s = FastDiagPoissonSolver3D(n, n, n, dx=1.0 / n, real_t=np.float32)
+    gen = torch.Generator(device="cuda").manual_seed(11)
+    rhs = torch.randn((3, n, n, n), dtype=torch.float32, device="cuda", generator=gen)
+    out = torch.empty_like(rhs)
+    for _ in range(3):
+        s.vector_field_solve(out, rhs)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    torch.cuda.synchronize()
+    ev[0].record()
+    for _ in range(reps):
+        s.vector_field_solve(out, rhs)
+    ev[1].record()
+    torch.cuda.synchronize()
+    ms = ev[0].elapsed_time(ev[1]) / reps
+    flops = 6 * 3 * 2.0 * n**4
+    tflops = flops / (ms * 1e-3) / 1e12
+    del s, out
+    torch.cuda.empty_cache()
+    flow = sps.UnboundedNavierStokesFlowSimulator3D(
+        grid_size=(n, n, n), x_range=1.0, kinematic_viscosity=5e-4, real_t=np.float32,
+        with_forcing=True, with_free_stream_flow=True, poisson_solver_type="fast_diagonalisation")
+    flow.vorticity_field.copy_(rhs)
+    U = np.array([1.0, 0.0, 0.0])
+    for _ in range(3):
+        flow.time_step(1e-4, free_stream_velocity=U)
+    torch.cuda.synchronize()
+    ev[0].record()
+    for _ in range(reps):
+        flow.time_step(1e-4, free_stream_velocity=U)
+    ev[1].record()
+    torch.cuda.synchronize()
+    step_ms = ev[0].elapsed_time(ev[1]) / reps
+    result = {
+        "workload": FD_WORKLOAD,
+        "ms_per_solve": round(ms, 4),
+        "roofline": {
+            "bound": "mfma", "achieved": round(tflops, 2), "peak": FP32_MFMA_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(tflops / FP32_MFMA_PEAK_TFLOPS, 4),
+            "flops_per_solve": int(flops), "traffic": None,
+        },
+        "flow_step_ms": round(step_ms, 4),
+        "flow_step_cell_updates_per_s": n**3 / (step_ms * 1e-3),
+    }
+    if cpu:
+        from threadpoolctl import threadpool_info
+
+        from oracle import fastdiag as FD
+
+        h = rhs.cpu().numpy()
+        o = FD.FastDiagPoissonSolver3D(n, n, n, np.float32(1.0 / n), np.float32)
+        ho = np.zeros_like(h)
+        t0 = time.perf_counter()
+        o.vector_field_solve(ho, h)
+        dt = time.perf_counter() - t0
+        threads = max((i.get("num_threads", 1) for i in threadpool_info() if i.get("user_api") == "blas"),
+                      default=1)
+        result["cpu_baseline"] = {
+            "value": round(1e3 * dt, 1), "unit": "ms per vector solve", "cores": threads, "kind": "port",
+            "sample": f"one {n}^3 fp32 vector solve of the oracle (numpy eig + tensordot, threaded BLAS)",
+        }
+    return result
+
+
 def copy_probe(n, reps):
     """A plain device copy of known size (one (n, n, n) fp32 field per launch, read once
     and written once) through the library's elementwise copy kernel: the calibration run
@@ -336,6 +420,9 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=200,
                     help="steps of the forward vs forward+adjoint comparison (0 = skip)")
     ap.add_argument("--adjoint-only", action="store_true", help="run only the adjoint comparison")
+    ap.add_argument("--fastdiag-reps", type=int, default=20,
+                    help="timed fast-diagonalisation solves and steps (0 = skip)")
+    ap.add_argument("--fastdiag-only", action="store_true", help="run only the fast-diagonalisation solve")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -351,6 +438,12 @@ def main():
 
         torch.cuda.set_device(0)
         print(json.dumps(adjoint_vs_forward(steps=args.adjoint_steps)))
+        return 0
+    if args.fastdiag_only:
+        import torch
+
+        torch.cuda.set_device(0)
+        print(json.dumps(fastdiag_solve(reps=max(1, args.fastdiag_reps), cpu=args.cpu_steps > 0)))
         return 0
     if args.batch_only:
         import torch
@@ -424,6 +517,10 @@ def main():
     if args.adjoint_steps > 0:
         torch.cuda.empty_cache()
         adjoint = adjoint_vs_forward(steps=args.adjoint_steps)
+    fastdiag = None
+    if args.fastdiag_reps > 0:
+        torch.cuda.empty_cache()
+        fastdiag = fastdiag_solve(reps=args.fastdiag_reps, cpu=args.cpu_steps > 0)
     cpu = None
     if args.cpu_steps > 0:
         try:
@@ -456,6 +553,7 @@ def main():
         "roofline": roof,
         "batched": batched,
         "adjoint": adjoint,
+        "fast_diagonalisation": fastdiag,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

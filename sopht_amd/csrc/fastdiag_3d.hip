@@ -46,21 +46,45 @@ namespace {
 
 // f[c][kz][ky][kx] *= 1 / ((lz + ly) + lx); the mode with all three indices last (the
 // constant eigenvector, eigenvalue 0) is set to 0 (FastDiagPoissonSolver3D.py:140-145).
-// Grid: x over the (ky, kx) plane, y over (component, kz) planes.
-template <typename T>
+// Grid: y over (component, kz) planes, x over bands of kScaleRows rows of a plane. A thread
+// owns V consecutive kx (one 16-byte access when V * sizeof(T) == 16) in one row at a time;
+// the 256 threads cover 256 / (nx / V) rows at once (or loop along a row when nx / V > 256).
+constexpr int kScaleRows = 16;
+
+template <typename T, int V>
+struct alignas(V * sizeof(T)) Pack {
+  T v[V];
+};
+
+template <typename T, int V>
 __global__ void __launch_bounds__(256) k_spectral_scale(T* __restrict__ f, const T* __restrict__ lz,
                                                         const T* __restrict__ ly,
                                                         const T* __restrict__ lx, int nz, int ny,
                                                         int nx) {
-  const int q = blockIdx.x * 256 + threadIdx.x;
-  if (q >= ny * nx) return;
+  const int vx = nx / V;                    // packs per row
+  const int rows = vx >= 256 ? 1 : 256 / vx;
+  const int tr = int(threadIdx.x) / (vx >= 256 ? 256 : vx);
+  const int tc = int(threadIdx.x) - tr * (vx >= 256 ? 256 : vx);
+  if (tr >= rows) return;
   const int kz = int(blockIdx.y) % nz;
-  const int ky = q / nx;
-  const int kx = q - ky * nx;
-  const bool null_mode = kz == nz - 1 && ky == ny - 1 && kx == nx - 1;
-  const T inv = null_mode ? T(0) : T(1) / ((lz[kz] + ly[ky]) + lx[kx]);
-  T* p = f + int64_t(blockIdx.y) * ny * nx + q;
-  *p = *p * inv;
+  const T a = lz[kz];
+  const int y0 = int(blockIdx.x) * kScaleRows;
+  const int y1 = min(ny, y0 + kScaleRows);
+  Pack<T, V>* plane = reinterpret_cast<Pack<T, V>*>(f + int64_t(blockIdx.y) * ny * nx);
+  for (int ky = y0 + tr; ky < y1; ky += rows) {
+    const T ab = a + ly[ky];
+    const bool last_row = kz == nz - 1 && ky == ny - 1;
+    for (int c = tc; c < vx; c += 256) {
+      Pack<T, V> p = plane[int64_t(ky) * vx + c];
+#pragma unroll
+      for (int j = 0; j < V; ++j) {
+        const int kx = c * V + j;
+        const T inv = (last_row && kx == nx - 1) ? T(0) : T(1) / (ab + lx[kx]);
+        p.v[j] = p.v[j] * inv;
+      }
+      plane[int64_t(ky) * vx + c] = p;
+    }
+  }
 }
 
 // C = op(A) B or A B in rocBLAS's column-major convention, strided over a batch.
@@ -139,11 +163,17 @@ int solve_impl(sopht_fastdiag3d* P, T* sol, const T* rhs, int ncomp, hipStream_t
   if ((rc = apply_axis<T>(P, 1, F[1], w, sol, ncomp))) return rc;
   if (tm) tm->mark(first + 1, s);
   if ((rc = apply_axis<T>(P, 0, F[0], sol, w, ncomp))) return rc;
-  const dim3 grid(ceil_div(P->n[1] * P->n[2], 256), unsigned(ncomp * P->n[0]));
-  k_spectral_scale<T><<<grid, 256, 0, s>>>(w, static_cast<const T*>(P->lam_dev[0]),
-                                           static_cast<const T*>(P->lam_dev[1]),
-                                           static_cast<const T*>(P->lam_dev[2]), int(P->n[0]),
-                                           int(P->n[1]), int(P->n[2]));
+  {
+    const dim3 grid(ceil_div(P->n[1], kScaleRows), unsigned(ncomp * P->n[0]));
+    const T* lz = static_cast<const T*>(P->lam_dev[0]);
+    const T* ly = static_cast<const T*>(P->lam_dev[1]);
+    const T* lx = static_cast<const T*>(P->lam_dev[2]);
+    constexpr int V = 16 / sizeof(T);
+    if (P->n[2] % V == 0)  // rows of whole 16-byte packs: every plane starts 16-byte aligned
+      k_spectral_scale<T, V><<<grid, 256, 0, s>>>(w, lz, ly, lx, int(P->n[0]), int(P->n[1]), int(P->n[2]));
+    else
+      k_spectral_scale<T, 1><<<grid, 256, 0, s>>>(w, lz, ly, lx, int(P->n[0]), int(P->n[1]), int(P->n[2]));
+  }
   SOPHT_CHECK_LAUNCH();
   if (tm) tm->mark(first + 2, s);
   if ((rc = apply_axis<T>(P, 2, B[2], w, sol, ncomp))) return rc;
