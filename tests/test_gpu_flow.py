@@ -169,6 +169,46 @@ def test_coupled_step_fp32_vs_fp64_oracle(dev):
     assert ew < 1e-5 and eu < 1e-5
 
 
+def test_benchmark_workload_fp32_vs_fp64_at_256(dev):
+    """The benchmarked workload at its full size (256^3, random-phase vorticity |k| <= 8 seed 42
+    scaled to max|w| = 10, nu = 5e-4, sphere r = 0.08 with 3000 Fibonacci points, U = (1, 0, 0)):
+    three coupled steps in FP32 against the same steps in FP64, which match the oracle to 1e-10
+    on every grid the oracle covers; both take the FP64 run's dt. Vorticity and velocity
+    relative L2 <= 1e-5."""
+    import torch
+
+    S = sps()
+    n = 256
+    w0 = synthetic.random_phase_vorticity((n, n, n), 1.0, 8, 42, 10.0)
+    U = np.array([1.0, 0.0, 0.0])
+    sims = []
+    for real_t in (np.float64, np.float32):
+        flow = S.UnboundedNavierStokesFlowSimulator3D(
+            grid_size=(n, n, n), x_range=1.0, kinematic_viscosity=5e-4, cfl=0.1, real_t=real_t,
+            with_forcing=True, with_free_stream_flow=True)
+        flow.vorticity_field.copy_(torch.from_numpy(w0.astype(real_t)))
+        flow.recover_velocity(U)
+        inter = S.RigidBodyFlowInteraction(
+            S.RigidBody((0.5, 0.5, 0.5), 0.08), flow.eul_grid_forcing_field, flow.velocity_field,
+            -5e4, -10.0, flow.dx, 3, S.FibonacciSphereForcingGrid, real_t=real_t, num_lag_nodes=3000)
+        sims.append((flow, inter))
+    del w0
+    (f64, i64), (f32, i32) = sims
+    for _ in range(3):
+        dt = f64.compute_stable_timestep()
+        for flow, inter in sims:
+            inter.time_step(dt)
+            inter()
+            flow.time_step(dt, free_stream_velocity=U)
+    i32.check_support_in_domain()
+    for name in ("vorticity_field", "velocity_field"):
+        a = getattr(f32, name).double()
+        b = getattr(f64, name)
+        err = (torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b)).item()
+        print(f"256^3 fp32 vs fp64 after 3 steps: {name} rel-L2 {err:.3e}")
+        assert err <= 1e-5, (name, err)
+
+
 @pytest.mark.parametrize("precision", ["single", "double"])
 @pytest.mark.parametrize("shape", [(16, 16, 16), (8, 12, 6)])
 def test_fused_velocity_max_matches_full_reduction(dev, precision, shape, rng):
