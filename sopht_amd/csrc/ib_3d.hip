@@ -100,6 +100,9 @@ int sopht_spread3d_create(int64_t max_nodes, sopht_spread3d** out) {
       hipMalloc(&w->poff, sizeof(int) * slots) != hipSuccess ||
       hipMalloc(&w->plist, sizeof(uint32_t) * slots) != hipSuccess ||
       hipMalloc(&w->owners, sizeof(uint32_t) * slots) != hipSuccess ||
+      hipMalloc(&w->table, sizeof(int) * (kTableMax + 1)) != hipSuccess ||
+      hipMalloc(&w->bbox, sizeof(int) * 8) != hipSuccess ||
+      hipMemset(w->bbox, 0, sizeof(int) * 8) != hipSuccess ||
       hipMemset(w->prev, 0xff, sizeof(uint64_t) * p) != hipSuccess ||
       hipMemcpy(w->flag, init_flags, sizeof(init_flags), hipMemcpyHostToDevice) != hipSuccess) {
     sopht_spread3d_destroy(w);
@@ -119,6 +122,8 @@ int sopht_spread3d_destroy(sopht_spread3d* w) {
   if (w->poff) (void)hipFree(w->poff);
   if (w->plist) (void)hipFree(w->plist);
   if (w->owners) (void)hipFree(w->owners);
+  if (w->table) (void)hipFree(w->table);
+  if (w->bbox) (void)hipFree(w->bbox);
   delete w;
   return SOPHT_OK;
 }
@@ -146,10 +151,12 @@ int sopht_lag_to_eul_spread_3d(int dtype, int ncomp, sopht_spread3d* ws, void* c
                                                             ws->prev, ws->flag, force);
   SOPHT_CHECK_LAUNCH();
   if (padded <= 8192) {
-    k_bitonic_lds<<<1, 1024, padded * sizeof(uint64_t), s>>>(ws->prev, ws->keys, int(padded), ws->flag);
+    k_bitonic_lds<<<1, 1024, bitonic_lds_bytes(padded), s>>>(ws->prev, ws->keys, int(padded),
+                                                             ws->flag, ny, nx, ws->table, ws->bbox,
+                                                             table_cap_for(padded));
     SOPHT_CHECK_LAUNCH();
   } else {
-    k_copy_keys<<<blocks_for(padded, 256), 256, 0, s>>>(ws->prev, ws->keys, padded, ws->flag);
+    k_copy_keys<<<blocks_for(padded, 256), 256, 0, s>>>(ws->prev, ws->keys, padded, ws->flag, ws->bbox);
     SOPHT_CHECK_LAUNCH();
     for (int64_t k = 2; k <= padded; k <<= 1)
       for (int64_t j = k >> 1; j > 0; j >>= 1) {
@@ -161,24 +168,24 @@ int sopht_lag_to_eul_spread_3d(int dtype, int ncomp, sopht_spread3d* ws, void* c
   // lists; sorted keys in LDS for the owners' binary searches when they fit (128 KiB)
   if (n <= 16384)
     k_spread_owner<true><<<blocks_for(n * 64, 512), 512, n * sizeof(uint64_t), s>>>(
-        nz, ny, nx, ws->keys, n, ws->flag, ws->pcount, ws->owners);
+        nz, ny, nx, ws->keys, n, ws->flag, ws->pcount, ws->poff, ws->owners, ws->table, ws->bbox);
   else
     k_spread_owner<false><<<blocks_for(n * 64, 512), 512, 0, s>>>(nz, ny, nx, ws->keys, n, ws->flag,
-                                                                  ws->pcount, ws->owners);
+                                                                  ws->pcount, ws->poff, ws->owners,
+                                                                  ws->table, ws->bbox);
   SOPHT_CHECK_LAUNCH();
-  const int mblocks = std::min(2048, blocks_for(n * 64, 64 * kMergeWaves));
-  if (n <= 8192)  // sorted keys in LDS (64 KiB) beside the merge stage
-    k_spread_merge<true><<<mblocks, 64 * kMergeWaves, n * sizeof(uint64_t), s>>>(
-        nz, ny, nx, ws->keys, n, ws->flag, ws->pcount, ws->poff, ws->owners, ws->plist);
-  else
-    k_spread_merge<false><<<mblocks, 64 * kMergeWaves, 0, s>>>(
-        nz, ny, nx, ws->keys, n, ws->flag, ws->pcount, ws->poff, ws->owners, ws->plist);
+  // about one wavefront per node (an owned cell has ~16 contributors, so a node owns ~4 cells
+  // on average): the owners' latency chains then overlap instead of queueing in a few waves
+  const int mblocks = std::min(8192, blocks_for(n * 64, 64 * kMergeWaves / 4));
+  k_spread_merge<<<mblocks, 64 * kMergeWaves, 0, s>>>(nz, ny, nx, ws->keys, n, ws->flag, ws->pcount,
+                                                     ws->poff, ws->owners, ws->plist, ws->table,
+                                                     ws->bbox);
   SOPHT_CHECK_LAUNCH();
   IB_DISPATCH(dtype, ([&] {
                 EulW3<T> F;
                 for (int c = 0; c < 3; ++c) F.p[c] = c < ncomp ? static_cast<T*>(eul[c]) : nullptr;
                 F.sz = es[0]; F.sy = es[1]; F.sx = es[2];
-                k_spread_replay_owners<T><<<std::min(2048, blocks_for(n * 64, 256)), 256, 0, s>>>(
+                k_spread_replay_owners<T><<<std::min(8192, blocks_for(n * 64, 64)), 256, 0, s>>>(
                     ncomp, F, nz, ny, nx, static_cast<const T*>(lag), static_cast<const T*>(weights),
                     ws->keys, n, ws->pcount, ws->poff, ws->plist, ws->owners, ws->flag);
               }()));

@@ -23,6 +23,12 @@ struct sopht_spread3d {
   int* poff;
   uint32_t* plist;
   uint32_t* owners;   // owning slots t of the plan (flag[3] of them), for the list merge
+  // Lower-bound table of the sorted keys over the bounding box of the node base cells (built
+  // with the sort when the box fits, else bbox[0] = 0 and the plan build searches the keys):
+  // bbox = {valid, z0, y0, x0, Z, Y, X}; table[l] = number of sorted keys whose cell comes
+  // before box cell l (lexicographic (z, y, x) in the box), table[Z Y X] = valid keys.
+  int* table;
+  int* bbox;
   int64_t plan_nz, plan_ny, plan_nx;  // grid the plan was built for (host copy)
 };
 
@@ -338,21 +344,139 @@ __device__ __forceinline__ void bitonic_block(const uint64_t* prev, uint64_t* ke
   for (int i = threadIdx.x; i < padded; i += blockDim.x) keys[i] = sk[i];
 }
 
+// Box table of the sorted keys (see sopht_spread3d::table), built by the sorting workgroup
+// from its LDS copy of the keys: the valid keys' bounding box (z from the first and last key,
+// y and x by LDS min/max), per-cell counts in LDS, and their exclusive scan. A box of more
+// than `cap` cells leaves bbox[0] = 0. Keys in key order are in lexicographic (z, y, x) order
+// inside the box, so a lower bound of any cell is one table entry after clamping.
+constexpr int kTableMax = 32768;
+
+__device__ void cell_table_block(const uint64_t* sk, int padded, int64_t ny, int64_t nx, int cap,
+                                 int* cnt, int* scan, int* sh, int* table, int* bb) {
+  const int tid = int(threadIdx.x), nt = int(blockDim.x);
+  if (tid == 0) {
+    sh[0] = 0;
+    sh[1] = INT32_MAX; sh[2] = -1;  // y min, max
+    sh[3] = INT32_MAX; sh[4] = -1;  // x min, max
+  }
+  __syncthreads();
+  int ymn = INT32_MAX, ymx = -1, xmn = INT32_MAX, xmx = -1;
+  for (int i = tid; i < padded; i += nt) {
+    const uint64_t k = sk[i];
+    if (k == ~uint64_t(0)) continue;
+    if (i + 1 == padded || sk[i + 1] == ~uint64_t(0)) sh[0] = i + 1;  // the last valid key
+    const int64_t cell = int64_t(k >> 32);
+    const int y = int((cell / nx) % ny), x = int(cell % nx);
+    ymn = y < ymn ? y : ymn;
+    ymx = y > ymx ? y : ymx;
+    xmn = x < xmn ? x : xmn;
+    xmx = x > xmx ? x : xmx;
+  }
+  for (int sft = 1; sft < 64; sft <<= 1) {  // wavefront reduction, then one LDS atomic each
+    const int a0 = __shfl_xor(ymn, sft, 64), a1 = __shfl_xor(ymx, sft, 64);
+    const int a2 = __shfl_xor(xmn, sft, 64), a3 = __shfl_xor(xmx, sft, 64);
+    ymn = a0 < ymn ? a0 : ymn;
+    ymx = a1 > ymx ? a1 : ymx;
+    xmn = a2 < xmn ? a2 : xmn;
+    xmx = a3 > xmx ? a3 : xmx;
+  }
+  if ((tid & 63) == 0) {
+    atomicMin(&sh[1], ymn);
+    atomicMax(&sh[2], ymx);
+    atomicMin(&sh[3], xmn);
+    atomicMax(&sh[4], xmx);
+  }
+  __syncthreads();
+  const int nv = sh[0];
+  const int64_t plane = nx * ny;
+  const int z0 = nv > 0 ? int(int64_t(sk[0] >> 32) / plane) : 0;
+  const int Z = nv > 0 ? int(int64_t(sk[nv - 1] >> 32) / plane) - z0 + 1 : 0;
+  const int Y = sh[2] - sh[1] + 1, X = sh[4] - sh[3] + 1;
+  const int64_t vol = int64_t(Z) * Y * X;
+  if (nv == 0 || vol > cap) {  // uniform
+    if (tid == 0) bb[0] = 0;
+    return;
+  }
+  const int y0 = sh[1], x0 = sh[3];
+  for (int l = tid; l < vol; l += nt) cnt[l] = 0;
+  __syncthreads();
+  for (int i = tid; i < nv; i += nt) {
+    const int64_t cell = int64_t(sk[i] >> 32);
+    const int z = int(cell / plane), y = int((cell / nx) % ny), x = int(cell % nx);
+    atomicAdd(&cnt[((z - z0) * Y + (y - y0)) * X + (x - x0)], 1);
+  }
+  __syncthreads();
+  const int v32 = int(vol);
+  const int chunk = (v32 + nt - 1) / nt;
+  const int lb = tid * chunk < v32 ? tid * chunk : v32;
+  const int le = lb + chunk < v32 ? lb + chunk : v32;
+  int sum = 0;
+  for (int l = lb; l < le; ++l) sum += cnt[l];
+  scan[tid] = sum;
+  __syncthreads();
+  for (int off = 1; off < nt; off <<= 1) {  // inclusive scan of the chunk sums
+    const int v = tid >= off ? scan[tid - off] : 0;
+    __syncthreads();
+    scan[tid] += v;
+    __syncthreads();
+  }
+  int run = scan[tid] - sum;
+  for (int l = lb; l < le; ++l) {
+    table[l] = run;
+    run += cnt[l];
+  }
+  if (tid == 0) {
+    table[v32] = nv;
+    bb[0] = 1; bb[1] = z0; bb[2] = y0; bb[3] = x0; bb[4] = Z; bb[5] = Y; bb[6] = X;
+  }
+}
+
+// Dynamic LDS of k_bitonic_lds: the keys, then cap counts, the 1024 scan slots and 8 ints.
+inline int table_cap_for(int64_t padded) {
+  const int64_t budget = 160 * 1024 - 512 - padded * 8 - 1024 * 4 - 8 * 4;
+  const int64_t cap = budget / 4;
+  return int(cap < kTableMax ? (cap > 0 ? cap : 0) : kTableMax);
+}
+inline size_t bitonic_lds_bytes(int64_t padded) {
+  return size_t(padded) * 8 + size_t(table_cap_for(padded)) * 4 + 1024 * 4 + 8 * 4;
+}
+
 __global__ void __launch_bounds__(1024) k_bitonic_lds(const uint64_t* prev, uint64_t* keys,
-                                                      int padded, int* flag) {
+                                                      int padded, int* flag, int64_t ny,
+                                                      int64_t nx, int* table, int* bb, int cap) {
   extern __shared__ uint64_t sk[];
+  if (flag_load(flag + 1) == 0) return;  // keys unchanged: sort and table stay valid
   bitonic_block(prev, keys, padded, flag, sk);
+  int* cnt = reinterpret_cast<int*>(sk + padded);
+  cell_table_block(sk, padded, ny, nx, cap, cnt, cnt + cap, cnt + cap + 1024, table, bb);
+}
+
+// Lower bound of cell (z, y, x) in the sorted keys from the box table: the number of valid
+// keys whose cell comes before it.
+__device__ __forceinline__ int64_t table_lb(const int* T, const int* bb, int64_t z, int64_t y,
+                                            int64_t x) {
+  const int64_t Z = bb[4], Y = bb[5], X = bb[6];
+  const int64_t zz = z - bb[1];
+  if (zz < 0) return 0;
+  if (zz >= Z) return T[Z * Y * X];
+  const int64_t yy = y - bb[2];
+  if (yy < 0) return T[zz * Y * X];
+  if (yy >= Y) return T[(zz + 1) * Y * X];
+  int64_t xx = x - bb[3];
+  xx = xx < 0 ? 0 : (xx > X ? X : xx);  // xx = X is the next row's start
+  return T[(zz * Y + yy) * X + xx];
 }
 
 // Large-node path of the key sort (only when the keys changed, as bitonic_block): copy, then
 // global compare-exchange steps.
-__global__ void k_copy_keys(const uint64_t* src, uint64_t* dst, int64_t m, int* flag) {
+__global__ void k_copy_keys(const uint64_t* src, uint64_t* dst, int64_t m, int* flag, int* bb) {
   if (flag_load(flag + 1) == 0) return;  // plan still valid: keys stay sorted
   const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (i < m) dst[i] = src[i];
   if (i == 0) {
     atomicExch(flag + 2, 0);
     atomicExch(flag + 3, 0);
+    bb[0] = 0;  // no box table on this path: the plan build searches the keys
   }
 }
 
@@ -471,67 +595,136 @@ __device__ __forceinline__ void spread_plan_slot(int64_t nz, int64_t ny, int64_t
 // Plan build for moving bodies, in two phases (only when the keys changed).
 // Phase 1, one thread per support slot t = 64 p + s: slot t owns its cell c when sorted node p
 // is the first of c's contributors in key order, i.e. when the first non-empty range among
-// c's candidate base-cell ranges (searched in increasing key order) starts at p. Owners go to
-// the owner list; every other slot gets pcount = 0.
-__device__ __forceinline__ void spread_owner_slot(int64_t nz, int64_t ny, int64_t nx,
-                                                  const uint64_t* keys, int64_t n, int* flag,
-                                                  int* pcount, uint32_t* owners, int64_t gid) {
+// c's candidate base-cell ranges (searched in increasing key order) starts at p. An owner
+// also counts its contributors m (the lengths of all 16 ranges); a workgroup then reserves
+// its owners' places in the owner list and their m-entry windows of the contributor list
+// with one atomic each (same-address atomics from every owner serialise). pcount[t] = m for
+// owners, 0 otherwise. Ranges: a box-table lookup (two per range) when the table is valid,
+// else binary searches of the sorted keys.
+__device__ __forceinline__ int spread_owner_count(int64_t nz, int64_t ny, int64_t nx,
+                                                  const uint64_t* keys, int64_t n, int64_t gid,
+                                                  const int* table, const int* bb, bool use_table) {
   const int64_t p = gid >> 6;
   const int slot = int(gid & 63);
-  if (p >= n) return;
-  pcount[gid] = 0;
   const uint64_t kp = keys[p];
-  if (kp == ~uint64_t(0)) return;  // out-of-domain node (sorted to the end)
+  if (kp == ~uint64_t(0)) return 0;  // out-of-domain node (sorted to the end)
   const int64_t cell0 = int64_t(kp >> 32);
   const int64_t bx = cell0 % nx, by = (cell0 / nx) % ny, bz = cell0 / (nx * ny);
   const int64_t cz = bz + (slot >> 4), cy = by + ((slot >> 2) & 3), cx = bx + (slot & 3);
+  int64_t qx0 = cx - 3, qx1 = cx;  // inclusive base x range, the same for every row
+  if (qx0 < 0) qx0 = 0;
+  if (qx1 + W4 > nx) qx1 = nx - W4;
+  if (use_table) {  // uniform: all 32 lookups issued before any is used
+    int a[16], b[16];
+#pragma unroll
+    for (int g = 0; g < 16; ++g) {
+      const int64_t qz = cz - (3 - (g >> 2)), qy = cy - (3 - (g & 3));  // increasing key order
+      const bool row = qz >= 0 && qy >= 0 && qz + W4 <= nz && qy + W4 <= ny && qx0 <= qx1;
+      a[g] = row ? int(table_lb(table, bb, qz, qy, qx0)) : 0;
+      b[g] = row ? int(table_lb(table, bb, qz, qy, qx1 + 1)) : 0;
+    }
+    int m = 0;
+    bool first = true;
+#pragma unroll
+    for (int g = 0; g < 16; ++g) {
+      if (b[g] > a[g]) {
+        if (first && a[g] != p) return 0;  // an earlier node is the first contributor
+        first = false;
+        m += b[g] - a[g];
+      }
+    }
+    return m;  // p itself lies in the first non-empty range, so m >= 1
+  }
+  bool first = true;
+  int m = 0;
   for (int g = 0; g < 16; ++g) {
     const int dz = 3 - (g >> 2), dy = 3 - (g & 3);  // increasing key order
     const int64_t qz = cz - dz, qy = cy - dy;
     if (qz < 0 || qy < 0 || qz + W4 > nz || qy + W4 > ny) continue;
-    int64_t qx0 = cx - 3, qx1 = cx;  // inclusive base x range
-    if (qx0 < 0) qx0 = 0;
-    if (qx1 + W4 > nx) qx1 = nx - W4;
     if (qx0 > qx1) continue;
-    const uint64_t c0 = uint64_t((qz * ny + qy) * nx + qx0);
-    const uint64_t c1 = uint64_t((qz * ny + qy) * nx + qx1);
-    const int64_t a = lower_bound_cell(keys, n, c0);
-    if (a < n && (keys[a] >> 32) <= c1) {  // first non-empty range
-      if (a == p) owners[atomicAdd(flag + 3, 1)] = uint32_t(gid);
-      return;
+    const int64_t a = lower_bound_cell(keys, n, uint64_t((qz * ny + qy) * nx + qx0));
+    const int64_t b = lower_bound_cell(keys, n, uint64_t((qz * ny + qy) * nx + qx1) + 1);
+    if (b > a) {
+      if (first) {
+        if (a != p) return 0;  // an earlier node is the first contributor
+        first = false;
+      }
+      m += int(b - a);
     }
   }
+  return m;  // p itself lies in the first non-empty range, so m >= 1
 }
 
 template <bool LDS_KEYS>
 __global__ void __launch_bounds__(512) k_spread_owner(int64_t nz, int64_t ny, int64_t nx,
                                                       const uint64_t* gkeys, int64_t n, int* flag,
-                                                      int* pcount, uint32_t* owners) {
+                                                      int* pcount, int* poff, uint32_t* owners,
+                                                      const int* table, const int* bb) {
   if (flag_load(flag + 1) == 0) return;  // plan still valid (uniform for the launch)
   extern __shared__ uint64_t skeys[];
+  __shared__ int wo[8], wm[8], base_o, base_m;
+  const bool use_table = bb[0] != 0;     // uniform
   const uint64_t* keys = gkeys;
   if constexpr (LDS_KEYS) {
-    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) skeys[i] = gkeys[i];
-    __syncthreads();
-    keys = skeys;
+    if (!use_table) {  // the binary searches then run on LDS
+      for (int64_t i = threadIdx.x; i < n; i += blockDim.x) skeys[i] = gkeys[i];
+      __syncthreads();
+      keys = skeys;
+    }
   }
-  spread_owner_slot(nz, ny, nx, keys, n, flag, pcount, owners,
-                    blockIdx.x * int64_t(blockDim.x) + threadIdx.x);
+  const int tid = int(threadIdx.x), lane = tid & 63, w = tid >> 6;
+  const int64_t gid = blockIdx.x * int64_t(blockDim.x) + tid;
+  const bool live = (gid >> 6) < n;
+  const int m = live ? spread_owner_count(nz, ny, nx, keys, n, gid, table, bb, use_table) : 0;
+  int so = m > 0 ? 1 : 0, sm = m;  // inclusive scans over the wavefront
+  for (int sft = 1; sft < 64; sft <<= 1) {
+    const int a = __shfl_up(so, sft, 64), b = __shfl_up(sm, sft, 64);
+    if (lane >= sft) {
+      so += a;
+      sm += b;
+    }
+  }
+  if (lane == 63) {
+    wo[w] = so;
+    wm[w] = sm;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int to = 0, tm = 0;
+    for (int i = 0; i < int(blockDim.x >> 6); ++i) {
+      const int a = wo[i], b = wm[i];
+      wo[i] = to;
+      wm[i] = tm;
+      to += a;
+      tm += b;
+    }
+    base_o = to > 0 ? atomicAdd(flag + 3, to) : 0;
+    base_m = tm > 0 ? atomicAdd(flag + 2, tm) : 0;
+  }
+  __syncthreads();
+  if (live) {
+    pcount[gid] = m;
+    if (m > 0) {
+      owners[base_o + wo[w] + so - 1] = uint32_t(gid);
+      poff[gid] = base_m + wm[w] + sm - m;
+    }
+  }
 }
 
 // Phase 2, one wavefront per owner: the contributors of cell c are the nodes whose base cell
 // is c - (dz, dy, dx) for (dz, dy, dx) in [0, 3]^3; lane (dz, dy, dx) holds that base cell's
-// run of the sorted keys (in increasing node order). A prefix sum over the lanes sizes the
-// list (m contributors, one reservation per owner); the runs are staged in the wavefront's
+// run of the sorted keys (in increasing node order). A prefix sum over the lanes places each
+// run in the owner's window of m entries (reserved by phase 1); the runs are staged in the wavefront's
 // LDS window when they fit, and the wavefront merges the 64 runs by node: each output is the
 // minimum head over the lanes (node numbers are unique), and the contribution's support
 // slot in its node's block is the lane number itself.
-constexpr int kMergeWaves = 4, kMergeCap = 2048;  // LDS entries per wavefront
+constexpr int kMergeWaves = 4, kMergeCap = 1024;  // LDS entries per wavefront
 
 __device__ __forceinline__ void spread_merge_owner(int64_t nz, int64_t ny, int64_t nx,
                                                    const uint64_t* keys, int64_t n, int* flag,
                                                    int* pcount, int* poff, uint32_t* plist,
-                                                   int64_t gid, int lane, uint32_t* stage) {
+                                                   int64_t gid, int lane, uint32_t* stage,
+                                                   const int* table, const int* bb, bool use_table) {
   const int64_t p = gid >> 6;
   const int slot = int(gid & 63);
   const int64_t cell0 = int64_t(keys[p] >> 32);
@@ -540,9 +733,14 @@ __device__ __forceinline__ void spread_merge_owner(int64_t nz, int64_t ny, int64
   const int64_t qz = cz - (lane >> 4), qy = cy - ((lane >> 2) & 3), qx = cx - (lane & 3);
   int64_t pos = 0, hi = 0;
   if (qz >= 0 && qy >= 0 && qx >= 0 && qz + W4 <= nz && qy + W4 <= ny && qx + W4 <= nx) {
-    const uint64_t qc = uint64_t((qz * ny + qy) * nx + qx);
-    pos = lower_bound_cell(keys, n, qc);
-    hi = lower_bound_cell(keys, n, qc + 1);
+    if (use_table) {
+      pos = table_lb(table, bb, qz, qy, qx);
+      hi = table_lb(table, bb, qz, qy, qx + 1);
+    } else {
+      const uint64_t qc = uint64_t((qz * ny + qy) * nx + qx);
+      pos = lower_bound_cell(keys, n, qc);
+      hi = lower_bound_cell(keys, n, qc + 1);
+    }
   }
   const int len = int(hi - pos);
   int incl = len;  // inclusive prefix sum over the lanes
@@ -550,14 +748,12 @@ __device__ __forceinline__ void spread_merge_owner(int64_t nz, int64_t ny, int64
     const int v = __shfl_up(incl, sft, 64);
     if (lane >= sft) incl += v;
   }
-  const int m = __shfl(incl, 63, 64);
-  int base = 0;
-  if (lane == 0) base = atomicAdd(flag + 2, m);
-  base = __shfl(base, 0, 64);
-  if (lane == 0) {
-    pcount[gid] = m;
-    poff[gid] = base;
-  }
+  // the lanes' runs telescope to the owner pass's ranges, so the total equals pcount[gid];
+  // the window reserved there is never exceeded
+  const int mc = __shfl(incl, 63, 64);
+  const int reserved = pcount[gid];
+  const int m = mc < reserved ? mc : reserved;
+  const int base = poff[gid];
   const bool staged = m <= kMergeCap;  // uniform
   const int mine = incl - len;         // this lane's window in the stage
   if (staged)
@@ -578,26 +774,21 @@ __device__ __forceinline__ void spread_merge_owner(int64_t nz, int64_t ny, int64
   }
 }
 
-template <bool LDS_KEYS>
+// Keys are read from global memory (L2): with the box table the runs need two lookups per lane;
+// without it the binary searches run there too.
 __global__ void __launch_bounds__(64 * kMergeWaves) k_spread_merge(
-    int64_t nz, int64_t ny, int64_t nx, const uint64_t* gkeys, int64_t n, int* flag, int* pcount,
-    int* poff, const uint32_t* owners, uint32_t* plist) {
+    int64_t nz, int64_t ny, int64_t nx, const uint64_t* keys, int64_t n, int* flag, int* pcount,
+    int* poff, const uint32_t* owners, uint32_t* plist, const int* table, const int* bb) {
   __shared__ uint32_t stage[kMergeWaves][kMergeCap];
-  extern __shared__ uint64_t skeys[];
   if (flag_load(flag + 1) == 0) return;  // plan still valid (uniform for the launch)
-  const uint64_t* keys = gkeys;
-  if constexpr (LDS_KEYS) {  // the binary searches then run on LDS
-    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) skeys[i] = gkeys[i];
-    __syncthreads();
-    keys = skeys;
-  }
+  const bool use_table = bb[0] != 0;     // uniform
   const int lane = int(threadIdx.x & 63);
   const int w = int(threadIdx.x >> 6);
   const int64_t waves = int64_t(gridDim.x) * kMergeWaves;
   const int64_t count = flag_load(flag + 3);
   for (int64_t o = int64_t(blockIdx.x) * kMergeWaves + w; o < count; o += waves)
     spread_merge_owner(nz, ny, nx, keys, n, flag, pcount, poff, plist, int64_t(owners[o]), lane,
-                       stage[w]);
+                       stage[w], table, bb, use_table);
 }
 
 // Plan replay for the batched step (every call; one thread per support slot): each owned cell
