@@ -41,6 +41,25 @@ constexpr int W4 = 4;  // 2 * interp_kernel_width
 // from every thread of a launch serialises on the flag's address).
 __device__ __forceinline__ int flag_load(const int* f) { return __atomic_load_n(f, __ATOMIC_RELAXED); }
 
+// Wavefront-scope ordering of LDS accesses: one wavefront's LDS operations execute in order;
+// the fences keep the compiler from moving accesses across this point.
+__device__ __forceinline__ void ib_wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Value of v in lane j (j uniform over the wavefront).
+__device__ __forceinline__ float lane_value(float v, int j) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), j));
+}
+__device__ __forceinline__ double lane_value(double v, int j) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(b)), j));
+  const uint32_t hi = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(b >> 32)), j));
+  return __builtin_bit_cast(double, (uint64_t(hi) << 32) | lo);
+}
+
 // numpy floor_divide for doubles (npy_divmod): exact floor of a / b including the
 // cases where a / b rounds onto an integer.
 __device__ __forceinline__ double np_floor_divide(double a, double b) {
@@ -754,14 +773,30 @@ __device__ __forceinline__ void spread_merge_owner(int64_t nz, int64_t ny, int64
   const int reserved = pcount[gid];
   const int m = mc < reserved ? mc : reserved;
   const int base = poff[gid];
-  const bool staged = m <= kMergeCap;  // uniform
-  const int mine = incl - len;         // this lane's window in the stage
-  if (staged)
-    for (int i = 0; i < len; ++i) stage[mine + i] = uint32_t(keys[pos + i] & 0xffffffffu);
+  const int mine = incl - len;  // this lane's window in the stage
+  if (mc <= kMergeCap) {        // uniform: rank the staged entries
+    // entries (node << 6 | slot) with unique nodes: their order is the node order, and an
+    // entry's place in the list is the number of staged entries below it
+    ib_wave_sync();  // the previous owner's reads of the stage are done
+    for (int i = 0; i < len; ++i)
+      stage[mine + i] = (uint32_t(keys[pos + i] & 0xffffffffu) << 6) | uint32_t(lane);
+    ib_wave_sync();
+    for (int t0 = 0; t0 < m; t0 += 64) {
+      if (t0 + lane < m) {
+        const uint32_t e = stage[t0 + lane];
+        int rank = 0;
+        for (int u = 0; u < m; ++u) rank += stage[u] < e ? 1 : 0;  // broadcast reads
+        plist[base + rank] = e;
+      }
+    }
+    return;
+  }
+  // more contributors than the stage holds: merge the lanes' runs from the keys, the minimum
+  // head over the wavefront at a time
   int i = 0;  // position within the lane's run
   for (int k = 0; k < m; ++k) {  // m is uniform over the wavefront
     uint32_t head = 0xffffffffu;
-    if (i < len) head = staged ? stage[mine + i] : uint32_t(keys[pos + i] & 0xffffffffu);
+    if (i < len) head = uint32_t(keys[pos + i] & 0xffffffffu);
     uint32_t mn = head;
     for (int sft = 1; sft < 64; sft <<= 1) {
       const uint32_t o = __shfl_xor(mn, sft, 64);
@@ -856,7 +891,7 @@ __global__ void __launch_bounds__(256) k_spread_replay_owners(
       }
       const int cnt = m - k0 < 64 ? m - k0 : 64;
       for (int j = 0; j < cnt; ++j)
-        for (int c = 0; c < ncomp; ++c) acc[c] = acc[c] + __shfl(p[c], j, 64);
+        for (int c = 0; c < ncomp; ++c) acc[c] = acc[c] + lane_value(p[c], j);
     }
     if (lane == 0)
       for (int c = 0; c < ncomp; ++c) F.p[c][cidx] = acc[c];
