@@ -91,11 +91,11 @@ int sopht_spread3d_create(int64_t max_nodes, sopht_spread3d** out) {
   int64_t p = 1;
   while (p < max_nodes) p <<= 1;
   w->padded = p;
-  const int init_flags[4] = {0, 1, 0, 0};  // no out-of-domain node yet; first call builds
+  const int init_flags[8] = {0, 1, 0, 0, 0, 0, 0, 0};  // no out-of-domain node yet; first call builds
   const int64_t slots = 64 * (max_nodes > 0 ? max_nodes : 1);
   if (hipMalloc(&w->keys, sizeof(uint64_t) * p) != hipSuccess ||
       hipMalloc(&w->prev, sizeof(uint64_t) * p) != hipSuccess ||
-      hipMalloc(&w->flag, 4 * sizeof(int)) != hipSuccess ||
+      hipMalloc(&w->flag, 8 * sizeof(int)) != hipSuccess ||
       hipMalloc(&w->pcount, sizeof(int) * slots) != hipSuccess ||
       hipMalloc(&w->poff, sizeof(int) * slots) != hipSuccess ||
       hipMalloc(&w->plist, sizeof(uint32_t) * slots) != hipSuccess ||
@@ -185,9 +185,12 @@ int sopht_lag_to_eul_spread_3d(int dtype, int ncomp, sopht_spread3d* ws, void* c
                 EulW3<T> F;
                 for (int c = 0; c < 3; ++c) F.p[c] = c < ncomp ? static_cast<T*>(eul[c]) : nullptr;
                 F.sz = es[0]; F.sy = es[1]; F.sx = es[2];
-                k_spread_replay_owners<T><<<std::min(8192, blocks_for(n * 64, 64)), 256, 0, s>>>(
-                    ncomp, F, nz, ny, nx, static_cast<const T*>(lag), static_cast<const T*>(weights),
-                    ws->keys, n, ws->pcount, ws->poff, ws->plist, ws->owners, ws->flag);
+                const T* L = static_cast<const T*>(lag);
+                const T* W = static_cast<const T*>(weights);
+                // about four wavefronts per node for the wavefront form (a node owns ~4 cells)
+                k_spread_replay<T><<<std::min(8192, blocks_for(n * 4, 4)), 256, 0, s>>>(
+                    ncomp, F, nz, ny, nx, L, W, ws->keys, n, ws->pcount, ws->poff, ws->plist,
+                    ws->owners, ws->flag);
               }()));
   SOPHT_CHECK_LAUNCH();
   return SOPHT_OK;

@@ -15,7 +15,8 @@ struct sopht_spread3d {
   uint64_t* keys;     // (key << 32 | node), sorted, padded
   uint64_t* prev;     // unsorted keys of the previous call, padded
   int* flag;          // [0] out-of-domain, [1] keys changed since the plan was built,
-                      // [2] fill counter of the plan's contributor list, [3] owner count
+                      // [2] fill counter of the plan's contributor list, [3] owner count,
+                      // [4] largest contributor count of an owned cell (8 ints)
   // Gather plan, rebuilt only when the keys change: for support slot t = 64 p + s of sorted
   // node p that owns its cell, pcount[t] contributors starting at plist[poff[t]], each
   // (node << 6 | slot) in increasing node order; pcount[t] = 0 for the other slots.
@@ -465,6 +466,7 @@ __global__ void __launch_bounds__(1024) k_bitonic_lds(const uint64_t* prev, uint
                                                       int64_t nx, int* table, int* bb, int cap) {
   extern __shared__ uint64_t sk[];
   if (flag_load(flag + 1) == 0) return;  // keys unchanged: sort and table stay valid
+  if (threadIdx.x == 0) atomicExch(flag + 4, 0);
   bitonic_block(prev, keys, padded, flag, sk);
   int* cnt = reinterpret_cast<int*>(sk + padded);
   cell_table_block(sk, padded, ny, nx, cap, cnt, cnt + cap, cnt + cap + 1024, table, bb);
@@ -495,6 +497,7 @@ __global__ void k_copy_keys(const uint64_t* src, uint64_t* dst, int64_t m, int* 
   if (i == 0) {
     atomicExch(flag + 2, 0);
     atomicExch(flag + 3, 0);
+    atomicExch(flag + 4, 0);
     bb[0] = 0;  // no box table on this path: the plan build searches the keys
   }
 }
@@ -681,7 +684,7 @@ __global__ void __launch_bounds__(512) k_spread_owner(int64_t nz, int64_t ny, in
                                                       const int* table, const int* bb) {
   if (flag_load(flag + 1) == 0) return;  // plan still valid (uniform for the launch)
   extern __shared__ uint64_t skeys[];
-  __shared__ int wo[8], wm[8], base_o, base_m;
+  __shared__ int wo[8], wm[8], wx[8], base_o, base_m;
   const bool use_table = bb[0] != 0;     // uniform
   const uint64_t* keys = gkeys;
   if constexpr (LDS_KEYS) {
@@ -695,30 +698,35 @@ __global__ void __launch_bounds__(512) k_spread_owner(int64_t nz, int64_t ny, in
   const int64_t gid = blockIdx.x * int64_t(blockDim.x) + tid;
   const bool live = (gid >> 6) < n;
   const int m = live ? spread_owner_count(nz, ny, nx, keys, n, gid, table, bb, use_table) : 0;
-  int so = m > 0 ? 1 : 0, sm = m;  // inclusive scans over the wavefront
+  int so = m > 0 ? 1 : 0, sm = m, mx = m;  // inclusive scans (and the maximum) over the wavefront
   for (int sft = 1; sft < 64; sft <<= 1) {
     const int a = __shfl_up(so, sft, 64), b = __shfl_up(sm, sft, 64);
+    const int c = __shfl_xor(mx, sft, 64);
     if (lane >= sft) {
       so += a;
       sm += b;
     }
+    mx = c > mx ? c : mx;
   }
   if (lane == 63) {
     wo[w] = so;
     wm[w] = sm;
+    wx[w] = mx;
   }
   __syncthreads();
   if (tid == 0) {
-    int to = 0, tm = 0;
+    int to = 0, tm = 0, tx = 0;
     for (int i = 0; i < int(blockDim.x >> 6); ++i) {
       const int a = wo[i], b = wm[i];
       wo[i] = to;
       wm[i] = tm;
       to += a;
       tm += b;
+      tx = wx[i] > tx ? wx[i] : tx;
     }
     base_o = to > 0 ? atomicAdd(flag + 3, to) : 0;
     base_m = tm > 0 ? atomicAdd(flag + 2, tm) : 0;
+    if (tx > 0) atomicMax(flag + 4, tx);
   }
   __syncthreads();
   if (live) {
@@ -855,21 +863,34 @@ __device__ __forceinline__ void spread_replay_slot(int ncomp, const EulW3<T>& F,
   for (int c = 0; c < ncomp; ++c) F.p[c][cidx] = acc[c];
 }
 
-// Plan replay, one wavefront per owned cell (the owner list of the plan build): the lanes load
-// up to 64 contributions at once and form their products F_lag[node] w[slot][node]; the
-// products are then added to the cell's value one after another in increasing node order
-// (every lane keeps the same running sum), each sum rounded separately, exactly as the
-// reference's serial `F[block] += F_n * w_n`. Clears the keys-changed flag for the next call.
+// Plan replay over the owner list of the plan build, in one of two forms chosen on the device
+// from the plan's largest contributor count (uniform for the launch):
+// - at most kThreadReplayMax contributors per cell: one thread per owned cell, the per-slot sum
+//   of spread_replay_slot without the idle slots;
+// - more: one wavefront per owned cell. The lanes load up to 64 contributions at once and form
+//   their products F_lag[node] w[slot][node]; the products are then added to the cell's value
+//   one after another in increasing node order (every lane keeps the same running sum).
+// Either way every product and sum is rounded separately, exactly as the reference's serial
+// `F[block] += F_n * w_n`. Clears the keys-changed flag for the next call.
+constexpr int kThreadReplayMax = 32;
+
 template <typename T>
-__global__ void __launch_bounds__(256) k_spread_replay_owners(
+__global__ void __launch_bounds__(256) k_spread_replay(
     int ncomp, EulW3<T> F, int64_t nz, int64_t ny, int64_t nx, const T* lag, const T* weights,
     const uint64_t* keys, int64_t n, const int* pcount, const int* poff, const uint32_t* plist,
     const uint32_t* owners, int* flag) {
 #pragma clang fp contract(off)
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicExch(flag + 1, 0);
+  const int64_t count = flag_load(flag + 3);
+  if (flag_load(flag + 4) <= kThreadReplayMax) {
+    for (int64_t o = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; o < count;
+         o += int64_t(gridDim.x) * blockDim.x)
+      spread_replay_slot(ncomp, F, nz, ny, nx, lag, weights, keys, n, pcount, poff, plist,
+                         int64_t(owners[o]));
+    return;
+  }
   const int lane = int(threadIdx.x & 63);
   const int64_t waves = int64_t(gridDim.x) * (blockDim.x >> 6);
-  const int64_t count = flag_load(flag + 3);
-  if (blockIdx.x == 0 && threadIdx.x == 0) atomicExch(flag + 1, 0);
   for (int64_t o = int64_t(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6); o < count; o += waves) {
     const int64_t gid = int64_t(owners[o]);
     const int m = pcount[gid];
