@@ -19,7 +19,8 @@ def solver_cls():
 
 
 @pytest.mark.parametrize("precision", ["single", "double"])
-@pytest.mark.parametrize("shape,x_range", [((16, 16, 16), 2.0), ((8, 12, 16), 1.0), ((24, 24, 24), 1.0), ((6, 4, 8), 0.7)])
+@pytest.mark.parametrize("shape,x_range", [((16, 16, 16), 2.0), ((8, 12, 16), 1.0), ((24, 24, 24), 1.0), ((6, 4, 8), 0.7),
+                                           ((256, 8, 8), 1.0), ((256, 6, 16), 1.5)])
 def test_vector_solve_matches_oracle(dev, precision, shape, x_range, rng):
     real_t = PREC[precision]
     rhs = rng.standard_normal((3, *shape)).astype(real_t)
