@@ -183,14 +183,15 @@ class CoupledFlowRollout:
                 ck.append(tuple(x.clone() for x in (
                     flow.vorticity_field, flow.velocity_field,
                     inter.lag_grid_position_mismatch_field, inter.lag_grid_velocity_mismatch_field)))
-            dt = float(dt_schedule[s]) if dt_schedule is not None else float(flow.compute_stable_timestep())
-            dts[s] = dt
-            times[s] = t
-            inter.time_step(dt=dt)
+            # the body's state depends on t only: queued before the dt read-back waits
             X, V = self.kin.state(theta, t)
             if record:
                 Xs[s].copy_(X)
                 Vs[s].copy_(V)
+            dt = float(dt_schedule[s]) if dt_schedule is not None else float(flow.compute_stable_timestep())
+            dts[s] = dt
+            times[s] = t
+            inter.time_step(dt=dt)
             inter.compute_interaction_force_on_eul_and_lag_grid(
                 flow.eul_grid_forcing_field, flow.velocity_field, X, V)
             torch.sum(inter.lag_grid_forcing_field, dim=1, out=forces[s])
