@@ -217,12 +217,14 @@ __global__ void __launch_bounds__(kMarchThreads) k_sc_batch(const Member<T>* M, 
 
 template <typename T, int VEC>
 __global__ void __launch_bounds__(kMarchThreads) k_s2_batch(const Member<T>* M, const int64_t* off,
-                                                            int count, int remap) {
+                                                            int count, int remap, T* umax) {
   const int64_t item = xcd_item(remap);
   const int m = find_member(off, count, item);
   const Member<T>& S = M[m];
+  // per work item maximum of |u0| + |u1| + |u2| at slot item: member m's slots are
+  // off[m] .. off[m+1], which the next step's dt reduces
   s2_body<T, VEC>(S.g, S.psi, S.u, T(0.5 / S.dx), T(S.U[0]), T(S.U[1]), T(S.U[2]), S.m_tx, S.m_ty,
-                  item - off[m]);
+                  item - off[m], kMarchChunk, umax + off[m]);
 }
 
 inline unsigned nblocks(int64_t n) { return unsigned(n); }
@@ -250,6 +252,7 @@ struct sopht_ns3d_batch {
   int64_t* off = nullptr;   // device: 6 offset tables of count + 1 entries
   int64_t tot[6] = {0, 0, 0, 0, 0, 0};  // S1 items, march items, CFL blocks, nodes, plan groups, zero blocks
   void* partial = nullptr;  // CFL partial maxima
+  void* s2max = nullptr;    // per S2 work item maxima of the last step's velocity
   std::vector<BatchClass> classes;
   std::vector<sopht_spread3d*> ws;  // spread workspaces of the members with bodies
   int max_padded = 0;
@@ -372,6 +375,7 @@ int batch_build(sopht_ns3d_batch* B, const sopht_batch_member* mem, hipStream_t 
   B->member_bytes = sizeof(Member<T>) * count;
   SOPHT_CHECK_HIP(hipMalloc(&B->off, sizeof(int64_t) * offs.size()));
   SOPHT_CHECK_HIP(hipMalloc(&B->partial, sizeof(T) * std::max<int64_t>(1, B->tot[kOffCfl])));
+  SOPHT_CHECK_HIP(hipMalloc(&B->s2max, sizeof(T) * std::max<int64_t>(1, B->tot[kOffMarch])));
   SOPHT_CHECK_HIP(hipMemcpyAsync(B->members, host.data(), B->member_bytes, hipMemcpyHostToDevice, s));
   SOPHT_CHECK_HIP(hipMemcpyAsync(B->off, offs.data(), sizeof(int64_t) * offs.size(), hipMemcpyHostToDevice, s));
   // Poisson classes
@@ -403,19 +407,28 @@ int batch_build(sopht_ns3d_batch* B, const sopht_batch_member* mem, hipStream_t 
   return SOPHT_OK;
 }
 
+// One batched step. fused_cfl: dt from the maxima that the previous step's S2 left per work
+// item (the velocity is still the one S2 wrote); otherwise from a full read of every member's
+// velocity (the first step of a call: the fields may have changed in between).
 template <typename T>
-int batch_step(sopht_ns3d_batch* B, hipStream_t s) {
+int batch_step(sopht_ns3d_batch* B, hipStream_t s, bool fused_cfl) {
   const Member<T>* M = static_cast<const Member<T>*>(B->members);
   Member<T>* Mw = static_cast<Member<T>*>(B->members);
   const int count = B->count;
   const int64_t* off = B->off;
   auto O = [&](int t) { return off + t * (count + 1); };
   T* partial = static_cast<T*>(B->partial);
+  T* s2max = static_cast<T*>(B->s2max);
   // dt
-  k_cfl_partial_batch<T><<<nblocks(B->tot[kOffCfl]), kCflBlock, 0, s>>>(M, O(kOffCfl), count, partial);
-  SOPHT_CHECK_LAUNCH();
-  k_cfl_dt_batch<T><<<count, kCflBlock, 0, s>>>(Mw, O(kOffCfl), partial);
-  SOPHT_CHECK_LAUNCH();
+  if (fused_cfl) {
+    k_cfl_dt_batch<T><<<count, kCflBlock, 0, s>>>(Mw, O(kOffMarch), s2max);
+    SOPHT_CHECK_LAUNCH();
+  } else {
+    k_cfl_partial_batch<T><<<nblocks(B->tot[kOffCfl]), kCflBlock, 0, s>>>(M, O(kOffCfl), count, partial);
+    SOPHT_CHECK_LAUNCH();
+    k_cfl_dt_batch<T><<<count, kCflBlock, 0, s>>>(Mw, O(kOffCfl), partial);
+    SOPHT_CHECK_LAUNCH();
+  }
   // interactors
   if (B->tot[kOffNode] > 0) {
     k_vbf_time_step_batch<T><<<count, 256, 0, s>>>(M);
@@ -453,15 +466,15 @@ int batch_step(sopht_ns3d_batch* B, hipStream_t s) {
     if (rc) return rc;
   }
   if (B->vec == 1)
-    k_s2_batch<T, 1><<<nblocks(B->tot[kOffMarch]), kMarchThreads, 0, s>>>(M, O(kOffMarch), count, rm);
+    k_s2_batch<T, 1><<<nblocks(B->tot[kOffMarch]), kMarchThreads, 0, s>>>(M, O(kOffMarch), count, rm, s2max);
   else
-    k_s2_batch<T, 16 / sizeof(T)><<<nblocks(B->tot[kOffMarch]), kMarchThreads, 0, s>>>(M, O(kOffMarch), count, rm);
+    k_s2_batch<T, 16 / sizeof(T)><<<nblocks(B->tot[kOffMarch]), kMarchThreads, 0, s>>>(M, O(kOffMarch), count, rm, s2max);
   SOPHT_CHECK_LAUNCH();
   return SOPHT_OK;
 }
 
-int batch_step_any(sopht_ns3d_batch* B, hipStream_t s) {
-  return B->dtype == SOPHT_F32 ? batch_step<float>(B, s) : batch_step<double>(B, s);
+int batch_step_any(sopht_ns3d_batch* B, hipStream_t s, bool fused_cfl) {
+  return B->dtype == SOPHT_F32 ? batch_step<float>(B, s, fused_cfl) : batch_step<double>(B, s, fused_cfl);
 }
 
 }  // namespace
@@ -493,14 +506,17 @@ int sopht_ns3d_batch_step(sopht_ns3d_batch* B, int steps, int use_graph, void* s
   SOPHT_REQUIRE(steps >= 0, "steps must be non-negative");
   hipStream_t s = static_cast<hipStream_t>(stream);
   int k = 0;
-  if (!B->forcing_zeroed && steps > 0) {  // the first step also zeroes F; it is not captured
-    int rc = batch_step_any(B, s);
+  // the first step of a call runs eagerly with dt from a full read of the velocities (the
+  // caller may have changed them since the last call; the very first step also zeroes F);
+  // the following steps take dt from the maxima their predecessor's S2 kept
+  if (steps > 0) {
+    int rc = batch_step_any(B, s, false);
     if (rc) return rc;
     ++k;
   }
   if (!use_graph) {
     for (; k < steps; ++k) {
-      int rc = batch_step_any(B, s);
+      int rc = batch_step_any(B, s, true);
       if (rc) return rc;
     }
     return SOPHT_OK;
@@ -515,7 +531,7 @@ int sopht_ns3d_batch_step(sopht_ns3d_batch* B, int steps, int use_graph, void* s
   SOPHT_CHECK_HIP(hipStreamWaitEvent(B->cs, B->ev_in, 0));
   if (!B->exec) {
     SOPHT_CHECK_HIP(hipStreamBeginCapture(B->cs, hipStreamCaptureModeThreadLocal));
-    int rc = batch_step_any(B, B->cs);
+    int rc = batch_step_any(B, B->cs, true);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(B->cs, &g);
     if (rc) {
@@ -571,6 +587,7 @@ int sopht_ns3d_batch_destroy(sopht_ns3d_batch* B) {
   if (B->members) (void)hipFree(B->members);
   if (B->off) (void)hipFree(B->off);
   if (B->partial) (void)hipFree(B->partial);
+  if (B->s2max) (void)hipFree(B->s2max);
   delete B;
   return SOPHT_OK;
 }

@@ -519,7 +519,7 @@ __device__ __forceinline__ void s2_body(Grid g, const T* __restrict__ psi, T* __
       if (int(threadIdx.x) < k) red[threadIdx.x] = red[threadIdx.x + k] > red[threadIdx.x] ? red[threadIdx.x + k] : red[threadIdx.x];
       __syncthreads();
     }
-    if (threadIdx.x == 0) umax[blockIdx.x] = red[0];
+    if (threadIdx.x == 0) umax[item] = red[0];  // one slot per work item
   }
 }
 
