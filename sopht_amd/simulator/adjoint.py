@@ -140,13 +140,37 @@ class CoupledFlowRollout:
         self.times = None
         self._shadow = None
         self._seg_ws = []  # one spreading workspace per step of a segment (see gradient)
+        # One spreading workspace per step of the rollout when they fit the budget: the
+        # recorded forward spreads step s through workspace s, and the backward's recompute of
+        # step s (same node positions) then replays that plan instead of rebuilding it.
+        self._step_ws = []
+        self._recorded_with_step_ws = False
+        if self.num_steps * self._ws_bytes() <= self.STEP_WS_BUDGET:  # allocated once, here
+            self._workspace(self._step_ws, self.num_steps - 1)
         del torch
 
+    # device bytes of one spreading workspace for n nodes, and the budget of the per-step set
+    STEP_WS_BUDGET = 8 << 30
+
+    def _ws_bytes(self):
+        n = max(self.n, 1)
+        padded = 1 << (n - 1).bit_length()
+        return 4 * 64 * n * 4 + 2 * 8 * padded + 4 * (32768 + 1) + 64
+
+    def _workspace(self, store, i):
+        while len(store) <= i:
+            w = C.c_void_p()
+            L.call("sopht_spread3d_create", int(max(self.n, 1)), C.byref(w))
+            store.append(w)
+        return store[i]
+
     def __del__(self):
-        for w in getattr(self, "_seg_ws", []):
-            if w.value:
-                self._lib.sopht_spread3d_destroy(w)
+        for store in (getattr(self, "_seg_ws", []), getattr(self, "_step_ws", [])):
+            for w in store:
+                if w.value:
+                    self._lib.sopht_spread3d_destroy(w)
         self._seg_ws = []
+        self._step_ws = []
 
     # ---------------------------------------------------------------- forward
     def _restore(self):
@@ -177,6 +201,9 @@ class CoupledFlowRollout:
         prep = getattr(self.kin, "prepare", None)
         if prep is not None:
             prep(theta)
+        step_ws = record and len(self._step_ws) >= ns
+        comm = inter.eul_lag_grid_communicator
+        own_ws = comm._spread_ws
         t = self.time0
         for s in range(ns):
             if record and s % k == 0:
@@ -192,8 +219,13 @@ class CoupledFlowRollout:
             dts[s] = dt
             times[s] = t
             inter.time_step(dt=dt)
-            inter.compute_interaction_force_on_eul_and_lag_grid(
-                flow.eul_grid_forcing_field, flow.velocity_field, X, V)
+            if step_ws:
+                comm._spread_ws = self._workspace(self._step_ws, s)
+            try:
+                inter.compute_interaction_force_on_eul_and_lag_grid(
+                    flow.eul_grid_forcing_field, flow.velocity_field, X, V)
+            finally:
+                comm._spread_ws = own_ws
             torch.sum(inter.lag_grid_forcing_field, dim=1, out=forces[s])
             forces[s].neg_()
             flow.time_step(dt=dt, free_stream_velocity=self.U)
@@ -205,6 +237,7 @@ class CoupledFlowRollout:
             self._X = Xs
             self._V = Vs
             self._theta = theta.clone()
+            self._recorded_with_step_ws = step_ws
         return forces
 
     # ---------------------------------------------------------------- backward
@@ -254,13 +287,15 @@ class CoupledFlowRollout:
         cc = float(sh.virtual_boundary_damping_coeff)
         st = L.stream_ptr()
         lib = self._lib
-        # step j of a segment spreads with workspace j in the recompute and in the reverse
-        # sweep: the reverse sweep's spreading (same node positions) replays the plan the
-        # recompute built instead of rebuilding it
-        while len(self._seg_ws) < seg:
-            w = C.c_void_p()
-            L.call("sopht_spread3d_create", int(max(n, 1)), C.byref(w))
-            self._seg_ws.append(w)
+        # step s spreads with the workspace the recorded forward used for it (its plan is then
+        # replayed by the recompute and by the reverse sweep), or else with workspace j of the
+        # segment, whose plan the recompute builds and the reverse sweep replays
+        if self._recorded_with_step_ws:
+            ws_of = lambda s, j: self._step_ws[s]  # noqa: E731
+        else:
+            for j in range(seg):
+                self._workspace(self._seg_ws, j)
+            ws_of = lambda s, j: self._seg_ws[j]  # noqa: E731
         Fst = L.strides3(F[0])
         Fp = L.comps(F)
         for g in reversed(range(len(self._ck))):
@@ -278,7 +313,7 @@ class CoupledFlowRollout:
                 sh.compute_interaction_force_on_lag_grid(U[j], self._X[s], self._V[s])
                 Fl[j].copy_(sh.lag_grid_forcing_field)
                 L.check(lib.sopht_lag_to_eul_spread_3d(
-                    self._tag, 3, self._seg_ws[j], Fp, Fst, nz, ny, nx, L.ptr(Fl[j]),
+                    self._tag, 3, ws_of(s, j), Fp, Fst, nz, ny, nx, L.ptr(Fl[j]),
                     L.ptr(sh.interp_weights), L.ptr(sh.nearest_eul_grid_index_to_lag_grid), n, st))
                 flow.trajectory_step(W[j % 2], U[j], F, W[(j + 1) % 2], U[j + 1], dt, self.U,
                                      vorticity1_out=W1[j])
@@ -292,7 +327,7 @@ class CoupledFlowRollout:
                 L.check(lib.sopht_vbf_adjoint_3d(
                     self._tag, L.ptr(wbar), pF, L.ptr(U[j]), nz, ny, nx, L.ptr(self._X[s]),
                     L.ptr(Fl[j]), n, dx, shift, kk, cc, dt, L.ptr(fbar[s]), L.ptr(dXb), L.ptr(dVb),
-                    L.ptr(Xb[s]), L.ptr(Vb[s]), L.ptr(wts), L.ptr(idx), L.ptr(lag), self._seg_ws[j],
+                    L.ptr(Xb[s]), L.ptr(Vb[s]), L.ptr(wts), L.ptr(idx), L.ptr(lag), ws_of(s, j),
                     L.ptr(ubar), st))
         with torch.enable_grad():
             th = self._theta.detach().clone().requires_grad_(True)
