@@ -168,14 +168,20 @@ def build_rollout(shape, real_t, n_nodes, semi_axes, nu, period, num_steps, chec
     return ro, X0, k, c, U
 
 
-def test_rollout_gradient_matches_oracle(dev):
+@pytest.mark.parametrize("step_workspaces", [True, False])
+def test_rollout_gradient_matches_oracle(dev, step_workspaces, monkeypatch):
+    """With one spreading workspace per recorded step (the backward replays the forward's
+    plans) and, past the workspace budget, per step of a segment (the recompute rebuilds)."""
     import torch
 
+    if not step_workspaces:
+        monkeypatch.setattr(sps().CoupledFlowRollout, "STEP_WS_BUDGET", 0)
     shape = (16, 16, 16)
     nu, period, ns = 1e-2, 0.02, 6
     omega0 = synthetic.random_phase_vorticity(shape, 1.0, max_wavenumber=3, seed=3, max_vorticity=4.0)
     ro, X0, k, c, U = build_rollout(shape, np.float64, 40, (0.2, 0.12, 0.12), nu, period, ns, 4,
                                     omega0=omega0, k_coef=-60.0, c_coef=-3.0)
+    assert (len(ro._step_ws) == ns) == step_workspaces
     theta = torch.tensor(np.random.default_rng(7).normal(0.0, 0.5, 16), device=dev, requires_grad=True)
     forces = sps().body_forces(theta, ro)
     J = sps().time_averaged_force(forces, ro.dts)
