@@ -854,6 +854,9 @@ __device__ __forceinline__ void spread_replay_slot(int ncomp, const EulW3<T>& F,
   T acc[3];
   for (int c = 0; c < ncomp; ++c) acc[c] = F.p[c][cidx];
   const int o = poff[gid];
+  // unrolled so that the loads of several contributions are in flight together; the sums
+  // still run one after another in list order
+#pragma unroll 4
   for (int k = 0; k < m; ++k) {
     const uint32_t e = plist[o + k];
     const int64_t node = int64_t(e >> 6);
