@@ -253,6 +253,7 @@ struct sopht_ns3d_batch {
   int64_t tot[6] = {0, 0, 0, 0, 0, 0};  // S1 items, march items, CFL blocks, nodes, plan groups, zero blocks
   void* partial = nullptr;  // CFL partial maxima
   void* s2max = nullptr;    // per S2 work item maxima of the last step's velocity
+  int plans_built = 0;      // the spreading plans of the (static) bodies exist
   std::vector<BatchClass> classes;
   std::vector<sopht_spread3d*> ws;  // spread workspaces of the members with bodies
   int max_padded = 0;
@@ -443,11 +444,16 @@ int batch_step(sopht_ns3d_batch* B, hipStream_t s, bool fused_cfl) {
     k_vbf_lag_force_batch<T><<<nblocks(ceil_div(B->tot[kOffNode], 4)), 256, 0, s>>>(
         M, O(kOffNode), count, B->tot[kOffNode]);
     SOPHT_CHECK_LAUNCH();
-    k_spread_sort_batch<T><<<count, 1024, size_t(std::max(B->max_padded, 1)) * sizeof(uint64_t), s>>>(M);
-    SOPHT_CHECK_LAUNCH();
-    k_spread_plan_batch<T><<<nblocks(B->tot[kOffPlan]), 512, size_t(std::max(B->max_padded, 1)) * sizeof(uint64_t), s>>>(
-        M, O(kOffPlan), count);
-    SOPHT_CHECK_LAUNCH();
+    // the bodies keep the positions they had when the batch was built, so the plans built by
+    // the first step stay valid: later steps only replay them
+    if (!B->plans_built) {
+      k_spread_sort_batch<T><<<count, 1024, size_t(std::max(B->max_padded, 1)) * sizeof(uint64_t), s>>>(M);
+      SOPHT_CHECK_LAUNCH();
+      k_spread_plan_batch<T><<<nblocks(B->tot[kOffPlan]), 512, size_t(std::max(B->max_padded, 1)) * sizeof(uint64_t), s>>>(
+          M, O(kOffPlan), count);
+      SOPHT_CHECK_LAUNCH();
+      B->plans_built = 1;
+    }
     k_spread_replay_batch<T><<<nblocks(B->tot[kOffPlan]), 512, 0, s>>>(M, O(kOffPlan), count);
     SOPHT_CHECK_LAUNCH();
   }
