@@ -195,24 +195,28 @@ __global__ void __launch_bounds__(256) k_zero_forcing_batch(const Member<T>* M, 
 // ---- flow sweeps; prefactors as the single-simulator step forms them from dt
 template <typename T>
 __global__ void __launch_bounds__(S1Tile<T>::NT) __attribute__((amdgpu_waves_per_eu(6)))
-k_s1_batch(const Member<T>* M, const int64_t* off, int count, int remap) {
+k_s1_batch(const Member<T>* M, const int64_t* off, int count, int remap, int* fflags) {
   const int64_t item = xcd_item(remap);
   const int m = find_member(off, count, item);
   const Member<T>& S = M[m];
   const T pF = T(S.dt / (2.0 * S.dx * S.rho));
   const T pB = T(S.dt / (2.0 * S.dx));
+  // per S1 tile: did it see a bit of F set (member m's flags start at its first S1 item)
   s1_body<T>(S.g, S.w, S.u, S.F, S.psi, pF, pB, S.with_forcing, kS1Chunk, S.s1_tx, S.s1_ty,
-             item - off[m]);
+             item - off[m], nullptr, fflags + off[m]);
 }
 
 template <typename T, int VEC>
 __global__ void __launch_bounds__(kMarchThreads) k_sc_batch(const Member<T>* M, const int64_t* off,
-                                                            int count, int remap) {
+                                                            int count, int remap, const int* fflags,
+                                                            const int64_t* off_s1) {
   const int64_t item = xcd_item(remap);
   const int m = find_member(off, count, item);
   const Member<T>& S = M[m];
   const T pD = T(S.nu * S.dt / S.dx / S.dx);
-  sc_body<T, VEC>(S.g, S.psi, S.w, S.F, pD, S.pen, S.with_forcing, S.m_tx, S.m_ty, item - off[m]);
+  // F is zeroed only in the S1 tiles that saw a bit of it set (elsewhere it is zero already)
+  sc_body<T, VEC>(S.g, S.psi, S.w, S.F, pD, S.pen, S.with_forcing, S.m_tx, S.m_ty, item - off[m],
+                  kMarchChunk, fflags + off_s1[m], S.s1_tx, S.s1_ty);
 }
 
 template <typename T, int VEC>
@@ -254,6 +258,7 @@ struct sopht_ns3d_batch {
   void* partial = nullptr;  // CFL partial maxima
   void* s2max = nullptr;    // per S2 work item maxima of the last step's velocity
   int plans_built = 0;      // the spreading plans of the (static) bodies exist
+  int* s1flags = nullptr;   // per S1 work item: did it see a bit of F set
   std::vector<BatchClass> classes;
   std::vector<sopht_spread3d*> ws;  // spread workspaces of the members with bodies
   int max_padded = 0;
@@ -377,6 +382,7 @@ int batch_build(sopht_ns3d_batch* B, const sopht_batch_member* mem, hipStream_t 
   SOPHT_CHECK_HIP(hipMalloc(&B->off, sizeof(int64_t) * offs.size()));
   SOPHT_CHECK_HIP(hipMalloc(&B->partial, sizeof(T) * std::max<int64_t>(1, B->tot[kOffCfl])));
   SOPHT_CHECK_HIP(hipMalloc(&B->s2max, sizeof(T) * std::max<int64_t>(1, B->tot[kOffMarch])));
+  SOPHT_CHECK_HIP(hipMalloc(&B->s1flags, sizeof(int) * std::max<int64_t>(1, B->tot[kOffS1])));
   SOPHT_CHECK_HIP(hipMemcpyAsync(B->members, host.data(), B->member_bytes, hipMemcpyHostToDevice, s));
   SOPHT_CHECK_HIP(hipMemcpyAsync(B->off, offs.data(), sizeof(int64_t) * offs.size(), hipMemcpyHostToDevice, s));
   // Poisson classes
@@ -459,13 +465,15 @@ int batch_step(sopht_ns3d_batch* B, hipStream_t s, bool fused_cfl) {
   }
   // flow
   const int r1 = B->tot[kOffS1] % 8 == 0 ? 1 : 0;
-  k_s1_batch<T><<<nblocks(B->tot[kOffS1]), S1Tile<T>::NT, 0, s>>>(M, O(kOffS1), count, r1);
+  k_s1_batch<T><<<nblocks(B->tot[kOffS1]), S1Tile<T>::NT, 0, s>>>(M, O(kOffS1), count, r1, B->s1flags);
   SOPHT_CHECK_LAUNCH();
   const int rm = B->tot[kOffMarch] % 8 == 0 ? 1 : 0;
   if (B->vec == 1)
-    k_sc_batch<T, 1><<<nblocks(B->tot[kOffMarch]), kMarchThreads, 0, s>>>(M, O(kOffMarch), count, rm);
+    k_sc_batch<T, 1><<<nblocks(B->tot[kOffMarch]), kMarchThreads, 0, s>>>(M, O(kOffMarch), count, rm,
+                                                                         B->s1flags, O(kOffS1));
   else
-    k_sc_batch<T, 16 / sizeof(T)><<<nblocks(B->tot[kOffMarch]), kMarchThreads, 0, s>>>(M, O(kOffMarch), count, rm);
+    k_sc_batch<T, 16 / sizeof(T)><<<nblocks(B->tot[kOffMarch]), kMarchThreads, 0, s>>>(
+        M, O(kOffMarch), count, rm, B->s1flags, O(kOffS1));
   SOPHT_CHECK_LAUNCH();
   for (BatchClass& C : B->classes) {
     int rc = poisson_table_solve(C.plan, C.ncomp, C.sol_tab, C.strides, C.rhs_tab, C.strides, s);
@@ -594,6 +602,7 @@ int sopht_ns3d_batch_destroy(sopht_ns3d_batch* B) {
   if (B->off) (void)hipFree(B->off);
   if (B->partial) (void)hipFree(B->partial);
   if (B->s2max) (void)hipFree(B->s2max);
+  if (B->s1flags) (void)hipFree(B->s1flags);
   delete B;
   return SOPHT_OK;
 }
