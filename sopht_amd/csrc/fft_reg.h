@@ -327,6 +327,58 @@ __device__ __forceinline__ void stockham_stage(Cplx<T>* v, Cplx<T>* xb, const Cp
   }
 }
 
+// Twiddles of butterfly input r of a stage with write stride NS and radix R, for butterfly k of
+// its group (k = j % NS): w^{r k} with w = exp(-2 pi i / (NS R)), computed as stockham_stage
+// does (r = 2^b from the table, the other powers as products of two of them).
+template <typename T, int L, int R, int NS>
+__device__ __forceinline__ void stage_twiddles(Cplx<T>* wr, const Cplx<T>* tw, int k) {
+  constexpr int step = L / (NS * R);
+  static_for<1, R>([&](auto r_) {
+    constexpr int r = decltype(r_)::value;
+    constexpr int hb = kTwiddleProducts ? 1 << ilog2(r) : r;
+    if constexpr (hb == r) wr[r] = tw[r * k * step];
+    else wr[r] = cmul(wr[hb], wr[r - hb]);
+  });
+}
+
+// The last stage of a plan (write stride NS, NS R = L, one butterfly per thread) with the
+// twiddles of the thread's butterfly given in wr (stage_twiddles with k = t % NS).
+template <typename T, int L, int R, int NS, bool INV>
+__device__ __forceinline__ void stockham_last_given(Cplx<T>* v, const Cplx<T>* wr) {
+  static_assert(LinePlan<L>::E == R && NS * R == L, "last stage, one butterfly per thread");
+  static_for<1, R>([&](auto r_) {
+    constexpr int r = decltype(r_)::value;
+    v[r] = INV ? cmulc(v[r], wr[r]) : cmul(v[r], wr[r]);
+  });
+  rdft<T, R, INV>(v);
+}
+
+// A stage with an exchange after it (write stride NS, one butterfly per thread) with the
+// twiddles of the thread's butterfly given in wr: the butterfly, the exchange writes, and the
+// read-back x[t + T e] for the next stage, as stockham_stage.
+template <typename T, int L, int R, int NS, bool INV>
+__device__ __forceinline__ void stockham_stage_given(Cplx<T>* v, Cplx<T>* xb, const Cplx<T>* wr,
+                                                     int t) {
+  constexpr int E = LinePlan<L>::E, TT = LinePlan<L>::T;
+  static_assert(E == R && NS > 1 && NS * R < L, "a middle stage, one butterfly per thread");
+  static_for<1, R>([&](auto r_) {
+    constexpr int r = decltype(r_)::value;
+    v[r] = INV ? cmulc(v[r], wr[r]) : cmul(v[r], wr[r]);
+  });
+  rdft<T, R, INV>(v);
+  const int base = (t / NS) * NS * R + t % NS;
+  static_for<0, R>([&](auto r_) {
+    constexpr int r = decltype(r_)::value;
+    xb[Swizzle<L, NS, R>::map(base + r * NS)] = v[r];
+  });
+  wave_sync();
+  static_for<0, E>([&](auto e_) {
+    constexpr int e = decltype(e_)::value;
+    v[e] = xb[Swizzle<L, NS, R>::map(t + TT * e)];
+  });
+  wave_sync();
+}
+
 template <typename T, int L, bool INV, bool HALF_IN, int NS, int R0, int... Rest>
 __device__ __forceinline__ void stockham_run(Cplx<T>* v, Cplx<T>* xb, const Cplx<T>* tw, int t) {
   stockham_stage<T, L, R0, NS, INV, HALF_IN && NS == 1>(v, xb, tw, t);

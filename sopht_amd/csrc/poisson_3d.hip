@@ -250,6 +250,25 @@ constexpr int kZconvWaves = sizeof(T) == 4 ? 4 : 2;
 // pencil tiles per z-pass workgroup at most (all of them for a 3-component solve)
 constexpr int kZconvJobs = 6;
 
+// Three-stage line plans whose second and last radix equal the elements per thread (L = 512:
+// 8 x 8 x 8): those stages are one butterfly per thread, so their twiddles depend on the
+// thread only and a z-pass workgroup computes them once for all its pencils.
+template <typename R>
+struct Plan3 {
+  static constexpr bool ok = false;
+  static constexpr int r0 = 0, r1 = 0, r2 = 0;
+};
+template <int A, int B, int C>
+struct Plan3<rfft::Radices<A, B, C>> {
+  static constexpr bool ok = true;
+  static constexpr int r0 = A, r1 = B, r2 = C;
+};
+template <int L>
+constexpr bool last_twiddles_hoisted() {
+  using P = Plan3<typename LinePlan<L>::R>;
+  return P::ok && P::r1 == LinePlan<L>::E && P::r2 == LinePlan<L>::E;
+}
+
 // K3: z pencils (c, ky, kx): zero-padded forward FFT, times G^ (real, even), inverse FFT,
 // crop to z < nz. A workgroup owns one kx tile of one |ky| = kyo and runs the pencils of
 // every component at ky = kyo and ky = 2ny - kyo (the same G^ line) one after another: G^
@@ -287,6 +306,14 @@ __global__ void __launch_bounds__((ColTile<T, L>::NT)) __attribute__((amdgpu_wav
     });
   }
   const int zs = 2 * int(g.ny) * int(g.rs);  // stride between consecutive z
+  // the twiddles of this thread's butterflies in the second and last stages, the same for
+  // every pencil of the workgroup
+  Cplx<T> w2[E], w3[E];
+  if constexpr (last_twiddles_hoisted<L>()) {
+    using P3 = Plan3<typename LinePlan<L>::R>;
+    rfft::stage_twiddles<T, L, P3::r1, P3::r0>(w2, tw, t % P3::r0);
+    rfft::stage_twiddles<T, L, P3::r2, P3::r0 * P3::r1>(w3, tw, t % (P3::r0 * P3::r1));
+  }
   auto column = [&](int jl) -> int {
     const int j = jbeg + jl;
     const int c = j % g.ncomp;
@@ -319,12 +346,27 @@ __global__ void __launch_bounds__((ColTile<T, L>::NT)) __attribute__((amdgpu_wav
     rfft::wave_sync();
     if (j + 1 < njobs) load(j + 1);
     if (wave_live) {
-      team_fft<T, L, false, true>(v, line, tw, t);
-      static_for<0, E>([&](auto e_) {
-        constexpr int e = decltype(e_)::value;
-        v[e] = Cplx<T>{v[e].re * gv[e], v[e].im * gv[e]};
-      });
-      team_fft<T, L, true>(v, line, tw, t);
+      if constexpr (last_twiddles_hoisted<L>()) {
+        using P3 = Plan3<typename LinePlan<L>::R>;
+        constexpr int R0 = P3::r0, R1 = P3::r1, R2 = P3::r2, NS3 = R0 * R1;
+        rfft::stockham_stage<T, L, R0, 1, false, true>(v, line, tw, t);
+        rfft::stockham_stage_given<T, L, R1, R0, false>(v, line, w2, t);
+        rfft::stockham_last_given<T, L, R2, NS3, false>(v, w3);
+        static_for<0, E>([&](auto e_) {
+          constexpr int e = decltype(e_)::value;
+          v[e] = Cplx<T>{v[e].re * gv[e], v[e].im * gv[e]};
+        });
+        rfft::stockham_stage<T, L, R0, 1, true, false>(v, line, tw, t);
+        rfft::stockham_stage_given<T, L, R1, R0, true>(v, line, w2, t);
+        rfft::stockham_last_given<T, L, R2, NS3, true>(v, w3);
+      } else {
+        team_fft<T, L, false, true>(v, line, tw, t);
+        static_for<0, E>([&](auto e_) {
+          constexpr int e = decltype(e_)::value;
+          v[e] = Cplx<T>{v[e].re * gv[e], v[e].im * gv[e]};
+        });
+        team_fft<T, L, true>(v, line, tw, t);
+      }
     }
     static_for<0, K>([&](auto e_) {
       constexpr int e = decltype(e_)::value;
