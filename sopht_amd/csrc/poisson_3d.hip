@@ -215,6 +215,9 @@ __global__ void __launch_bounds__((ColTile<T, L>::NT)) k_ycols(SlabGeom g, Cplx<
   const bool wave_live = kx0 + (tid / 64) * (64 / TT) < ncols;
   const int col = g.hi(c, int(z), 0, live ? int(kx) : 0);
   const int ys = int(g.rs);  // stride between consecutive y (or ky)
+  // the twiddle table in LDS (one load per thread), read by every stage of the 8 transforms
+  Cplx<T>* const twl = xb + WC * S;
+  for (int i = tid; i < L; i += CT::NT) twl[i] = tw[i];
   static_for<0, KIN>([&](auto k_) {
     constexpr int k = decltype(k_)::value;
     const int y = lr + TT * k;
@@ -229,7 +232,7 @@ __global__ void __launch_bounds__((ColTile<T, L>::NT)) k_ycols(SlabGeom g, Cplx<
     else v[e] = Cplx<T>{T(0), T(0)};
   });
   rfft::wave_sync();
-  if (wave_live) team_fft<T, L, INV, !INV>(v, line, tw, t);
+  if (wave_live) team_fft<T, L, INV, !INV>(v, line, twl, t);
   static_for<0, KOUT>([&](auto e_) {
     constexpr int e = decltype(e_)::value;
     line[t + TT * e] = v[e];
@@ -522,7 +525,7 @@ int launch_ycols(int64_t L, SlabGeom g, Cplx<T>* H, const Cplx<T>* tw, hipStream
   if (L == n) {                                                                       \
     using CT = ColTile<T, n>;                                                         \
     const int64_t nb = col_tile_blocks(g.nx + 1, int64_t(g.ncomp) * g.nz, CT::WC);    \
-    k_ycols<T, n, INV><<<nb, CT::NT, CT::lds, s>>>(g, H, tw);                         \
+    k_ycols<T, n, INV><<<nb, CT::NT, CT::lds + n * sizeof(Cplx<T>), s>>>(g, H, tw);   \
     SOPHT_CHECK_LAUNCH();                                                             \
     return SOPHT_OK;                                                                  \
   }
