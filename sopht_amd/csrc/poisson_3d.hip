@@ -176,7 +176,11 @@ __global__ void __launch_bounds__((RowTile<T, L>::NT)) k1_xr2c(SlabGeom g, CompP
     }
   });
   Cplx<T>* const line = xb + team * RT::S;
-  team_fft<T, L, false, true>(v, line, tw, t);
+  // the twiddle table in LDS, shared by the workgroup's teams
+  Cplx<T>* const twl = xb + RT::NTEAM * RT::S;
+  for (int i = threadIdx.x; i < L; i += RT::NT) twl[i] = tw[i];
+  __syncthreads();
+  team_fft<T, L, false, true>(v, line, twl, t);
   static_for<0, E>([&](auto e_) {
     constexpr int e = decltype(e_)::value;
     line[t + TT * e] = v[e];
@@ -408,6 +412,9 @@ __global__ void __launch_bounds__((RowTile<T, L>::NT)) k5_xc2r(SlabGeom g, const
   const Cplx<T>* ha = ra < rows ? H + g.row_h(ra) : nullptr;
   const Cplx<T>* hb = rb < rows ? H + g.row_h(rb) : nullptr;
   Cplx<T>* const line = xb + team * RT::S;
+  // the twiddle table in LDS, shared by the workgroup's teams (one barrier, below)
+  Cplx<T>* const twl = xb + RT::NTEAM * RT::S;
+  for (int i = threadIdx.x; i < L; i += RT::NT) twl[i] = tw[i];
   for (int k = t; k <= nx; k += TT) {
     Cplx<T> a = {T(0), T(0)}, bb = {T(0), T(0)};
     if (ha) a = ha[k];
@@ -425,8 +432,8 @@ __global__ void __launch_bounds__((RowTile<T, L>::NT)) k5_xc2r(SlabGeom g, const
     constexpr int e = decltype(e_)::value;
     v[e] = line[t + TT * e];
   });
-  rfft::wave_sync();
-  team_fft<T, L, true>(v, line, tw, t);
+  __syncthreads();  // the twiddle table is complete
+  team_fft<T, L, true>(v, line, twl, t);
   auto row_ptr = [&](int64_t r) -> T* {
     const int c = int(r / (g.nz * g.ny));
     const int64_t rem = r % (g.nz * g.ny);
@@ -510,7 +517,7 @@ int launch_k1(int64_t L, SlabGeom g, CompPtrs in, Cplx<T>* H, const Cplx<T>* tw,
 #define X(n)                                                                          \
   if (L == n) {                                                                       \
     using RT = RowTile<T, n>;                                                         \
-    k1_xr2c<T, n><<<ceil_div(pairs, RT::NTEAM), RT::NT, RT::lds, s>>>(g, in, H, tw);  \
+    k1_xr2c<T, n><<<ceil_div(pairs, RT::NTEAM), RT::NT, RT::lds + n * sizeof(Cplx<T>), s>>>(g, in, H, tw); \
     SOPHT_CHECK_LAUNCH();                                                             \
     return SOPHT_OK;                                                                  \
   }
@@ -560,9 +567,9 @@ int launch_k5(int64_t L, SlabGeom g, const Cplx<T>* H, CompPtrsW out, const Cplx
   if (L == n) {                                                                       \
     using RT = RowTile<T, n>;                                                         \
     if (acc)                                                                          \
-      k5_xc2r<T, n, true><<<ceil_div(pairs, RT::NTEAM), RT::NT, RT::lds, s>>>(g, H, out, tw); \
+      k5_xc2r<T, n, true><<<ceil_div(pairs, RT::NTEAM), RT::NT, RT::lds + n * sizeof(Cplx<T>), s>>>(g, H, out, tw); \
     else                                                                              \
-      k5_xc2r<T, n, false><<<ceil_div(pairs, RT::NTEAM), RT::NT, RT::lds, s>>>(g, H, out, tw); \
+      k5_xc2r<T, n, false><<<ceil_div(pairs, RT::NTEAM), RT::NT, RT::lds + n * sizeof(Cplx<T>), s>>>(g, H, out, tw); \
     SOPHT_CHECK_LAUNCH();                                                             \
     return SOPHT_OK;                                                                  \
   }
