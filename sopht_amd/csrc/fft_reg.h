@@ -341,42 +341,44 @@ __device__ __forceinline__ void stage_twiddles(Cplx<T>* wr, const Cplx<T>* tw, i
   });
 }
 
-// The last stage of a plan (write stride NS, NS R = L, one butterfly per thread) with the
-// twiddles of the thread's butterfly given in wr (stage_twiddles with k = t % NS).
+// A stage of radix R and write stride NS whose twiddles are given: wr[p][r] for butterfly p of
+// the thread (j = t + p T, from stage_twiddles with k = j % NS). Without an exchange after it
+// (NS R = L, the last stage) the outputs stay in v; otherwise the exchange writes and the
+// read-back x[t + T e] for the next stage follow, as in stockham_stage.
 template <typename T, int L, int R, int NS, bool INV>
-__device__ __forceinline__ void stockham_last_given(Cplx<T>* v, const Cplx<T>* wr) {
-  static_assert(LinePlan<L>::E == R && NS * R == L, "last stage, one butterfly per thread");
-  static_for<1, R>([&](auto r_) {
-    constexpr int r = decltype(r_)::value;
-    v[r] = INV ? cmulc(v[r], wr[r]) : cmul(v[r], wr[r]);
-  });
-  rdft<T, R, INV>(v);
-}
-
-// A stage with an exchange after it (write stride NS, one butterfly per thread) with the
-// twiddles of the thread's butterfly given in wr: the butterfly, the exchange writes, and the
-// read-back x[t + T e] for the next stage, as stockham_stage.
-template <typename T, int L, int R, int NS, bool INV>
-__device__ __forceinline__ void stockham_stage_given(Cplx<T>* v, Cplx<T>* xb, const Cplx<T>* wr,
-                                                     int t) {
+__device__ __forceinline__ void stockham_stage_given(Cplx<T>* v, Cplx<T>* xb,
+                                                     const Cplx<T> (*wr)[R], int t) {
   constexpr int E = LinePlan<L>::E, TT = LinePlan<L>::T;
-  static_assert(E == R && NS > 1 && NS * R < L, "a middle stage, one butterfly per thread");
-  static_for<1, R>([&](auto r_) {
-    constexpr int r = decltype(r_)::value;
-    v[r] = INV ? cmulc(v[r], wr[r]) : cmul(v[r], wr[r]);
+  constexpr int PER = E / R;
+  static_assert(E % R == 0 && NS > 1, "a stage with twiddles");
+  static_for<0, PER>([&](auto p_) {
+    constexpr int p = decltype(p_)::value;
+    Cplx<T> u[R];
+    static_for<0, R>([&](auto r_) { u[decltype(r_)::value] = v[p + PER * decltype(r_)::value]; });
+    static_for<1, R>([&](auto r_) {
+      constexpr int r = decltype(r_)::value;
+      u[r] = INV ? cmulc(u[r], wr[p][r]) : cmul(u[r], wr[p][r]);
+    });
+    rdft<T, R, INV>(u);
+    static_for<0, R>([&](auto r_) { v[p + PER * decltype(r_)::value] = u[decltype(r_)::value]; });
   });
-  rdft<T, R, INV>(v);
-  const int base = (t / NS) * NS * R + t % NS;
-  static_for<0, R>([&](auto r_) {
-    constexpr int r = decltype(r_)::value;
-    xb[Swizzle<L, NS, R>::map(base + r * NS)] = v[r];
-  });
-  wave_sync();
-  static_for<0, E>([&](auto e_) {
-    constexpr int e = decltype(e_)::value;
-    v[e] = xb[Swizzle<L, NS, R>::map(t + TT * e)];
-  });
-  wave_sync();
+  if constexpr (NS * R < L) {
+    static_for<0, PER>([&](auto p_) {
+      constexpr int p = decltype(p_)::value;
+      const int j = t + p * TT;
+      const int base = (j / NS) * NS * R + j % NS;
+      static_for<0, R>([&](auto r_) {
+        constexpr int r = decltype(r_)::value;
+        xb[Swizzle<L, NS, R>::map(base + r * NS)] = v[p + PER * r];
+      });
+    });
+    wave_sync();
+    static_for<0, E>([&](auto e_) {
+      constexpr int e = decltype(e_)::value;
+      v[e] = xb[Swizzle<L, NS, R>::map(t + TT * e)];
+    });
+    wave_sync();
+  }
 }
 
 template <typename T, int L, bool INV, bool HALF_IN, int NS, int R0, int... Rest>

@@ -257,9 +257,8 @@ constexpr int kZconvWaves = sizeof(T) == 4 ? 4 : 2;
 // pencil tiles per z-pass workgroup at most (all of them for a 3-component solve)
 constexpr int kZconvJobs = 6;
 
-// Three-stage line plans whose second and last radix equal the elements per thread (L = 512:
-// 8 x 8 x 8): those stages are one butterfly per thread, so their twiddles depend on the
-// thread only and a z-pass workgroup computes them once for all its pencils.
+// Three-stage line plans: the twiddles of a thread's butterflies in the second and last stages
+// depend on the thread only, so a z-pass workgroup computes them once for all its pencils.
 template <typename R>
 struct Plan3 {
   static constexpr bool ok = false;
@@ -271,10 +270,11 @@ struct Plan3<rfft::Radices<A, B, C>> {
   static constexpr int r0 = A, r1 = B, r2 = C;
 };
 template <int L>
-constexpr bool last_twiddles_hoisted() {
+constexpr bool last_twiddles_hoisted() {  // the E = 8 plans (L = 128, 256, 512): 24-28 VGPRs
   using P = Plan3<typename LinePlan<L>::R>;
-  return P::ok && P::r1 == LinePlan<L>::E && P::r2 == LinePlan<L>::E;
+  return P::ok && LinePlan<L>::E == 8;
 }
+constexpr int at_least_1(int v) { return v > 0 ? v : 1; }
 
 // K3: z pencils (c, ky, kx): zero-padded forward FFT, times G^ (real, even), inverse FFT,
 // crop to z < nz. A workgroup owns one kx tile of one |ky| = kyo and runs the pencils of
@@ -315,11 +315,19 @@ __global__ void __launch_bounds__((ColTile<T, L>::NT)) __attribute__((amdgpu_wav
   const int zs = 2 * int(g.ny) * int(g.rs);  // stride between consecutive z
   // the twiddles of this thread's butterflies in the second and last stages, the same for
   // every pencil of the workgroup
-  Cplx<T> w2[E], w3[E];
+  using P3 = Plan3<typename LinePlan<L>::R>;
+  constexpr int R1 = at_least_1(P3::r1), R2 = at_least_1(P3::r2);
+  constexpr int PER2 = at_least_1(E / R1), PER3 = at_least_1(E / R2);
+  Cplx<T> w2[PER2][R1], w3[PER3][R2];
   if constexpr (last_twiddles_hoisted<L>()) {
-    using P3 = Plan3<typename LinePlan<L>::R>;
-    rfft::stage_twiddles<T, L, P3::r1, P3::r0>(w2, tw, t % P3::r0);
-    rfft::stage_twiddles<T, L, P3::r2, P3::r0 * P3::r1>(w3, tw, t % (P3::r0 * P3::r1));
+    static_for<0, PER2>([&](auto p_) {
+      constexpr int p = decltype(p_)::value;
+      rfft::stage_twiddles<T, L, R1, P3::r0>(w2[p], tw, (t + p * TT) % P3::r0);
+    });
+    static_for<0, PER3>([&](auto p_) {
+      constexpr int p = decltype(p_)::value;
+      rfft::stage_twiddles<T, L, R2, P3::r0 * R1>(w3[p], tw, (t + p * TT) % (P3::r0 * R1));
+    });
   }
   auto column = [&](int jl) -> int {
     const int j = jbeg + jl;
@@ -354,18 +362,17 @@ __global__ void __launch_bounds__((ColTile<T, L>::NT)) __attribute__((amdgpu_wav
     if (j + 1 < njobs) load(j + 1);
     if (wave_live) {
       if constexpr (last_twiddles_hoisted<L>()) {
-        using P3 = Plan3<typename LinePlan<L>::R>;
-        constexpr int R0 = P3::r0, R1 = P3::r1, R2 = P3::r2, NS3 = R0 * R1;
+        constexpr int R0 = P3::r0, NS3 = R0 * R1;
         rfft::stockham_stage<T, L, R0, 1, false, true>(v, line, tw, t);
         rfft::stockham_stage_given<T, L, R1, R0, false>(v, line, w2, t);
-        rfft::stockham_last_given<T, L, R2, NS3, false>(v, w3);
+        rfft::stockham_stage_given<T, L, R2, NS3, false>(v, line, w3, t);
         static_for<0, E>([&](auto e_) {
           constexpr int e = decltype(e_)::value;
           v[e] = Cplx<T>{v[e].re * gv[e], v[e].im * gv[e]};
         });
         rfft::stockham_stage<T, L, R0, 1, true, false>(v, line, tw, t);
         rfft::stockham_stage_given<T, L, R1, R0, true>(v, line, w2, t);
-        rfft::stockham_last_given<T, L, R2, NS3, true>(v, w3);
+        rfft::stockham_stage_given<T, L, R2, NS3, true>(v, line, w3, t);
       } else {
         team_fft<T, L, false, true>(v, line, tw, t);
         static_for<0, E>([&](auto e_) {
