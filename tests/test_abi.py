@@ -9,14 +9,31 @@ import pytest
 
 ROOT = Path(__file__).resolve().parent.parent
 LIB = ROOT / "sopht_amd" / "libsopht_amd.so"
+H5_LIB = ROOT / "sopht_amd" / "libsopht_amd_h5.so"
+# each header declares the entry points of one shared library
+LIBRARY_OF = {"sopht_amd.h": LIB, "sopht_amd_h5.h": H5_LIB}
 
 
-def declared_symbols():
-    names = set()
-    for h in (ROOT / "include").glob("*.h"):
-        text = h.read_text()
-        names |= set(re.findall(r"^(?:const\s+char\s*\*|int64_t|int)\s+(sopht_\w+)\s*\(", text, re.M))
-    return names
+def declared_symbols(header="sopht_amd.h"):
+    text = (ROOT / "include" / header).read_text()
+    return set(re.findall(r"^(?:const\s+char\s*\*|int64_t|int)\s+(sopht_\w+)\s*\(", text, re.M))
+
+
+def test_every_header_has_a_library():
+    assert {h.name for h in (ROOT / "include").glob("*.h")} == set(LIBRARY_OF)
+
+
+@pytest.mark.skipif(not H5_LIB.exists(), reason="library not built (python -m sopht_amd.build)")
+def test_checkpoint_library_exports_and_bindings():
+    from sopht_amd.utils import io
+
+    names = declared_symbols("sopht_amd_h5.h")
+    assert {"sopht_h5_file_create", "sopht_h5_dataset_write", "sopht_h5_list"} <= names
+    lib = ctypes.CDLL(str(H5_LIB))
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+    assert names == set(io.SIGNATURES)
+    io._h5()  # sets argtypes for every symbol; raises if one is missing
 
 
 def test_header_declares_entry_points():
