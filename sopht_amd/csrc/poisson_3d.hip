@@ -38,6 +38,17 @@ struct sopht_poisson3d {
 namespace sopht {
 namespace {
 
+// A streaming store of one complex value: the next pass reads it back from HBM, so it need
+// not stay in the cache hierarchy.
+template <typename T>
+__device__ __forceinline__ void store_stream(Cplx<T>* p, Cplx<T> v) {
+  typedef T pair __attribute__((ext_vector_type(2)));
+  pair w;
+  w.x = v.re;
+  w.y = v.im;
+  __builtin_nontemporal_store(w, reinterpret_cast<pair*>(p));
+}
+
 using rfft::LinePlan;
 using rfft::region_stride;
 using rfft::static_for;
@@ -191,8 +202,8 @@ __global__ void __launch_bounds__((RowTile<T, L>::NT)) k1_xr2c(SlabGeom g, CompP
   for (int k = t; k <= g.nx; k += TT) {
     const Cplx<T> zk = line[k];
     const Cplx<T> zm = line[(L - k) % L];
-    if (ha) ha[k] = Cplx<T>{T(0.5) * (zk.re + zm.re), T(0.5) * (zk.im - zm.im)};
-    if (hb) hb[k] = Cplx<T>{T(0.5) * (zk.im + zm.im), T(0.5) * (zm.re - zk.re)};
+    if (ha) store_stream(ha + k, Cplx<T>{T(0.5) * (zk.re + zm.re), T(0.5) * (zk.im - zm.im)});
+    if (hb) store_stream(hb + k, Cplx<T>{T(0.5) * (zk.im + zm.im), T(0.5) * (zm.re - zk.re)});
   }
 }
 
@@ -246,7 +257,10 @@ __global__ void __launch_bounds__((ColTile<T, L>::NT)) k_ycols(SlabGeom g, Cplx<
     static_for<0, KOUT>([&](auto k_) {
       constexpr int k = decltype(k_)::value;
       const int y = lr + TT * k;
-      H[col + y * ys] = xb[lw * S + y];
+      if constexpr (INV)
+        H[col + y * ys] = xb[lw * S + y];
+      else
+        store_stream(H + col + y * ys, xb[lw * S + y]);
     });
 }
 
@@ -392,7 +406,7 @@ __global__ void __launch_bounds__((ColTile<T, L>::NT)) __attribute__((amdgpu_wav
       static_for<0, K>([&](auto k_) {
         constexpr int k = decltype(k_)::value;
         const int z = lr + TT * k;
-        H[col + z * zs] = xb[lw * S + z];
+        store_stream(H + col + z * zs, xb[lw * S + z]);
       });
     }
     __syncthreads();
