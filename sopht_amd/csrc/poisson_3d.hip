@@ -469,8 +469,8 @@ __global__ void __launch_bounds__((RowTile<T, L>::NT)) k5_xc2r(SlabGeom g, const
       if (pa) pa[x * out.sx] += v[e].re;
       if (pb) pb[x * out.sx] += v[e].im;
     } else {
-      if (pa) pa[x * out.sx] = v[e].re;
-      if (pb) pb[x * out.sx] = v[e].im;
+      if (pa) __builtin_nontemporal_store(v[e].re, pa + x * out.sx);
+      if (pb) __builtin_nontemporal_store(v[e].im, pb + x * out.sx);
     }
   });
 }

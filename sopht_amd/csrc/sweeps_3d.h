@@ -178,9 +178,9 @@ __device__ __forceinline__ void s1_body(Grid g, const T* w, const T* u, const T*
         o2 = o2 + pB * c2;
       }
       const int j = r * sz + y * sy + x;
-      w2out[j] = o0;
-      w2out[n + j] = o1;
-      w2out[2 * n + j] = o2;
+      __builtin_nontemporal_store(o0, w2out + j);
+      __builtin_nontemporal_store(o1, w2out + n + j);
+      __builtin_nontemporal_store(o2, w2out + 2 * n + j);
       if (w1out) {  // omega1 as well (trajectories recorded for the adjoint)
         w1out[j] = a1prev[0];
         w1out[n + j] = a1prev[1];
@@ -243,6 +243,19 @@ __device__ __forceinline__ VecT<T, VEC> vload(const T* __restrict__ p, int off) 
 template <typename T, int VEC>
 __device__ __forceinline__ void vstore(T* __restrict__ p, int off, const VecT<T, VEC>& v) {
   *reinterpret_cast<VecT<T, VEC>*>(p + off) = v;
+}
+// A streaming (non-temporal) store: for outputs the next kernel reads back from HBM.
+template <typename T, int VEC>
+__device__ __forceinline__ void vstore_stream(T* __restrict__ p, int off, const VecT<T, VEC>& v) {
+  if constexpr (VEC == 1) {
+    __builtin_nontemporal_store(v.v[0], p + off);
+  } else {
+    typedef T vec __attribute__((ext_vector_type(VEC)));
+    vec w;
+#pragma unroll
+    for (int k = 0; k < VEC; ++k) w[k] = v.v[k];
+    __builtin_nontemporal_store(w, reinterpret_cast<vec*>(p + off));
+  }
 }
 template <typename T, int VEC>
 __device__ __forceinline__ VecT<T, VEC> vzero() {
@@ -421,7 +434,7 @@ __device__ __forceinline__ void sc_body(Grid g, const T* __restrict__ w2, T* __r
           if (pen_cell) v = ((v * fx[k]) * fy) * fz;
           r.v[k] = v;
         }
-        vstore<T, VEC>(w + c * n, o, r);
+        vstore_stream<T, VEC>(w + c * n, o, r);
         if (zero_forcing) vstore<T, VEC>(F + c * n, o, vzero<T, VEC>());
       }
     }
@@ -505,7 +518,7 @@ __device__ __forceinline__ void s2_body(Grid g, const T* __restrict__ psi, T* __
       vmax = sabs > vmax ? sabs : vmax;
     }
     if (act)
-      for (int c = 0; c < 3; ++c) vstore<T, VEC>(u + c * n, i, o[c]);
+      for (int c = 0; c < 3; ++c) vstore_stream<T, VEC>(u + c * n, i, o[c]);
     p0m = p0c;
     p1m = p1c;
     p0c = p0p;
