@@ -117,8 +117,10 @@ class _File:
     def __enter__(self):
         return self
 
-    def __exit__(self, *exc):
-        _check(_h5().sopht_h5_file_close(self.id))
+    def __exit__(self, exc_type, exc, tb):
+        rc = _h5().sopht_h5_file_close(self.id)
+        if exc_type is None:  # a close failure must not hide the error that is propagating
+            _check(rc)
         return False
 
     def create_group(self, path: str) -> None:
