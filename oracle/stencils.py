@@ -200,10 +200,10 @@ def _boundary_slabs(w):
 def penalise_field_boundary(field, width, dx, x_grid_field, y_grid_field, z_grid_field):
     """``penalise_field_boundary_3d.py:13-240`` on a scalar field, in place.
 
-    Sequence of ``:384-403``: along x then y then z, copy the plane at index ``w-1``
+    Sequence of ``:191-208``: along x then y then z, copy the plane at index ``w-1``
     (resp. ``-w``) into the ``w``-wide border slab, then multiply the slab by
     ``sin(pi/(2 w dx) * distance from the first/last cell centre)``. ``width == 0`` is a
-    no-op (``:224-236``).
+    no-op (``:29-50``).
 
     The back-slab source is the plane ``n - w`` written with explicit indices: the
     reference's ``-w:(-w+1)`` slice is empty for ``w == 1`` and raises there; for every
@@ -233,7 +233,7 @@ def penalise_field_boundary(field, width, dx, x_grid_field, y_grid_field, z_grid
 
 
 def penalise_vector_field_boundary(vector_field, width, dx, x_grid, y_grid, z_grid):
-    """Vector form (``penalise_field_boundary_3d.py:413-430``): each component in turn."""
+    """Vector form (``penalise_field_boundary_3d.py:210-237``): each component in turn."""
     for i in range(3):
         penalise_field_boundary(vector_field[i], width, dx, x_grid, y_grid, z_grid)
 

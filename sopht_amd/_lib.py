@@ -215,3 +215,16 @@ def require_device_tensor(t, real_t, name, ndim=None):
     if ndim is not None and t.dim() != ndim:
         msg = f"{name} must have {ndim} dimensions, got shape {tuple(t.shape)}"
         raise ValueError(msg)
+
+
+def written(*tensors) -> None:
+    """Bump torch's version counter of tensors a library call wrote through raw pointers.
+
+    Writes through ``data_ptr()`` are invisible to torch; callers that cache results keyed
+    on a tensor's version (the simulator's fused velocity maxima) rely on this to see them.
+    Views share their base's counter, so marking a component marks the whole field."""
+    import torch
+
+    live = [t for t in tensors if t is not None]
+    if live:
+        torch.autograd.graph.increment_version(live)

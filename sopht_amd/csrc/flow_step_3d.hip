@@ -190,7 +190,7 @@ int sopht_ns3d_create_with_poisson_solver(int dtype, int64_t nz, int64_t ny, int
   P->with_free_stream = with_free_stream;
   // sine factors sin(pi/(2 w dx) * distance from the first / last cell centre); cell
   // centres are (i + 1/2) dx, so the distance of slab cell i is i dx
-  // (penalise_field_boundary_3d.py:257-264,282-284,294-296)
+  // (penalise_field_boundary_3d.py:69,85-102)
   for (int a = 0; a < 3; ++a)
     for (int i = 0; i < width; ++i) {
       const double pre = (M_PI / 2.0) / (double(width) * P->dx);

@@ -67,23 +67,35 @@ static int code_of(hid_t type) {
   return -1;
 }
 
-/* Whether every prefix of an absolute path names a link (H5Lexists needs the parents). */
+/* Whether every prefix of an absolute path names a link (H5Lexists needs the parents):
+   1 yes, 0 no, -1 out of host memory. Paths of any length (the copy is on the heap). */
 static int path_exists(hid_t file, const char* path) {
-  char buf[1024];
   size_t n = strlen(path);
-  if (n == 0 || n >= sizeof buf) return 0;
+  if (n == 0) return 0;
   if (strcmp(path, "/") == 0) return 1;
+  char* buf = (char*)malloc(n + 1);
+  if (!buf) return -1;
   memcpy(buf, path, n + 1);
-  for (size_t i = 1; i <= n; ++i) {
+  int found = 1;
+  for (size_t i = 1; i <= n && found; ++i) {
     if (buf[i] == '/' || buf[i] == '\0') {
       char c = buf[i];
       buf[i] = '\0';
       htri_t e = H5Lexists(file, buf, H5P_DEFAULT);
       buf[i] = c;
-      if (e <= 0) return 0;
+      if (e <= 0) found = 0;
     }
   }
-  return 1;
+  free(buf);
+  return found;
+}
+
+/* SOPHT_H5_OK when the object at path exists, else the recorded error. */
+static int require_path(hid_t file, const char* path) {
+  int e = path_exists(file, path);
+  if (e < 0) return fail(SOPHT_H5_ERR_HDF5, "out of host memory looking up '%s'", path);
+  if (e == 0) return fail(SOPHT_H5_ERR_NOT_FOUND, "no object '%s'", path);
+  return SOPHT_H5_OK;
 }
 
 static hid_t make_space(int rank, const int64_t* dims) {
@@ -175,7 +187,8 @@ int sopht_h5_dataset_write(int64_t file, const char* path, int dtype, int rank, 
 int sopht_h5_dataset_shape(int64_t file, const char* path, int* rank, int64_t* dims, int* dtype) {
   quiet();
   if (!path || !rank || !dims || !dtype) return fail(SOPHT_H5_ERR_INVALID, "null argument");
-  if (!path_exists((hid_t)file, path)) return fail(SOPHT_H5_ERR_NOT_FOUND, "no object '%s'", path);
+  int found = require_path((hid_t)file, path);
+  if (found != SOPHT_H5_OK) return found;
   hid_t d = H5Dopen2((hid_t)file, path, H5P_DEFAULT);
   if (d < 0) return fail(SOPHT_H5_ERR_HDF5, "'%s' is not a dataset", path);
   hid_t space = H5Dget_space(d), type = H5Dget_type(d);
@@ -190,7 +203,8 @@ int sopht_h5_dataset_read(int64_t file, const char* path, int dtype, int64_t cou
   quiet();
   if (!path || (count > 0 && !out)) return fail(SOPHT_H5_ERR_INVALID, "null argument");
   if (mem_type(dtype) < 0) return fail(SOPHT_H5_ERR_INVALID, "unsupported dtype code %d", dtype);
-  if (!path_exists((hid_t)file, path)) return fail(SOPHT_H5_ERR_NOT_FOUND, "no object '%s'", path);
+  int found = require_path((hid_t)file, path);
+  if (found != SOPHT_H5_OK) return found;
   hid_t d = H5Dopen2((hid_t)file, path, H5P_DEFAULT);
   if (d < 0) return fail(SOPHT_H5_ERR_HDF5, "'%s' is not a dataset", path);
   hid_t space = H5Dget_space(d);
@@ -213,7 +227,8 @@ int sopht_h5_attr_write(int64_t file, const char* obj, const char* name, int dty
   if (!obj || !name || !data) return fail(SOPHT_H5_ERR_INVALID, "null argument");
   if (mem_type(dtype) < 0) return fail(SOPHT_H5_ERR_INVALID, "unsupported dtype code %d", dtype);
   if (!check_shape_args(rank, dims)) return fail(SOPHT_H5_ERR_INVALID, "bad attribute shape");
-  if (!path_exists((hid_t)file, obj)) return fail(SOPHT_H5_ERR_NOT_FOUND, "no object '%s'", obj);
+  int found = require_path((hid_t)file, obj);
+  if (found != SOPHT_H5_OK) return found;
   hid_t f = (hid_t)file;
   if (H5Aexists_by_name(f, obj, name, H5P_DEFAULT) > 0 && H5Adelete_by_name(f, obj, name, H5P_DEFAULT) < 0)
     return fail(SOPHT_H5_ERR_HDF5, "unable to replace attribute '%s' of '%s'", name, obj);
@@ -230,7 +245,8 @@ int sopht_h5_attr_write(int64_t file, const char* obj, const char* name, int dty
 
 /* Opens obj.attrs[name]; on failure returns -1 with *code set and the error recorded. */
 static hid_t open_attr(hid_t f, const char* obj, const char* name, int* code) {
-  if (!path_exists(f, obj)) return *code = fail(SOPHT_H5_ERR_NOT_FOUND, "no object '%s'", obj), -1;
+  int found = require_path(f, obj);
+  if (found != SOPHT_H5_OK) return *code = found, -1;
   if (H5Aexists_by_name(f, obj, name, H5P_DEFAULT) <= 0)
     return *code = fail(SOPHT_H5_ERR_NOT_FOUND, "no attribute '%s' on '%s'", name, obj), -1;
   hid_t a = H5Aopen_by_name(f, obj, name, H5P_DEFAULT, H5P_DEFAULT);

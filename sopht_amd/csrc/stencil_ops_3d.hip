@@ -197,7 +197,7 @@ int divergence_impl(void* div, const int64_t* ds, const void* const in[3], const
 
 // Penalisation, race-free in place: every border cell t takes the ORIGINAL value of its
 // clamped source c(t) (clamp of each index to [w-1, n-w]) times the three slab factors
-// (penalise_field_boundary_3d.py:384-403 collapsed to one pass). Targets are grouped by
+// (penalise_field_boundary_3d.py:191-208 collapsed to one pass). Targets are grouped by
 // source: one thread per source cell on the surface of the core box [w-1, n-w]^3 reads
 // its value once and writes its whole group (itself last), so no thread reads a cell
 // another thread writes.

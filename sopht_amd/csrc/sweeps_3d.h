@@ -331,7 +331,7 @@ __device__ __forceinline__ void x_nbrs(const VecT<T, VEC>& v, const T* __restric
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 // SC: diffusion of omega2 at the clamped index, penalised with the per-axis sine factors
-// (penalise_field_boundary_3d.py:384-403, x then y then z), into omega; F = 0. A thread's
+// (penalise_field_boundary_3d.py:191-208, x then y then z), into omega; F = 0. A thread's
 // row is the clamped row cy; the x clamp stays inside the thread's vector (VEC >= width).
 template <typename T, int VEC>
 __device__ __forceinline__ void sc_body(Grid g, const T* __restrict__ w2, T* __restrict__ w,

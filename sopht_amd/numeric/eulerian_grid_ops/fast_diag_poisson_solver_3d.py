@@ -113,6 +113,8 @@ class FastDiagPoissonSolver3D:
         )
         if sol is not solution_field:
             solution_field.copy_(sol)
+        else:
+            L.written(solution_field)
 
     def vector_field_solve(self, solution_vector_field, rhs_vector_field) -> None:
         """``vector_field_solve`` (``:191-202``): the three components in one batched sequence
@@ -134,3 +136,5 @@ class FastDiagPoissonSolver3D:
         )
         if sol is not solution_vector_field:
             solution_vector_field.copy_(sol)
+        else:
+            L.written(solution_vector_field)

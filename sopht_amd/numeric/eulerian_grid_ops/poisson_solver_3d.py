@@ -73,6 +73,7 @@ class UnboundedPoissonSolverPYFFTW3D:
             "sopht_poisson3d_solve", self._plan, L.ptr(solution_field), L.strides3(solution_field),
             L.ptr(rhs_field), L.strides3(rhs_field), L.stream_ptr(),
         )
+        L.written(solution_field)
 
     def vector_field_solve(self, solution_vector_field, rhs_vector_field) -> None:
         """``vector_field_solve`` (``:151-172``): the three components in one pass sequence."""
@@ -83,3 +84,4 @@ class UnboundedPoissonSolverPYFFTW3D:
             L.strides3(solution_vector_field[0]), L.comps(rhs_vector_field),
             L.strides3(rhs_vector_field[0]), L.stream_ptr(),
         )
+        L.written(solution_vector_field)

@@ -46,6 +46,7 @@ def gen_curl_pyst_kernel_3d(
             L.strides3(field[0]), nz, ny, nx, float(prefactor), int(bool(reset_ghost_zone)),
             L.stream_ptr(),
         )
+        L.written(curl)
 
     return curl_pyst_kernel_3d
 
@@ -68,6 +69,7 @@ def gen_update_vorticity_from_velocity_forcing_pyst_kernel_3d(
             L.strides3(vorticity_field[0]), L.comps(velocity_forcing_field),
             L.strides3(velocity_forcing_field[0]), nz, ny, nx, float(prefactor), L.stream_ptr(),
         )
+        L.written(vorticity_field)
 
     return update_vorticity_from_velocity_forcing_pyst_kernel_3d
 
@@ -91,6 +93,7 @@ def gen_update_vorticity_from_penalised_velocity_pyst_kernel_3d(
             L.strides3(penalised_velocity_field[0]), L.comps(velocity_field),
             L.strides3(velocity_field[0]), nz, ny, nx, float(prefactor), L.stream_ptr(),
         )
+        L.written(vorticity_field)
 
     return update_vorticity_from_penalised_velocity_pyst_kernel_3d
 
@@ -111,6 +114,7 @@ def gen_divergence_pyst_kernel_3d(
             L.strides3(field[0]), nz, ny, nx, float(inv_dx), int(bool(reset_ghost_zone)),
             L.stream_ptr(),
         )
+        L.written(divergence)
 
     return divergence_pyst_kernel_3d
 
@@ -136,6 +140,7 @@ def gen_diffusion_flux_pyst_kernel_3d(
             "sopht_diffusion_flux_3d", tag, L.ptr(diffusion_flux), L.strides3(diffusion_flux),
             L.ptr(field), L.strides3(field), nz, ny, nx, float(prefactor), reset, L.stream_ptr(),
         )
+        L.written(diffusion_flux)
 
     if field_type == "scalar":
         return diffusion_flux_pyst_kernel_3d
@@ -167,6 +172,7 @@ def gen_diffusion_timestep_euler_forward_pyst_kernel_3d(
             L.ptr(diffusion_flux), L.strides3(diffusion_flux), nz, ny, nx, float(nu_dt_by_dx2),
             L.stream_ptr(),
         )
+        L.written(field, diffusion_flux)
 
     if field_type == "scalar":
         return diffusion_timestep_euler_forward_pyst_kernel_3d
@@ -184,7 +190,7 @@ def gen_diffusion_timestep_euler_forward_pyst_kernel_3d(
 
 
 def penalisation_sine_tables(width, dx, x_grid_field, y_grid_field, z_grid_field) -> np.ndarray:
-    """Slab factors of ``penalise_field_boundary_3d.py:257-264,282-372``, in FP64 from the
+    """Slab factors of ``penalise_field_boundary_3d.py:69-180``, in FP64 from the
     given (host or device) grid fields: per axis x, y, z, ``width`` front factors
     ``sin(pi/(2 w dx) (c_i - c_0))`` then ``width`` back factors ``sin(pi/(2 w dx) (c_end - c_i))``."""
 
@@ -241,10 +247,17 @@ def gen_penalise_field_boundary_pyst_kernel_3d(
 
     def penalise_field_boundary_pyst_kernel_3d(field) -> None:
         nz, ny, nx = _shape3(field)
+        if width == 1:
+            # the reference's back-slab source ``field[..., -1:0]`` is empty for width 1, and
+            # its broadcast into the one-plane slab raises (penalise_field_boundary_3d.py:192)
+            msg = (f"could not broadcast input array from shape ({nz},{ny},0) "
+                   f"into shape ({nz},{ny},1)")
+            raise ValueError(msg)
         L.call(
             "sopht_penalise_field_boundary_3d", tag, L.ptr(field), L.strides3(field), nz, ny, nx,
             width, ctab, L.stream_ptr(),
         )
+        L.written(field)
 
     if field_type == "scalar":
         return penalise_field_boundary_pyst_kernel_3d
@@ -275,6 +288,7 @@ def gen_elementwise_sum_pyst_kernel_3d(
             "sopht_elementwise_sum_3d", tag, L.ptr(sum_field), L.strides3(sum_field), L.ptr(field_1),
             L.strides3(field_1), L.ptr(field_2), L.strides3(field_2), nz, ny, nx, L.stream_ptr(),
         )
+        L.written(sum_field)
 
     def elementwise_sum_pyst_kernel_3d(sum_field, field_1, field_2) -> None:
         if field_type == "vector":
@@ -300,6 +314,7 @@ def gen_set_fixed_val_pyst_kernel_3d(
         nz, ny, nx = _shape3(field)
         L.call("sopht_set_fixed_val_3d", tag, L.ptr(field), L.strides3(field), nz, ny, nx,
                float(fixed_val), L.stream_ptr())
+        L.written(field)
 
     if field_type == "scalar":
         return set_fixed_val_pyst_kernel_3d
@@ -323,6 +338,7 @@ def gen_elementwise_copy_pyst_kernel_3d(
         nz, ny, nx = _shape3(field)
         L.call("sopht_elementwise_copy_3d", tag, L.ptr(field), L.strides3(field), L.ptr(rhs_field),
                L.strides3(rhs_field), nz, ny, nx, L.stream_ptr())
+        L.written(field)
 
     return elementwise_copy_pyst_kernel_3d
 
@@ -340,6 +356,7 @@ def gen_elementwise_complex_product_pyst_kernel_3d(
         L.call("sopht_elementwise_complex_product_3d", tag, L.ptr(product_field),
                L.strides3(product_field), L.ptr(field_1), L.strides3(field_1), L.ptr(field_2),
                L.strides3(field_2), nz, ny, nx, L.stream_ptr())
+        L.written(product_field)
 
     return elementwise_complex_product_pyst_kernel_3d
 
@@ -361,6 +378,7 @@ def gen_set_fixed_val_at_boundaries_pyst_kernel_3d(
         nz, ny, nx = _shape3(field)
         L.call("sopht_set_fixed_val_at_boundaries_3d", tag, L.ptr(field), L.strides3(field), nz, ny,
                nx, width, float(fixed_val), L.stream_ptr())
+        L.written(field)
 
     if field_type == "scalar":
         return set_fixed_val_at_boundaries_pyst_kernel_3d
@@ -386,6 +404,7 @@ def gen_add_fixed_val_pyst_kernel_3d(
         nz, ny, nx = _shape3(field)
         L.call("sopht_add_fixed_val_3d", tag, L.ptr(sum_field), L.strides3(sum_field), L.ptr(field),
                L.strides3(field), nz, ny, nx, float(fixed_val), L.stream_ptr())
+        L.written(sum_field)
 
     if field_type == "scalar":
         return add_fixed_val_pyst_kernel_3d
@@ -411,6 +430,7 @@ def gen_elementwise_saxpby_pyst_kernel_3d(
         nz, ny, nx = _shape3(s)
         L.call("sopht_elementwise_saxpby_3d", tag, L.ptr(s), L.strides3(s), L.ptr(f1), L.strides3(f1),
                L.ptr(f2), L.strides3(f2), nz, ny, nx, float(a), float(b), L.stream_ptr())
+        L.written(s)
 
     def elementwise_saxpby_pyst_kernel_3d(sum_field, field_1, field_2, field_1_prefac, field_2_prefac) -> None:
         if field_type == "vector":
@@ -435,6 +455,7 @@ def gen_elementwise_cross_product_pyst_kernel_3d(
         L.call("sopht_elementwise_cross_product_3d", tag, L.comps(result_field),
                L.strides3(result_field[0]), L.comps(field_1), L.strides3(field_1[0]),
                L.comps(field_2), L.strides3(field_2[0]), nz, ny, nx, L.stream_ptr())
+        L.written(result_field)
 
     return elementwise_cross_product_pyst_kernel_3d
 
@@ -455,6 +476,7 @@ def gen_advection_flux_conservative_eno3_pyst_kernel_3d(
         L.call("sopht_advection_flux_conservative_eno3_3d", tag, L.ptr(advection_flux),
                L.strides3(advection_flux), L.ptr(field), L.strides3(field), L.comps(velocity),
                L.strides3(velocity[0]), nz, ny, nx, float(inv_dx), L.stream_ptr())
+        L.written(advection_flux)
 
     return advection_flux_conservative_eno3_pyst_kernel_3d
 
@@ -509,6 +531,7 @@ def gen_vorticity_stretching_flux_pyst_kernel_3d(
                L.strides3(vorticity_stretching_flux_field[0]), L.comps(vorticity_field),
                L.strides3(vorticity_field[0]), L.comps(velocity_field), L.strides3(velocity_field[0]),
                nz, ny, nx, float(prefactor), L.stream_ptr())
+        L.written(vorticity_stretching_flux_field)
 
     return vorticity_stretching_flux_pyst_kernel_3d
 
@@ -599,6 +622,7 @@ def gen_laplacian_filter_kernel_3d(
             L.strides3(scalar_field), nz, ny, nx, L.ptr(filter_flux_buffer), L.strides3(filter_flux_buffer),
             L.ptr(field_buffer), L.strides3(field_buffer), width, L.stream_ptr(),
         )
+        L.written(scalar_field, filter_flux_buffer, field_buffer)
 
     if field_type == "scalar":
         return scalar_field_filter_kernel_3d

@@ -83,6 +83,7 @@ class EulerianLagrangianGridCommunicator3D:
             L.ptr(nearest_eul_grid_index_to_lag_grid), L.ptr(local_eul_grid_support_of_lag_grid),
             n, float(self.dx), float(self.eul_grid_coord_shift), L.stream_ptr(),
         )
+        L.written(nearest_eul_grid_index_to_lag_grid, local_eul_grid_support_of_lag_grid)
 
     # :401-410 (cosine) / :429-516 (peskin)
     def _weights_kernel(self, interp_weights, local_eul_grid_support_of_lag_grid):
@@ -91,6 +92,7 @@ class EulerianLagrangianGridCommunicator3D:
             "sopht_lag_interp_weights_3d", self._tag, self._ktype, L.ptr(interp_weights),
             L.ptr(local_eul_grid_support_of_lag_grid), n, float(self.dx), L.stream_ptr(),
         )
+        L.written(interp_weights, local_eul_grid_support_of_lag_grid)
 
     def _eul_ptrs(self, eul_grid_field):
         if eul_grid_field.dim() == 4:
@@ -107,6 +109,7 @@ class EulerianLagrangianGridCommunicator3D:
             int(shp[0]), int(shp[1]), int(shp[2]), L.ptr(interp_weights),
             L.ptr(nearest_eul_grid_index_to_lag_grid), n, float(self.dx), L.stream_ptr(),
         )
+        L.written(lag_grid_field)
 
     # :318-380
     def _spread_kernel(self, eul_grid_field, lag_grid_field, interp_weights, nearest_eul_grid_index_to_lag_grid):
@@ -117,6 +120,7 @@ class EulerianLagrangianGridCommunicator3D:
             int(shp[1]), int(shp[2]), L.ptr(lag_grid_field.contiguous()), L.ptr(interp_weights),
             L.ptr(nearest_eul_grid_index_to_lag_grid), n, L.stream_ptr(),
         )
+        L.written(eul_grid_field)
 
     def check_support_in_domain(self):
         """Synchronise and raise ``IndexError`` if a spread support block left the grid."""

@@ -158,6 +158,7 @@ class VirtualBoundaryForcing:
             L.ptr(t), L.ptr(lag_grid_velocity_mismatch_field), int(t.shape[-1]), float(dt),
             L.stream_ptr(),
         )
+        L.written(t)
 
     # :166-185
     @staticmethod
@@ -190,6 +191,12 @@ class VirtualBoundaryForcing:
             L.ptr(self.interp_weights), L.ptr(self.lag_grid_flow_velocity_field),
             L.ptr(self.lag_grid_velocity_mismatch_field), L.ptr(self.lag_grid_position_mismatch_field),
             L.ptr(self.lag_grid_forcing_field), L.stream_ptr(),
+        )
+        L.written(
+            self.nearest_eul_grid_index_to_lag_grid, self.local_eul_grid_support_of_lag_grid,
+            self.interp_weights, self.lag_grid_flow_velocity_field,
+            self.lag_grid_velocity_mismatch_field, self.lag_grid_position_mismatch_field,
+            self.lag_grid_forcing_field,
         )
 
     # :232-253

@@ -106,6 +106,14 @@ class BatchedFlowSimulation:
     def step(self, steps: int = 1, use_graph: bool = True) -> None:
         """Advance every member by ``steps`` steps (asynchronous on the current stream)."""
         L.call("sopht_ns3d_batch_step", self._batch, int(steps), int(bool(use_graph)), L.stream_ptr())
+        # the members' fields were written through raw pointers: their version counters
+        # must move, or a member's cached velocity maxima would still look current
+        for flow, inter in zip(self._flows, self._inters):
+            L.written(flow.vorticity_field, flow.velocity_field, flow.stream_func_field,
+                      flow.eul_grid_forcing_field if flow.with_forcing else None)
+            if inter is not None:
+                L.written(inter.lag_grid_flow_velocity_field, inter.lag_grid_velocity_mismatch_field,
+                          inter.lag_grid_position_mismatch_field, inter.lag_grid_forcing_field)
 
     def synchronize(self):
         """Wait for the device; set each member's ``time`` (and its interactor's) and return

@@ -180,12 +180,22 @@ class UnboundedNavierStokesFlowSimulator3D(FlowSimulator):
         fs = np.asarray(free_stream_velocity, dtype=np.float64).reshape(3)
         return (C.c_double * 3)(*fs)
 
+    def _check_penalty_width(self):
+        if self.penalty_zone_width == 1:
+            # as the reference's penalisation kernel, whose back-slab source is an empty
+            # slice for width 1 (penalise_field_boundary_3d.py:192)
+            nz, ny, nx = self.grid_size
+            msg = f"could not broadcast input array from shape ({nz},{ny},0) into shape ({nz},{ny},1)"
+            raise ValueError(msg)
+
     def _step(self, dt, free_stream_velocity, forcing):
+        self._check_penalty_width()
         L.call(
             "sopht_ns3d_step", self._plan, L.ptr(self.vorticity_field), L.ptr(self.velocity_field),
             L.ptr(self.stream_func_field), L.ptr(forcing), L.ptr(self.buffer_vector_field),
             float(dt), self._fs(free_stream_velocity), L.stream_ptr(),
         )
+        L.written(self.vorticity_field, self.velocity_field, self.stream_func_field, forcing)
         self._velocity_version = self.velocity_field._version
 
     # :449-485
@@ -208,14 +218,17 @@ class UnboundedNavierStokesFlowSimulator3D(FlowSimulator):
             L.ptr(forcing), L.ptr(vorticity_out), L.ptr(velocity_out), L.ptr(self.stream_func_field),
             L.ptr(vorticity1_out), float(dt), self._fs(free_stream_velocity), L.stream_ptr(),
         )
+        L.written(vorticity_out, velocity_out, self.stream_func_field, vorticity1_out)
 
     def recover_velocity(self, free_stream_velocity=_zeros_3):
         """Velocity recovery alone (``:475-485``): penalise vorticity, Poisson, curl, free stream."""
+        self._check_penalty_width()
         L.call(
             "sopht_ns3d_recover_velocity", self._plan, L.ptr(self.vorticity_field),
             L.ptr(self.velocity_field), L.ptr(self.stream_func_field), self._fs(free_stream_velocity),
             L.stream_ptr(),
         )
+        L.written(self.vorticity_field, self.velocity_field, self.stream_func_field)
         self._velocity_version = self.velocity_field._version
 
     # :501-512
@@ -225,7 +238,10 @@ class UnboundedNavierStokesFlowSimulator3D(FlowSimulator):
     def max_abs_velocity_sum(self):
         """``max over cells of sum_i |u_i|`` (one readback). When the velocity field is the one
         the last step wrote (torch's version counter of the tensor is unchanged), the maxima
-        its velocity sweep kept per workgroup are reduced instead of re-reading the field."""
+        its velocity sweep kept per workgroup are reduced instead of re-reading the field.
+        Every entry point of this package that writes a tensor through raw pointers (the
+        drop-in kernels, the solvers, the batched step) moves that counter; code that writes
+        the velocity field through its own raw pointers calls ``sopht_amd._lib.written`` on it."""
         lib = self._lib
         rc = -1
         if getattr(self, "_velocity_version", None) == self.velocity_field._version:

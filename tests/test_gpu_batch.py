@@ -78,6 +78,9 @@ def test_batch_matches_members_stepped_alone(dev, precision, use_graph):
             for name in ("lag_grid_position_mismatch_field", "lag_grid_velocity_mismatch_field",
                          "lag_grid_forcing_field", "lag_grid_flow_velocity_field"):
                 assert np.array_equal(to_host(getattr(ia, name)), to_host(getattr(ib, name))), (k, name)
+        # the batch wrote the member's velocity through raw pointers: dt must come from the
+        # velocity it left, not from maxima cached before the batch
+        assert fb.compute_stable_timestep() == fa.compute_stable_timestep(), k
     sim.close()
 
 

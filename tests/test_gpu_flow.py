@@ -96,6 +96,28 @@ def test_stable_timestep(dev, precision):
     assert g.compute_stable_timestep() == o.compute_stable_timestep()
 
 
+@pytest.mark.parametrize("precision", ["single", "double"])
+def test_stable_timestep_after_drop_in_write(dev, precision):
+    """A drop-in kernel that writes the velocity field after a step (through raw pointers)
+    must not leave dt on the maxima the step's velocity sweep kept."""
+    from sopht_amd.numeric import eulerian_grid_ops as spne
+
+    real_t = PREC[precision]
+    n = 16
+    g = sps().UnboundedNavierStokesFlowSimulator3D((n, n, n), 1.0, 1e-3, real_t=real_t,
+                                                 with_free_stream_flow=True)
+    w0 = synthetic.random_phase_vorticity((n, n, n), 1.0, 4, 5, 10.0).astype(real_t)
+    g.vorticity_field.copy_(to_dev(w0, dev))
+    g.time_step(1e-3, free_stream_velocity=np.array([1.0, 0.0, 0.0]))
+    before = g.compute_stable_timestep()
+    add = spne.gen_add_fixed_val_pyst_kernel_3d(real_t=real_t, field_type="vector")
+    add(sum_field=g.velocity_field, vector_field=g.velocity_field, fixed_vals=[3.0, 0.0, 0.0])
+    vmax = real_t(g.velocity_field.abs().sum(dim=0).max().item())
+    assert g.max_abs_velocity_sum() == vmax
+    assert g.compute_stable_timestep() == g._stable_timestep_from_max(vmax)
+    assert g.compute_stable_timestep() < before
+
+
 def test_vorticity_divergence_norm(dev, rng):
     n = 16
     w = rng.random((3, n, n, n))

@@ -107,7 +107,22 @@ def test_penalise_field_boundary(dev, real_t, width, field_type, rng):
         S.penalise_field_boundary(ref, width, dx, xg, yg, zg)
     fd = to_dev(f, dev)
     k = spne().gen_penalise_field_boundary_pyst_kernel_3d(width, dx, xg, yg, zg, real_t, field_type=field_type)
-    k(fd)
+    if width == 1:
+        # the reference raises for width 1 (empty back-slab source); the drop-in does too,
+        # and the C entry point computes the intended plane, as the oracle does
+        with pytest.raises(ValueError):
+            k(fd)
+        from sopht_amd import _lib as L
+        from sopht_amd.numeric.eulerian_grid_ops.stencil_ops_3d import penalisation_sine_tables
+        import ctypes as C
+
+        tab = penalisation_sine_tables(width, dx, xg, yg, zg)
+        ctab = (C.c_double * len(tab))(*tab)
+        for comp in (fd if field_type == "vector" else [fd]):
+            L.call("sopht_penalise_field_boundary_3d", L.dtype_tag(real_t), L.ptr(comp),
+                   L.strides3(comp), n, n, n, width, ctab, L.stream_ptr())
+    else:
+        k(fd)
     np.testing.assert_allclose(to_host(fd), ref, atol=tol(real_t), rtol=0)
 
 

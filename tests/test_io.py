@@ -329,3 +329,21 @@ def test_h5_tools_and_library_agree_on_version():
     out = subprocess.run([str(H5_BIN / "h5dump"), "-V"], capture_output=True, text=True).stdout
     major, minor, rel = (int(x) for x in re.search(r"(\d+)\.(\d+)\.(\d+)", out).groups())
     assert sio._h5().sopht_h5_library_version() == major * 10000 + minor * 100 + rel
+
+
+def test_long_object_paths_round_trip(tmp_cwd):
+    """Objects whose paths exceed any fixed buffer (2 kB here) are written, found and read
+    back, and a missing one of the same length is reported as missing."""
+    name = "g" * 700
+    path = f"/{name}/{name}/field"
+    data = np.arange(12, dtype=np.float64).reshape(3, 4)
+    with sio._File("long.h5", "w") as f:
+        f.create_group(f"/{name}")
+        f.create_group(f"/{name}/{name}")
+        f.write(path, data)
+        f.set_attr(path, "time", 0.5)
+    with sio._File("long.h5", "r") as f:
+        np.testing.assert_array_equal(f.read(path), data)
+        assert float(f.attr(path, "time")) == 0.5
+        with pytest.raises(KeyError):
+            f.read(path + "x")
