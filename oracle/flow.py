@@ -151,7 +151,10 @@ class ImmersedBodyFlowInteraction(ib.VirtualBoundaryForcing):
     coefficients rescaled by ``max_lag_grid_dx**(d-1)`` (``:83-86``); forcing reset off by
     default (``:33``); ``__call__`` (``:113-122``)."""
 
-    def __init__(self, flow, forcing_grid, k, c, real_t, enable_eul_grid_forcing_reset=False):
+    def __init__(self, flow, forcing_grid, k, c, real_t, enable_eul_grid_forcing_reset=False,
+                 body_flow_forces=None, body_flow_torques=None):
+        self.body_flow_forces = np.zeros((3, 1)) if body_flow_forces is None else body_flow_forces
+        self.body_flow_torques = np.zeros((3, 1)) if body_flow_torques is None else body_flow_torques
         ds = forcing_grid.get_maximum_lagrangian_grid_spacing()
         super().__init__(
             k * ds**2,
@@ -172,4 +175,19 @@ class ImmersedBodyFlowInteraction(ib.VirtualBoundaryForcing):
             self.flow.velocity_field,
             self.forcing_grid.position_field,
             self.forcing_grid.velocity_field,
+        )
+
+    def compute_interaction_on_lag_grid(self):
+        """``:102-111``: grid kinematics, then the Lagrangian forcing alone (no spreading)."""
+        self.forcing_grid.compute_lag_grid_position_field()
+        self.forcing_grid.compute_lag_grid_velocity_field()
+        self.compute_interaction_force_on_lag_grid(
+            self.flow.velocity_field, self.forcing_grid.position_field, self.forcing_grid.velocity_field
+        )
+
+    def compute_flow_forces_and_torques(self):
+        """``:124-131``: the Lagrangian forcing, transferred to the body by its grid."""
+        self.compute_interaction_on_lag_grid()
+        self.forcing_grid.transfer_forcing_from_grid_to_body(
+            self.body_flow_forces, self.body_flow_torques, self.lag_grid_forcing_field
         )

@@ -15,6 +15,14 @@ from .flow import (
     create_unbounded_flow_simulator_3d,
 )
 from .passive_transport import PassiveTransportFlowSimulator
+from .cosserat_rod import (
+    CosseratRodElementCentricForcingGrid,
+    CosseratRodFlowInteraction,
+    CosseratRodNodalForcingGrid,
+    CosseratRodSurfaceForcingGrid,
+    FlowForces,
+    StraightRod,
+)
 from .immersed_body import (
     FibonacciEllipsoidForcingGrid,
     FibonacciSphereForcingGrid,
@@ -28,6 +36,12 @@ from .immersed_body import (
 
 __all__ = [
     "BatchedFlowSimulation",
+    "CosseratRodElementCentricForcingGrid",
+    "CosseratRodFlowInteraction",
+    "CosseratRodNodalForcingGrid",
+    "CosseratRodSurfaceForcingGrid",
+    "FlowForces",
+    "StraightRod",
     "CoupledFlowRollout",
     "HeaveKinematics",
     "IBFlowRolloutFunction",

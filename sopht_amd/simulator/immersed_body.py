@@ -53,6 +53,15 @@ class ImmersedBodyForcingGrid:
         self.num_lag_nodes = num_lag_nodes
         self.position_field = np.zeros((self.grid_dim, self.num_lag_nodes))
         self.velocity_field = np.zeros_like(self.position_field)
+        if grid_dim == 2:
+            # the reference's notice for planar grids (immersed_body_forcing_grid.py:26-33)
+            logger.warning(
+                "\n=================================================="
+                "\n2D body forcing grid generated, this assumes the body"
+                "\nmoves in XY plane! Please initialize your body such that"
+                "\nensuing dynamics are constrained in XY plane!"
+                "\n=================================================="
+            )
 
     @abstractmethod
     def compute_lag_grid_position_field(self) -> None: ...
@@ -199,13 +208,7 @@ class ImmersedBodyFlowInteraction(VirtualBoundaryForcing):
         self.eul_grid_forcing_field = eul_grid_forcing_field
         self.eul_grid_velocity_field = eul_grid_velocity_field
         max_lag_grid_dx = self.forcing_grid.get_maximum_lagrangian_grid_spacing()
-        grid_type = type(self.forcing_grid).__name__
-        if max_lag_grid_dx > 2 * dx:
-            logger.warning("%s: max Lagrangian spacing %s > 2 dx (%s): grid too coarse", grid_type, max_lag_grid_dx, dx)
-        elif max_lag_grid_dx < 0.5 * dx:
-            logger.warning("%s: max Lagrangian spacing %s < 0.5 dx (%s): grid too fine", grid_type, max_lag_grid_dx, dx)
-        else:
-            logger.info("%s: Lagrangian grid resolved like the Eulerian grid (dx = %s)", grid_type, dx)
+        self._log_resolution(max_lag_grid_dx, dx)
         # :83-86 rescale by the Lagrangian spacing
         virtual_boundary_stiffness_coeff *= max_lag_grid_dx ** (grid_dim - 1)
         virtual_boundary_damping_coeff *= max_lag_grid_dx ** (grid_dim - 1)
@@ -222,6 +225,28 @@ class ImmersedBodyFlowInteraction(VirtualBoundaryForcing):
             num_threads,
             start_time,
         )
+
+    def _log_resolution(self, max_lag_grid_dx, dx):
+        """The reference's resolution notice, same text and levels (:52-79): a warning when
+        the Lagrangian spacing is above 2 dx (the delta kernel's support) or below dx / 2."""
+        banner = "\n=================================================="
+        head = f"{banner}\nFor {type(self.forcing_grid).__name__}:\nEulerian grid spacing (dx): {dx}"
+        if max_lag_grid_dx > 2 * dx:
+            body = (f"\nMax Lagrangian grid spacing: {max_lag_grid_dx} > 2 * dx"
+                    "\nThe Lagrangian grid of the body is too coarse relative to"
+                    "\nthe Eulerian grid of the flow, which can lead to unexpected"
+                    "\nconvergence. Please make the Lagrangian grid finer.")
+            log = logger.warning
+        elif max_lag_grid_dx < 0.5 * dx:
+            body = (f"\nMax Lagrangian grid spacing: {max_lag_grid_dx} < 0.5 * dx"
+                    "\nThe Lagrangian grid of the body is too fine relative to"
+                    "\nthe Eulerian grid of the flow, which corresponds to redundant"
+                    "\nforcing points. Please make the Lagrangian grid coarser.")
+            log = logger.warning
+        else:
+            body = "\nLagrangian grid is resolved almost the same\nas the Eulerian grid of the flow."
+            log = logger.info
+        log(head + body + banner)
 
     def compute_interaction_on_lag_grid(self):
         self.forcing_grid.compute_lag_grid_position_field()

@@ -78,3 +78,45 @@ def fibonacci_sphere(num_points, radius, center) -> np.ndarray:
 def fibonacci_ellipsoid(num_points, semi_axes, center) -> np.ndarray:
     s = np.asarray(semi_axes, dtype=np.float64).reshape(3, 1)
     return np.asarray(center, dtype=np.float64).reshape(3, 1) + s * fibonacci_unit_sphere(num_points)
+
+
+class HeavingRod:
+    """Prescribed heave of a straight rod (the flexible-filament workload's base motion,
+    applied to the whole rod): every node at ``rest + (0, A sin(2 pi f t), 0)`` with velocity
+    ``(0, 2 pi f A cos(2 pi f t), 0)``; directors and angular velocities stay as built.
+    Stands in for the rod's own time stepper on the host, which is outside this package."""
+
+    def __init__(self, rod, amplitude=0.05, frequency=1.0):
+        self.rod = rod
+        self.amplitude = float(amplitude)
+        self.frequency = float(frequency)
+        self.rest = rod.position_collection.copy()
+
+    def set_time(self, t: float) -> None:
+        w = 2.0 * np.pi * self.frequency
+        self.rod.position_collection[...] = self.rest
+        self.rod.position_collection[1] += self.amplitude * np.sin(w * t)
+        self.rod.velocity_collection[...] = 0.0
+        self.rod.velocity_collection[1] = self.amplitude * w * np.cos(w * t)
+
+
+def rod_coupled_step(flow, interactor, flow_forces, rod, kinematics, rod_dt, free_stream_velocity, dt_prefac=0.25):
+    """One flow step of the flexible-filament driver (``flow_past_rod_case.py:231-250``):
+    the flow dt (CFL with ``dt_prefac``), ``int(dt / min(dt, rod_dt))`` rod substeps, each
+    moving the rod (here: its prescribed kinematics), applying the flow forces to it (the
+    interactor's forces and torques, without spreading) and advancing the interactor, then
+    one interactor call (spreading) and the flow step. Returns the flow dt."""
+    dt = flow.compute_stable_timestep(dt_prefac=dt_prefac)
+    substeps = int(dt / min(dt, rod_dt))
+    local_dt = dt / substeps
+    t = flow.time
+    for _ in range(substeps):
+        t = t + local_dt
+        kinematics.set_time(t)
+        rod.external_forces[...] = 0.0
+        rod.external_torques[...] = 0.0
+        flow_forces.apply_forces(rod, t)
+        interactor.time_step(dt=local_dt)
+    interactor()
+    flow.time_step(dt=dt, free_stream_velocity=free_stream_velocity)
+    return dt
