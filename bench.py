@@ -23,7 +23,10 @@ Secondary objects on the same line: "batched" (64 small coupled instances in one
 step against the same instances stepped one after another), "adjoint" (forward rollout
 against forward with checkpoints plus the adjoint sweep, 128^3, 200 steps) and
 "fast_diagonalisation" (the Neumann fast-diagonalisation vector solve at 256^3 FP32 against
-the FP32 MFMA peak, the flow step with that solver, and the numpy oracle of the solve).
+the FP32 MFMA peak, the flow step with that solver, and the numpy oracle of the solve),
+"rod_coupling" (the flexible-filament driver loop at 128^3 with a heaving rod: rod substeps with
+the flow forces, one spreading per flow step) and "other_sizes" (the sphere workload at 192^3
+fp32 and fp64 and at 256^3 fp64, each with its roofline).
 """
 
 from __future__ import annotations
@@ -292,6 +295,112 @@ def adjoint_vs_forward(n=128, steps=200, checkpoint=20, seed=5):
     }
 
 
+ROD_WORKLOAD = ("heaving flexible-filament coupling, 128^3 fp32, nu=1e-3, U=(1,0,0): rod length 0.3, 64 "
+                "elements, radius 0.005, base (0.3, 0.5, 0.5) along x, heave 0.05 sin(2 pi t) in y, forcing at "
+                "the 65 nodes, rod substeps of 0.01 l_elem with the flow forces applied, CFL dt x 0.25")
+
+
+def rod_coupled(n=128, steps=20, warmup=2):
+    """The flexible-filament driver loop (flow_past_rod_case.py:231-250) on a prescribed heave:
+    per flow step, rod substeps (kinematics on the host, the flow forces on the rod: Lagrangian
+    forcing on the device, host transfer), one spreading, the flow step. Reported against the
+    flow step alone at the same size."""
+    import logging
+
+    import torch
+
+    from sopht_amd import simulator as sps
+    from sopht_amd.utils import synthetic
+
+    logging.getLogger("sopht_amd").setLevel(logging.ERROR)
+    real_t = np.float32
+    flow = sps.UnboundedNavierStokesFlowSimulator3D(
+        grid_size=(n, n, n), x_range=1.0, kinematic_viscosity=1e-3, cfl=0.1, real_t=real_t,
+        with_forcing=True, with_free_stream_flow=True)
+    w0 = synthetic.random_phase_vorticity((n, n, n), 1.0, 6, 42, 10.0)
+    flow.vorticity_field.copy_(torch.from_numpy(w0.astype(real_t)))
+    U = np.array([1.0, 0.0, 0.0])
+    flow.recover_velocity(U)
+    n_elem, length = 64, 0.3
+    rod = sps.StraightRod(n_elem, np.array([0.3, 0.5, 0.5]), np.array([1.0, 0.0, 0.0]), np.array([0.0, 1.0, 0.0]),
+                          length, 0.005, density=1.0)
+    inter = sps.CosseratRodFlowInteraction(rod, flow.eul_grid_forcing_field, flow.velocity_field, -2e4, -1e2, flow.dx,
+                                           3, sps.CosseratRodNodalForcingGrid, real_t=real_t)
+    kin = synthetic.HeavingRod(rod, amplitude=0.05, frequency=1.0)
+    ff = sps.FlowForces(inter)
+    rod_dt = 0.01 * length / n_elem
+    for _ in range(warmup):
+        synthetic.rod_coupled_step(flow, inter, ff, rod, kin, rod_dt, U)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    subs = 0
+    for _ in range(steps):
+        dt = synthetic.rod_coupled_step(flow, inter, ff, rod, kin, rod_dt, U)
+        subs += int(dt / min(dt, rod_dt))
+    torch.cuda.synchronize()
+    coupled = time.perf_counter() - t0
+    inter.check_support_in_domain()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        flow.time_step(dt=1e-4, free_stream_velocity=U)
+    torch.cuda.synchronize()
+    plain = time.perf_counter() - t0
+    return {
+        "workload": ROD_WORKLOAD,
+        "ms_per_flow_step": round(1e3 * coupled / steps, 4),
+        "cell_updates_per_s": n**3 * steps / coupled,
+        "rod_substeps_per_flow_step": round(subs / steps, 2),
+        "ms_per_rod_substep": round(1e3 * (coupled - plain) / max(subs, 1), 4),
+        "flow_step_alone_ms": round(1e3 * plain / steps, 4),
+        "steps": steps,
+        "note": "rod mechanics prescribed on the host (pyelastica absent); parity of the rod dynamics unpinned",
+    }
+
+
+def plain_rate(n, real_t, steps=20, warmup=3):
+    """The coupled sphere loop at another size or precision (the 192^3 / 256^3 workload
+    family): ms per step, cell-updates/s and the step's fraction of the 76-word HBM model,
+    with the dominant kernel's roofline."""
+    import ctypes as C
+
+    import torch
+
+    from sopht_amd import _lib as L
+
+    flow, inter, U = build(n, real_t)
+    for _ in range(warmup):
+        step(flow, inter, U)
+    lib = L.lib()
+    L.check(lib.sopht_ns3d_enable_timing(flow._plan, 1))
+    ms = (C.c_double * 16)()
+    cnt = C.c_int64()
+    lib.sopht_ns3d_stage_times(flow._plan, ms, 16, C.byref(cnt), 1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step(flow, inter, U)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    nst = lib.sopht_ns3d_stage_times(flow._plan, ms, 16, C.byref(cnt), 0)
+    names = [lib.sopht_ns3d_stage_name(i).decode() for i in range(nst)]
+    avg = {names[i]: ms[i] / max(cnt.value, 1) for i in range(nst)}
+    word = np.dtype(real_t).itemsize
+    words = stage_words(n, n, n)
+    dom = max(avg, key=avg.get)
+    gbs = words[dom] * n**3 * word / (avg[dom] * 1e-3) / 1e9
+    ms_step = 1e3 * el / steps
+    step_gbs = STEP_WORDS_PER_CELL * word * n**3 / (ms_step * 1e-3) / 1e9
+    return {
+        "grid": n, "dtype": "f32" if real_t == np.float32 else "f64",
+        "ms_per_step": round(ms_step, 4), "cell_updates_per_s": n**3 * steps / el,
+        "step_hbm_frac_76w": round(step_gbs / HBM_PEAK_GBS, 4),
+        "stage_ms": {k: round(v, 4) for k, v in avg.items()},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4)},
+        "steps": steps,
+    }
+
+
 FD_WORKLOAD = "Neumann fast-diagonalisation vector Poisson solve, 256^3 fp32 (3 components)"
 FP32_MFMA_PEAK_TFLOPS = 157.3
 
@@ -369,6 +478,17 @@ def fastdiag_solve(n=256, reps=20, cpu=True):
     return result
 
 
+def other_sizes(steps):
+    """The sphere workload at 192^3 (fp32, fp64) and 256^3 fp64."""
+    import torch
+
+    out = []
+    for n, real_t in ((192, np.float32), (192, np.float64), (256, np.float64)):
+        out.append(plain_rate(n, real_t, steps))
+        torch.cuda.empty_cache()
+    return out
+
+
 def copy_probe(n, reps):
     """A plain device copy of known size (one (n, n, n) fp32 field per launch, read once
     and written once) through the library's elementwise copy kernel: the calibration run
@@ -423,6 +543,12 @@ def main():
     ap.add_argument("--fastdiag-reps", type=int, default=20,
                     help="timed fast-diagonalisation solves and steps (0 = skip)")
     ap.add_argument("--fastdiag-only", action="store_true", help="run only the fast-diagonalisation solve")
+    ap.add_argument("--rod-steps", type=int, default=20,
+                    help="flow steps of the heaving-filament coupling at 128^3 (0 = skip)")
+    ap.add_argument("--rod-only", action="store_true", help="run only the heaving-filament coupling")
+    ap.add_argument("--sizes-steps", type=int, default=20,
+                    help="steps of the 192^3 fp32/fp64 and 256^3 fp64 lines (0 = skip)")
+    ap.add_argument("--sizes-only", action="store_true", help="run only the other sizes and precisions")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -444,6 +570,18 @@ def main():
 
         torch.cuda.set_device(0)
         print(json.dumps(fastdiag_solve(reps=max(1, args.fastdiag_reps), cpu=args.cpu_steps > 0)))
+        return 0
+    if args.rod_only:
+        import torch
+
+        torch.cuda.set_device(0)
+        print(json.dumps(rod_coupled(steps=max(1, args.rod_steps))))
+        return 0
+    if args.sizes_only:
+        import torch
+
+        torch.cuda.set_device(0)
+        print(json.dumps(other_sizes(max(1, args.sizes_steps))))
         return 0
     if args.batch_only:
         import torch
@@ -521,6 +659,14 @@ def main():
     if args.fastdiag_reps > 0:
         torch.cuda.empty_cache()
         fastdiag = fastdiag_solve(reps=args.fastdiag_reps, cpu=args.cpu_steps > 0)
+    rod = None
+    if args.rod_steps > 0:
+        torch.cuda.empty_cache()
+        rod = rod_coupled(steps=args.rod_steps)
+    sizes = None
+    if args.sizes_steps > 0:
+        torch.cuda.empty_cache()
+        sizes = other_sizes(args.sizes_steps)
     cpu = None
     if args.cpu_steps > 0:
         try:
@@ -554,6 +700,8 @@ def main():
         "batched": batched,
         "adjoint": adjoint,
         "fast_diagonalisation": fastdiag,
+        "rod_coupling": rod,
+        "other_sizes": sizes,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -128,6 +128,24 @@ int sopht_elementwise_complex_product_3d(int dtype, void* product, const int64_t
                                          const void* field_2, const int64_t* f2_strides,
                                          int64_t nz, int64_t ny, int64_t nx, void* stream);
 
+/* PassiveTransportFlowSimulator._advection_and_diffusion_time_step —
+ * sopht/simulator/flow/passive_transport_flow_simulators.py:100-113 (3D): the conservative
+ * ENO3 advection step (advection_timestep_3d.py:12-99, inv_dx = -dt_by_dx) followed by the
+ * Euler diffusion step (diffusion_timestep_3d.py:12-80, prefactor nu_dt_by_dx2) of ncomp
+ * (1 or 3) contiguous (nz, ny, nx) component fields, in place, in one fused sweep; scratch
+ * holds ncomp fields of the same shape (contents undefined on return). velocity: three
+ * contiguous components. */
+int sopht_passive_transport_step_3d(int dtype, int ncomp, void* const field[3], void* const scratch[3],
+                                    const void* const velocity[3], int64_t nz, int64_t ny,
+                                    int64_t nx, double dt_by_dx, double nu_dt_by_dx2, void* stream);
+
+/* compute_advection_diffusion_stable_timestep's reduction —
+ * sopht/simulator/flow/passive_transport_flow_simulators.py:139-155: magnitude (optional, n
+ * elements) = |u0| + |u1| + |u2| per cell, *out = its maximum (real_t, on the device), over n
+ * contiguous cells; partial holds partial_cap real_t words of scratch. */
+int sopht_velocity_abs_sum_max_3d(int dtype, const void* const velocity[3], int64_t n, void* magnitude,
+                                  void* partial, int64_t partial_cap, void* out, void* stream);
+
 /* gen_advection_flux_conservative_eno3_pyst_kernel_3d — .../advection_flux_3d.py:12-233.
  * Accumulates into flux on [2:-2]^3 (six sub-stencils in the reference order). */
 int sopht_advection_flux_conservative_eno3_3d(int dtype, void* flux, const int64_t* flux_strides,
@@ -228,6 +246,22 @@ int sopht_lag_to_eul_spread_3d(int dtype, int n_components, sopht_spread3d* ws,
                                int64_t num_lag_nodes, void* stream);
 /* Reads back (synchronising `stream`) and clears the out-of-domain flag of ws. */
 int sopht_spread3d_check(sopht_spread3d* ws, void* stream);
+
+/* No counterpart in the reference: sopht_lag_to_eul_spread_3d for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same arguments, result and order of summation, with no workspace: the kernels
+ * use a library-owned static device scratch of 8.5 MB per GPU (held until the process ends)
+ * sized for at most 8192 Lagrangian nodes; larger n is rejected with SOPHT_ERR_INVALID. The
+ * gather plan is rebuilt on every call (the scratch keeps only the last call's contents).
+ * Calls that overlap on one GPU (several streams, host threads) race on the scratch: order
+ * them on one stream. out_of_domain (optional device int) receives 1 when a node's 4^3
+ * support left the grid (those contributions are dropped, as in the workspace form). The
+ * call allocates nothing. */
+int sopht_lag_to_eul_spread_3d_noalloc(int dtype, int n_components, void* const eul[3],
+                                       const int64_t* eul_strides, int64_t nz, int64_t ny,
+                                       int64_t nx, const void* lag_field, const void* weights,
+                                       const int64_t* nearest_index, int64_t n_lag,
+                                       int* out_of_domain, void* stream);
 
 /* VirtualBoundaryForcing.compute_interaction_force_on_lag_grid —
  * sopht/numeric/immersed_boundary_ops/VirtualBoundaryForcing.py:187-230 fused:

@@ -74,6 +74,8 @@ SIGNATURES = {
     "sopht_elementwise_complex_product_3d": [_i, _vp, _pi64, _vp, _pi64, _vp, _pi64, _i64, _i64, _i64, _vp],
     "sopht_advection_flux_conservative_eno3_3d": [
         _i, _vp, _pi64, _vp, _pi64, _p3, _pi64, _i64, _i64, _i64, _d, _vp],
+    "sopht_passive_transport_step_3d": [_i, _i, _p3, _p3, _p3, _i64, _i64, _i64, _d, _d, _vp],
+    "sopht_velocity_abs_sum_max_3d": [_i, _p3, _i64, _vp, _vp, _i64, _vp, _vp],
     "sopht_vorticity_stretching_flux_3d": [_i, _p3, _pi64, _p3, _pi64, _p3, _pi64, _i64, _i64, _i64, _d, _vp],
     "sopht_poisson3d_create": [_i, _i64, _i64, _i64, _d, _vp, C.POINTER(_vp)],
     "sopht_poisson3d_solve": [_vp, _vp, _pi64, _vp, _pi64, _vp],
@@ -92,6 +94,7 @@ SIGNATURES = {
     "sopht_spread3d_destroy": [_vp],
     "sopht_lag_to_eul_spread_3d": [_i, _i, _vp, _p3, _pi64, _i64, _i64, _i64, _vp, _vp, _vp, _i64, _vp],
     "sopht_spread3d_check": [_vp, _vp],
+    "sopht_lag_to_eul_spread_3d_noalloc": [_i, _i, _p3, _pi64, _i64, _i64, _i64, _vp, _vp, _vp, _i64, _vp, _vp],
     "sopht_vbf_lag_force_3d": [
         _i, _i, _p3, _pi64, _i64, _i64, _i64, _vp, _vp, _i64, _d, _d, _d, _d,
         _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],

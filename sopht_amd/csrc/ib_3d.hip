@@ -15,6 +15,7 @@
 // EulerianLagrangianGridCommunicator3D.py:123-124) and are skipped.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <new>
 
 #include "common.h"
@@ -128,25 +129,19 @@ int sopht_spread3d_destroy(sopht_spread3d* w) {
   return SOPHT_OK;
 }
 
-int sopht_lag_to_eul_spread_3d(int dtype, int ncomp, sopht_spread3d* ws, void* const eul[3],
-                               const int64_t* es, int64_t nz, int64_t ny, int64_t nx,
-                               const void* lag, const void* weights, const int64_t* idx,
-                               int64_t n, void* stream) {
-  SOPHT_REQUIRE(ws != nullptr, "spread workspace is null");
-  SOPHT_REQUIRE(ncomp == 1 || ncomp == 3, "Invalid number of components for interpolation, must be either 1 or 3");
-  SOPHT_REQUIRE(n <= ws->capacity, "more Lagrangian nodes than the spread workspace holds");
-  SOPHT_REQUIRE(nz * ny * nx < (int64_t(1) << 32), "grid too large for 32-bit cell keys");
-  if (n == 0) return SOPHT_OK;
-  hipStream_t s = static_cast<hipStream_t>(stream);
+}  // extern "C"
+
+namespace {
+
+// Spreading through a workspace (the kernels of both spreading entry points). force: rebuild
+// the plan whatever the keys (a workspace whose previous contents are not this caller's).
+int spread_impl(int dtype, int ncomp, sopht_spread3d* ws, void* const eul[3], const int64_t* es,
+                int64_t nz, int64_t ny, int64_t nx, const void* lag, const void* weights,
+                const int64_t* idx, int64_t n, hipStream_t s, int force) {
   int64_t padded = 1;
   while (padded < n) padded <<= 1;
   // keys of this call into ws->prev (and the "changed" flag) over the workspace's padded
   // length, so a later call with fewer nodes still sees its stale tail as changed
-  // a plan built for another grid is rebuilt even if the packed keys happen to agree
-  const int force = (ws->plan_nz != nz || ws->plan_ny != ny || ws->plan_nx != nx) ? 1 : 0;
-  ws->plan_nz = nz;
-  ws->plan_ny = ny;
-  ws->plan_nx = nx;
   k_spread_keys<<<blocks_for(ws->padded, 256), 256, 0, s>>>(idx, n, ws->padded, nz, ny, nx,
                                                             ws->prev, ws->flag, force);
   SOPHT_CHECK_LAUNCH();
@@ -193,6 +188,132 @@ int sopht_lag_to_eul_spread_3d(int dtype, int ncomp, sopht_spread3d* ws, void* c
                     ws->owners, ws->flag);
               }()));
   SOPHT_CHECK_LAUNCH();
+  return SOPHT_OK;
+}
+
+// ---------------------------------------------------------------- spreading without allocation
+// The library-owned scratch of sopht_lag_to_eul_spread_3d_noalloc: a workspace for
+// kNoAllocNodes nodes carved out of one static device array. Each GPU that runs the entry
+// point holds its own copy for the life of the process; its address on a device comes from
+// hipGetSymbolAddress (the array's name is not a device address in host code).
+constexpr int64_t kNoAllocNodes = 8192;
+constexpr size_t kAlign = 256;
+constexpr size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
+constexpr size_t kKeysBytes = align_up(sizeof(uint64_t) * kNoAllocNodes);
+constexpr size_t kFlagBytes = align_up(8 * sizeof(int));
+constexpr size_t kSlotBytes = align_up(sizeof(int) * 64 * kNoAllocNodes);
+constexpr size_t kTableBytes = align_up(sizeof(int) * (kTableMax + 1));
+constexpr size_t kBboxBytes = align_up(sizeof(int) * 8);
+constexpr size_t kScratchBytes = 2 * kKeysBytes + kFlagBytes + 4 * kSlotBytes + kTableBytes + kBboxBytes;
+
+}  // namespace
+
+// 8.5 MB per GPU (the workspace of 8192 nodes), held until the process ends
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The scratch's device address on the current device, looked up once per device.
+int scratch_address(void** out) {
+  int dev = 0;
+  SOPHT_CHECK_HIP(hipGetDevice(&dev));
+  SOPHT_REQUIRE(dev >= 0 && dev < kMaxDevices, "device ordinal beyond the library's table");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    SOPHT_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return SOPHT_OK;
+}
+
+// Flags start as a fresh workspace's ({no out-of-domain node, keys changed}, empty box table);
+// a kernel, not a memset, so the call can sit in a captured graph.
+__global__ void k_noalloc_reset(int* flag, int* bbox) {
+  const int i = threadIdx.x;
+  if (i < 8) {
+    flag[i] = i == 1 ? 1 : 0;
+    bbox[i] = 0;
+  }
+}
+
+__global__ void k_noalloc_report(const int* flag, int* out_of_domain) {
+  if (threadIdx.x == 0) *out_of_domain = flag[0] != 0 ? 1 : 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sopht_lag_to_eul_spread_3d(int dtype, int ncomp, sopht_spread3d* ws, void* const eul[3],
+                               const int64_t* es, int64_t nz, int64_t ny, int64_t nx,
+                               const void* lag, const void* weights, const int64_t* idx,
+                               int64_t n, void* stream) {
+  SOPHT_REQUIRE(ws != nullptr, "spread workspace is null");
+  SOPHT_REQUIRE(ncomp == 1 || ncomp == 3, "Invalid number of components for interpolation, must be either 1 or 3");
+  SOPHT_REQUIRE(n <= ws->capacity, "more Lagrangian nodes than the spread workspace holds");
+  SOPHT_REQUIRE(nz * ny * nx < (int64_t(1) << 32), "grid too large for 32-bit cell keys");
+  if (n == 0) return SOPHT_OK;
+  // a plan built for another grid is rebuilt even if the packed keys happen to agree
+  const int force = (ws->plan_nz != nz || ws->plan_ny != ny || ws->plan_nx != nx) ? 1 : 0;
+  ws->plan_nz = nz;
+  ws->plan_ny = ny;
+  ws->plan_nx = nx;
+  return spread_impl(dtype, ncomp, ws, eul, es, nz, ny, nx, lag, weights, idx, n,
+                     static_cast<hipStream_t>(stream), force);
+}
+
+int sopht_lag_to_eul_spread_3d_noalloc(int dtype, int ncomp, void* const eul[3], const int64_t* es,
+                                       int64_t nz, int64_t ny, int64_t nx, const void* lag,
+                                       const void* weights, const int64_t* idx, int64_t n,
+                                       int* out_of_domain, void* stream) {
+  SOPHT_REQUIRE(ncomp == 1 || ncomp == 3, "Invalid number of components for interpolation, must be either 1 or 3");
+  SOPHT_REQUIRE(n >= 0, "negative number of Lagrangian nodes");
+  SOPHT_REQUIRE(n <= kNoAllocNodes,
+                "more Lagrangian nodes than the allocation-free spreading holds (at most 8192); "
+                "use sopht_lag_to_eul_spread_3d with a workspace");
+  SOPHT_REQUIRE(nz * ny * nx < (int64_t(1) << 32), "grid too large for 32-bit cell keys");
+  if (n == 0) return SOPHT_OK;
+  void* base = nullptr;
+  int rc = scratch_address(&base);
+  if (rc) return rc;
+  unsigned char* p = static_cast<unsigned char*>(base);
+  sopht_spread3d ws{};
+  ws.capacity = kNoAllocNodes;
+  ws.padded = kNoAllocNodes;  // a power of two
+  ws.keys = reinterpret_cast<uint64_t*>(p);
+  p += kKeysBytes;
+  ws.prev = reinterpret_cast<uint64_t*>(p);
+  p += kKeysBytes;
+  ws.flag = reinterpret_cast<int*>(p);
+  p += kFlagBytes;
+  ws.pcount = reinterpret_cast<int*>(p);
+  p += kSlotBytes;
+  ws.poff = reinterpret_cast<int*>(p);
+  p += kSlotBytes;
+  ws.plist = reinterpret_cast<uint32_t*>(p);
+  p += kSlotBytes;
+  ws.owners = reinterpret_cast<uint32_t*>(p);
+  p += kSlotBytes;
+  ws.table = reinterpret_cast<int*>(p);
+  p += kTableBytes;
+  ws.bbox = reinterpret_cast<int*>(p);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  // the scratch holds whatever the last call on this GPU left: reset the flags and rebuild
+  // the plan from this call's keys (every key slot up to the capacity is rewritten)
+  k_noalloc_reset<<<1, 64, 0, s>>>(ws.flag, ws.bbox);
+  SOPHT_CHECK_LAUNCH();
+  rc = spread_impl(dtype, ncomp, &ws, eul, es, nz, ny, nx, lag, weights, idx, n, s, 1);
+  if (rc) return rc;
+  if (out_of_domain) {
+    k_noalloc_report<<<1, 64, 0, s>>>(ws.flag, out_of_domain);
+    SOPHT_CHECK_LAUNCH();
+  }
   return SOPHT_OK;
 }
 

@@ -11,10 +11,9 @@ from .batch import BatchedFlowSimulation
 from .flow import (
     FlowSimulator,
     UnboundedNavierStokesFlowSimulator3D,
-    compute_advection_diffusion_stable_timestep,
     create_unbounded_flow_simulator_3d,
 )
-from .passive_transport import PassiveTransportFlowSimulator
+from .passive_transport import PassiveTransportFlowSimulator, compute_advection_diffusion_stable_timestep
 from .cosserat_rod import (
     CosseratRodElementCentricForcingGrid,
     CosseratRodFlowInteraction,

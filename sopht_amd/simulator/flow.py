@@ -270,19 +270,6 @@ class UnboundedNavierStokesFlowSimulator3D(FlowSimulator):
         return float(torch.linalg.vector_norm(divergence_field.double()).item()) * self.dx ** (self.grid_dim / 2.0)
 
 
-def compute_advection_diffusion_stable_timestep(
-    velocity_field, velocity_magnitude_field, grid_dim, dx, cfl, kinematic_viscosity, real_t=np.float32
-):
-    """``passive_transport_flow_simulators.py:139-155`` on a device velocity field (one
-    reduction on the device; ``velocity_magnitude_field`` receives ``sum_i |u_i|``)."""
-    import torch
-
-    tol = 10 * np.finfo(real_t).eps
-    torch.sum(torch.abs(velocity_field), dim=0, out=velocity_magnitude_field)
-    vmax = real_t(velocity_magnitude_field.max().item())
-    return min(cfl * dx / (vmax + tol), 0.9 * dx**2 / (2 * grid_dim) / kinematic_viscosity + tol)
-
-
 def create_unbounded_flow_simulator_3d(
     grid_size,
     x_range,

@@ -322,6 +322,40 @@ def test_passive_transport_time_step(dev, grid_size_x, precision, field_type, rn
     np.testing.assert_allclose(to_host(sim.primary_field), ref, rtol=tol(real_t), atol=tol(real_t))
 
 
+@pytest.mark.parametrize("shape", [(20, 37, 70), (33, 9, 130)])
+@pytest.mark.parametrize("precision", ["single", "double"])
+@pytest.mark.parametrize("field_type", ["scalar", "vector"])
+def test_passive_transport_fused_sweep_tiles(dev, shape, precision, field_type, rng):
+    """The fused transport sweep across tile, z-chunk and grid edges (several x and y tiles,
+    partial last tiles, z chunks) over three steps, against the oracle's advection step then
+    diffusion step per component."""
+    from oracle import stencils as OS
+
+    real_t = PREC[precision]
+    nu, dt = 1e-3, 2e-3
+    sim = sps().PassiveTransportFlowSimulator(
+        kinematic_viscosity=nu, grid_dim=3, grid_size=shape, x_range=1.0, real_t=real_t, field_type=field_type)
+    dx = sim.dx
+    u = (rng.random(tuple(sim.velocity_field.shape)) - 0.5).astype(real_t)
+    f = rng.random(tuple(sim.primary_field.shape)).astype(real_t)
+    sim.velocity_field.copy_(to_dev(u, dev))
+    sim.primary_field.copy_(to_dev(f, dev))
+    ref = f.copy()
+    buf = np.zeros(shape, dtype=real_t)
+    for _ in range(3):
+        sim.time_step(dt=dt)
+        OS.advection_timestep_euler_forward_conservative_eno3(ref, buf, u, real_t(dt / dx))
+        OS.diffusion_timestep_euler_forward(ref, buf, real_t(nu * dt / dx / dx))
+    np.testing.assert_allclose(to_host(sim.primary_field), ref, rtol=tol(real_t), atol=tol(real_t))
+    # the CFL reduction leaves sum_i |u_i| per cell in the scalar buffer, as the reference
+    dt_cfl = sim.compute_stable_timestep()
+    mag = np.sum(np.fabs(u), axis=0)
+    np.testing.assert_array_equal(to_host(sim.buffer_scalar_field), (np.fabs(u[0]) + np.fabs(u[1])) + np.fabs(u[2]))
+    tol_t = 10 * np.finfo(real_t).eps
+    want = min(0.1 * dx / (real_t(mag.max()) + tol_t), 0.9 * dx**2 / 6 / nu + tol_t)
+    assert dt_cfl == want
+
+
 @pytest.mark.parametrize("grid_size_x", [4, 8])
 @pytest.mark.parametrize("precision", ["single", "double"])
 def test_passive_transport_stable_timestep(dev, grid_size_x, precision):
