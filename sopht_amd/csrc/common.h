@@ -159,6 +159,12 @@ int poisson_vector_solve_timed(sopht_poisson3d* P, void* const sol[3], const int
 int laplacian_filter_field(int dtype, int order, int type, void* field, const int64_t* fs,
                            int64_t nz, int64_t ny, int64_t nx, void* flux, const int64_t* xs,
                            void* fbuf, const int64_t* bs, int bw, hipStream_t s);
+// The step's Laplacian vorticity filter on the three contiguous components of w: one launch
+// per axis (buf: three scratch components for the multiplicative filter's passes). The
+// standalone factory's kernel sequence covers the orders and rows this does not.
+bool fused_filter_supported(int order, int64_t nx);
+int fused_vorticity_filter(int dtype, int order, int type, void* w, void* buf, int64_t nz, int64_t ny,
+                           int64_t nx, hipStream_t s);
 // Fast-diagonalisation (Neumann) vector solve with stage marks first .. first+4.
 int fastdiag_vector_solve_timed(sopht_fastdiag3d* P, void* const sol[3], const int64_t* ss,
                                 const void* const rhs[3], const int64_t* rs, hipStream_t s,

@@ -83,10 +83,18 @@ int step_impl(sopht_ns3d* P, const void* vort_in, void* vort_out, const void* ve
     const int64_t n = g.nz * g.ny * g.nx;
     const int64_t st[3] = {g.ny * g.nx, g.nx, 1};
     T* bf = static_cast<T*>(buffer);
-    for (int c = 0; c < 3; ++c) {
-      int rc = laplacian_filter_field(P->dtype, P->filter_order, P->filter_type, w + c * n, st, g.nz,
-                                      g.ny, g.nx, bf, st, bf + n, st, 1, s);
+    if (fused_filter_supported(P->filter_order, g.nx)) {
+      // one launch per axis for the three components (the buffer holds the multiplicative
+      // filter's intermediate passes)
+      int rc = fused_vorticity_filter(P->dtype, P->filter_order, P->filter_type, w, bf, g.nz, g.ny,
+                                      g.nx, s);
       if (rc) return rc;
+    } else {
+      for (int c = 0; c < 3; ++c) {
+        int rc = laplacian_filter_field(P->dtype, P->filter_order, P->filter_type, w + c * n, st,
+                                        g.nz, g.ny, g.nx, bf, st, bf + n, st, 1, s);
+        if (rc) return rc;
+      }
     }
     for (int c = 0; c < 3; ++c) {
       int rc = sopht_penalise_field_boundary_3d(P->dtype, w + c * n, st, g.nz, g.ny, g.nx, P->width,

@@ -217,7 +217,7 @@ def test_c5_fp32_gradient_against_fp64_128(dev):
         J = sps().time_averaged_force(forces, dts)
         J.backward()
         grads[real_t] = to_host(theta.grad).astype(np.float64)
-        Js[real_t] = float(J)
+        Js[real_t] = float(J.detach())
         del ro, forces, J
         torch.cuda.empty_cache()
     err = rel_l2(grads[np.float32], grads[np.float64])

@@ -264,7 +264,7 @@ def test_fused_velocity_max_matches_full_reduction(dev, precision, shape, rng):
 
 
 @pytest.mark.parametrize("filter_type", ["multiplicative", "convolution"])
-@pytest.mark.parametrize("filter_order", [1, 2])
+@pytest.mark.parametrize("filter_order", [1, 2, 5, 9])
 @pytest.mark.parametrize("precision", ["single", "double"])
 def test_one_step_with_vorticity_filter(dev, filter_type, filter_order, precision, rng):
     """filter_vorticity=True (navier_stokes_flow_simulators.py:274-292,396-408,473): the fused
@@ -272,6 +272,20 @@ def test_one_step_with_vorticity_filter(dev, filter_type, filter_order, precisio
     real_t = PREC[precision]
     shape = (16, 12, 16)
     setting = {"order": filter_order, "type": filter_type}
+    _filtered_step_against_oracle(dev, shape, setting, real_t, rng)
+
+
+@pytest.mark.parametrize("shape", [(24, 12, 96), (12, 8, 384), (6, 48, 64)])
+@pytest.mark.parametrize("precision", ["single", "double"])
+def test_filtered_step_order5_convolution_tiles(dev, shape, precision, rng):
+    """The rod example's filter (order 5, convolution: flow_past_rod_case.py:103-104) across
+    row and line-block edges of the fused per-axis filter kernels (rows longer than one and
+    several wavefronts, more x than one line block), and the multiplicative order 2 default."""
+    for setting in ({"order": 5, "type": "convolution"}, {"order": 2, "type": "multiplicative"}):
+        _filtered_step_against_oracle(dev, shape, setting, PREC[precision], rng)
+
+
+def _filtered_step_against_oracle(dev, shape, setting, real_t, rng):
     kw = dict(kinematic_viscosity=1e-2, time=0.0, with_forcing=True, with_free_stream_flow=True)
     o = OF.UnboundedNavierStokesFlowSimulator3D(shape, 1.0, real_t=np.float64, filter_setting_dict=setting, **kw)
     g = sps().UnboundedNavierStokesFlowSimulator3D(grid_size=shape, x_range=1.0, real_t=real_t,
