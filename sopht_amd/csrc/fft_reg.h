@@ -211,7 +211,7 @@ SOPHT_LINE_PLAN(96, 12, 4, 4, 6)
 SOPHT_LINE_PLAN(128, 8, 8, 4, 4)
 SOPHT_LINE_PLAN(192, 12, 4, 4, 12)
 SOPHT_LINE_PLAN(256, 8, 8, 8, 4)
-SOPHT_LINE_PLAN(384, 24, 8, 4, 12)
+SOPHT_LINE_PLAN(384, 12, 4, 4, 2, 12)
 #ifndef SOPHT_PLAN512
 SOPHT_LINE_PLAN(512, 8, 8, 8, 8)
 #elif SOPHT_PLAN512 == 16
@@ -219,7 +219,7 @@ SOPHT_LINE_PLAN(512, 16, 16, 16, 2)
 #else
 SOPHT_LINE_PLAN(512, 32, 32, 16)
 #endif
-SOPHT_LINE_PLAN(768, 24, 8, 8, 12)
+SOPHT_LINE_PLAN(768, 12, 4, 4, 4, 12)
 SOPHT_LINE_PLAN(1024, 16, 16, 8, 8)
 #undef SOPHT_LINE_PLAN
 

@@ -302,6 +302,11 @@ __global__ void __launch_bounds__((ColTile<T, L>::NT)) k_ycols(SlabGeom g, Cplx<
 // for three 512-thread workgroups per CU, spills).
 template <typename T>
 constexpr int kZconvWaves = sizeof(T) == 4 ? 4 : 2;
+// more elements per thread need more registers: twelve (L = 384, 768) one wavefront less per
+// SIMD, sixteen (L = 1024) two less
+template <typename T, int L>
+constexpr int kZconvWavesFor = sizeof(T) != 4 ? kZconvWaves<T>
+                               : (LinePlan<L>::E >= 16 ? 2 : (LinePlan<L>::E > 8 ? 3 : kZconvWaves<T>));
 // pencil tiles per z-pass workgroup at most (all of them for a 3-component solve)
 constexpr int kZconvJobs = 6;
 
@@ -330,7 +335,7 @@ constexpr int at_least_1(int v) { return v > 0 ? v : 1; }
 // stays in registers, and the next pencil tile's loads are in flight during the transforms.
 // G^ is stored [kyo][kx][kz], kz fastest, so a team reads its line contiguously.
 template <typename T, int L>
-__global__ void __launch_bounds__((ColTile<T, L>::NT)) __attribute__((amdgpu_waves_per_eu(kZconvWaves<T>))) k3_zconv(SlabGeom g, Cplx<T>* H, const T* ghat,
+__global__ void __launch_bounds__((ColTile<T, L>::NT)) __attribute__((amdgpu_waves_per_eu(kZconvWavesFor<T, L>))) k3_zconv(SlabGeom g, Cplx<T>* H, const T* ghat,
                                                               const Cplx<T>* tw) {
   using CT = ColTile<T, L>;
   constexpr int E = CT::E, TT = CT::TT, WC = CT::WC, S = CT::S, K = E / 2;

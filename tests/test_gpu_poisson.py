@@ -154,3 +154,19 @@ def test_chunked_passes_in_the_step_and_adjoint(dev, mode, monkeypatch):
         results[m] = (forces.detach().clone(), theta.grad.clone(), ro.flow.vorticity_field.clone())
     for a, b in zip(results["none"], results[mode]):
         assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("precision", ["single", "double"])
+@pytest.mark.parametrize("shape", [(192, 6, 8), (8, 192, 6), (6, 8, 192), (4, 6, 384), (384, 4, 6)])
+def test_solve_on_384_and_768_point_lines(dev, precision, shape, rng):
+    """The twelve-point-per-thread plans of the 384- and 768-point lines (the 192^3 workload's
+    doubled axes) in every pass, against the oracle."""
+    real_t = PREC[precision]
+    rhs = rng.standard_normal((3, *shape)).astype(real_t)
+    ref = np.zeros_like(rhs, dtype=np.float64)
+    P.UnboundedPoissonSolver3D(*shape, x_range=1.0, real_t=np.float64).vector_field_solve(ref, rhs.astype(np.float64))
+    s = solver_cls()(*shape, x_range=1.0, real_t=real_t)
+    out = to_dev(np.zeros_like(rhs), dev)
+    s.vector_field_solve(solution_vector_field=out, rhs_vector_field=to_dev(rhs, dev))
+    bound = 1e-10 if real_t == np.float64 else 2e-5
+    assert rel_l2(to_host(out), ref) <= bound, rel_l2(to_host(out), ref)
