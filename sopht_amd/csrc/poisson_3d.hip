@@ -18,7 +18,6 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
-#include <cstring>
 #include <new>
 
 #include "common.h"
@@ -34,15 +33,6 @@ struct sopht_poisson3d {
   int64_t gz;       // G^ line stride (elements): nz + 1 rounded up to a 128-byte line
   int slab_comps;   // components the slab holds (3, or 3 per instance of a batched class)
   int64_t slab_bytes, ghat_bytes;
-  // z chunks of the row and y passes (1: none): each chunk's half-spectrum rows go from the x
-  // pass to the y pass (and back from the inverse y pass to the inverse x pass) through the
-  // cache, the y passes of chunk i running on the second stream beside the x pass of i + 1
-  int chunks;
-  int64_t zchunk;
-  int chunk_mode;      // 0 none, 1 z chunks (x and y passes), 2 kx chunks (y, z and inverse y passes)
-  int64_t lines, lchunk;  // kx chunks: slab lines (2 column tiles) in all and per chunk
-  hipStream_t aux;
-  hipEvent_t ev[2 * 64 + 2];
 };
 
 namespace sopht {
@@ -78,23 +68,17 @@ constexpr int kLineCplx = 128 / int(sizeof(Cplx<T>));
 // then go to workgroups b and b + 8, which the dispatcher sends to the same XCD one after the
 // other, so the second half is read from that XCD's L2.
 constexpr int kColHalf = SOPHT_COL_HALF;
-// z-chunking of the row and y passes: above kChunkMinBytes of x-pass output, chunks of about
-// kChunkBytes (two chunks in flight stay well inside the 256 MiB Infinity Cache)
-constexpr int64_t kChunkMinBytes = int64_t(128) << 20;
-constexpr int64_t kChunkBytes = int64_t(48) << 20;
-constexpr int kDefaultChunkMode = 0;
 constexpr int64_t kRowPad = SOPHT_ROW_PAD;
 
 // (row, first kx) of the column tile of workgroup b; false for the padding workgroups.
-// l0, nl (nl > 0): only the kx lines l0 .. l0 + nl - 1 (a kx chunk; a line is 2 wc columns).
 __device__ __forceinline__ bool col_tile(int64_t b, int64_t ncols, int64_t nrows, int wc,
-                                         int64_t& row, int64_t& kx0, int64_t l0 = 0, int64_t nl = 0) {
+                                         int64_t& row, int64_t& kx0) {
   if (kColHalf) {
-    const int64_t lines = nl > 0 ? nl : (ncols + 2 * wc - 1) / (2 * wc);
+    const int64_t lines = (ncols + 2 * wc - 1) / (2 * wc);
     const int64_t line = (b / 16) * 8 + b % 8;
     const int64_t half = (b / 8) % 2;
     row = line / lines;
-    kx0 = ((l0 + line % lines) * 2 + half) * wc;
+    kx0 = ((line % lines) * 2 + half) * wc;
     return row < nrows;
   }
   const int64_t tiles = (ncols + wc - 1) / wc;
@@ -103,9 +87,9 @@ __device__ __forceinline__ bool col_tile(int64_t b, int64_t ncols, int64_t nrows
   return row < nrows;
 }
 
-inline int64_t col_tile_blocks(int64_t ncols, int64_t nrows, int wc, int64_t nl = 0) {
+inline int64_t col_tile_blocks(int64_t ncols, int64_t nrows, int wc) {
   if (kColHalf) {
-    const int64_t lines = nl > 0 ? nl : (ncols + 2 * wc - 1) / (2 * wc);
+    const int64_t lines = (ncols + 2 * wc - 1) / (2 * wc);
     return 16 * ((lines * nrows + 7) / 8);
   }
   return (ncols + wc - 1) / wc * nrows;
@@ -118,8 +102,6 @@ struct SlabGeom {
   int64_t rs;          // row stride (complex elements), >= nx + 1
   int64_t gz;          // G^ line stride (elements), >= nz + 1
   int ncomp;
-  int64_t z0, zn;      // z planes z0 .. z0 + zn - 1 of the row and y-column passes (a chunk)
-  int64_t l0, nl;      // kx lines l0 .. l0 + nl - 1 of the column passes (nl = 0: all)
   // H index of (c, z, ky, kx)
   __device__ __forceinline__ int64_t h(int c, int64_t z, int64_t ky, int64_t kx) const {
     return ((int64_t(c) * nz + z) * (2 * ny) + ky) * rs + kx;
@@ -129,18 +111,10 @@ struct SlabGeom {
   __device__ __forceinline__ int hi(int c, int z, int ky, int kx) const {
     return ((c * int(nz) + z) * (2 * int(ny)) + ky) * int(rs) + kx;
   }
-  // input row r of the chunk: component, plane and row
-  __device__ __forceinline__ void row_of(int64_t r, int& c, int64_t& z, int64_t& y) const {
-    c = int(r / (zn * ny));
-    const int64_t rem = r % (zn * ny);
-    z = z0 + rem / ny;
-    y = rem % ny;
-  }
   __device__ __forceinline__ int64_t row_h(int64_t r) const {  // H row of input row r
-    int c;
-    int64_t z, y;
-    row_of(r, c, z, y);
-    return h(c, z, y, 0);
+    const int c = int(r / (nz * ny));
+    const int64_t rem = r % (nz * ny);
+    return h(c, rem / ny, rem % ny, 0);
   }
 };
 
@@ -183,7 +157,7 @@ struct ColTile {
 
 // K1: x real-to-complex of zero-padded rows, two rows (ra, rb) per complex FFT of length
 // L = 2nx; the pair is separated with A = (Z[k] + conj Z[L-k])/2, B = (Z[k] - conj Z[L-k])/(2i).
-template <typename T, int L, bool NT>
+template <typename T, int L>
 __global__ void __launch_bounds__((RowTile<T, L>::NT)) k1_xr2c(SlabGeom g, CompPtrs in, Cplx<T>* H,
                                                              const Cplx<T>* tw) {
   using RT = RowTile<T, L>;
@@ -192,14 +166,13 @@ __global__ void __launch_bounds__((RowTile<T, L>::NT)) k1_xr2c(SlabGeom g, CompP
   Cplx<T>* xb = reinterpret_cast<Cplx<T>*>(smem_raw);
   const int t = threadIdx.x % TT;
   const int team = threadIdx.x / TT;
-  const int64_t rows = int64_t(g.ncomp) * g.zn * g.ny;
+  const int64_t rows = int64_t(g.ncomp) * g.nz * g.ny;
   const int64_t q = int64_t(blockIdx.x) * RT::NTEAM + team;
   const int64_t ra = 2 * q, rb = 2 * q + 1;
   auto row_ptr = [&](int64_t r) -> const T* {
-    int c;
-    int64_t z, y;
-    g.row_of(r, c, z, y);
-    return static_cast<const T*>(in.get(c)) + z * in.sz + y * in.sy;
+    const int c = int(r / (g.nz * g.ny));
+    const int64_t rem = r % (g.nz * g.ny);
+    return static_cast<const T*>(in.get(c)) + (rem / g.ny) * in.sz + (rem % g.ny) * in.sy;
   };
   const T* pa = ra < rows ? row_ptr(ra) : nullptr;
   const T* pb = rb < rows ? row_ptr(rb) : nullptr;
@@ -229,15 +202,8 @@ __global__ void __launch_bounds__((RowTile<T, L>::NT)) k1_xr2c(SlabGeom g, CompP
   for (int k = t; k <= g.nx; k += TT) {
     const Cplx<T> zk = line[k];
     const Cplx<T> zm = line[(L - k) % L];
-    const Cplx<T> a = {T(0.5) * (zk.re + zm.re), T(0.5) * (zk.im - zm.im)};
-    const Cplx<T> b = {T(0.5) * (zk.im + zm.im), T(0.5) * (zm.re - zk.re)};
-    if constexpr (NT) {  // the next pass reads the slab back from HBM
-      if (ha) store_stream(ha + k, a);
-      if (hb) store_stream(hb + k, b);
-    } else {  // a z chunk: the y pass reads it next, from the cache
-      if (ha) ha[k] = a;
-      if (hb) hb[k] = b;
-    }
+    if (ha) store_stream(ha + k, Cplx<T>{T(0.5) * (zk.re + zm.re), T(0.5) * (zk.im - zm.im)});
+    if (hb) store_stream(hb + k, Cplx<T>{T(0.5) * (zk.im + zm.im), T(0.5) * (zm.re - zk.re)});
   }
 }
 
@@ -254,10 +220,10 @@ __global__ void __launch_bounds__((ColTile<T, L>::NT)) k_ycols(SlabGeom g, Cplx<
   const int lw = tid % WC, lr = tid / WC;  // load / store role
   const int team = tid / TT, t = tid % TT;  // transform role
   const int64_t ncols = g.nx + 1;
-  int64_t cz, kx0;  // cz = c * zn + (z - z0)
-  if (!col_tile(blockIdx.x, ncols, int64_t(g.ncomp) * g.zn, WC, cz, kx0, g.l0, g.nl)) return;
-  const int c = int(cz / g.zn);
-  const int64_t z = g.z0 + cz % g.zn;
+  int64_t cz, kx0;  // cz = c * nz + z
+  if (!col_tile(blockIdx.x, ncols, int64_t(g.ncomp) * g.nz, WC, cz, kx0)) return;
+  const int c = int(cz / g.nz);
+  const int64_t z = cz % g.nz;
   const int64_t kx = kx0 + lw;
   const bool live = kx < ncols;
   // a wavefront whose teams all have kx >= ncols skips its transforms
@@ -346,7 +312,7 @@ __global__ void __launch_bounds__((ColTile<T, L>::NT)) __attribute__((amdgpu_wav
   const int team = tid / TT, t = tid % TT;
   const int64_t ncols = g.nx + 1;
   int64_t kyo, kx0;  // kyo = 0..ny
-  if (!col_tile(blockIdx.x, ncols, g.ny + 1, WC, kyo, kx0, g.l0, g.nl)) return;
+  if (!col_tile(blockIdx.x, ncols, g.ny + 1, WC, kyo, kx0)) return;
   // jobs (component, ky or 2ny - ky) of this workgroup: chunk blockIdx.y of kZconvJobs
   const int alljobs = (kyo == 0 || kyo == g.ny) ? g.ncomp : 2 * g.ncomp;
   const int jbeg = int(blockIdx.y) * kZconvJobs;
@@ -465,7 +431,7 @@ __global__ void __launch_bounds__((RowTile<T, L>::NT)) k5_xc2r(SlabGeom g, const
   Cplx<T>* xb = reinterpret_cast<Cplx<T>*>(smem_raw);
   const int t = threadIdx.x % TT;
   const int team = threadIdx.x / TT;
-  const int64_t rows = int64_t(g.ncomp) * g.zn * g.ny;
+  const int64_t rows = int64_t(g.ncomp) * g.nz * g.ny;
   const int64_t q = int64_t(blockIdx.x) * RT::NTEAM + team;
   const int64_t ra = 2 * q, rb = 2 * q + 1;
   const int64_t nx = g.nx;
@@ -495,10 +461,9 @@ __global__ void __launch_bounds__((RowTile<T, L>::NT)) k5_xc2r(SlabGeom g, const
   __syncthreads();  // the twiddle table is complete
   team_fft<T, L, true>(v, line, twl, t);
   auto row_ptr = [&](int64_t r) -> T* {
-    int c;
-    int64_t z, y;
-    g.row_of(r, c, z, y);
-    return static_cast<T*>(out.get(c)) + z * out.sz + y * out.sy;
+    const int c = int(r / (g.nz * g.ny));
+    const int64_t rem = r % (g.nz * g.ny);
+    return static_cast<T*>(out.get(c)) + (rem / g.ny) * out.sz + (rem % g.ny) * out.sy;
   };
   T* pa = ra < rows ? row_ptr(ra) : nullptr;
   T* pb = rb < rows ? row_ptr(rb) : nullptr;
@@ -572,17 +537,13 @@ bool length_supported(int64_t L) {
 }
 
 template <typename T>
-int launch_k1(int64_t L, SlabGeom g, CompPtrs in, Cplx<T>* H, const Cplx<T>* tw, hipStream_t s,
-              bool nt = true) {
-  const int64_t rows = int64_t(g.ncomp) * g.zn * g.ny;
+int launch_k1(int64_t L, SlabGeom g, CompPtrs in, Cplx<T>* H, const Cplx<T>* tw, hipStream_t s) {
+  const int64_t rows = int64_t(g.ncomp) * g.nz * g.ny;
   const int64_t pairs = (rows + 1) / 2;
 #define X(n)                                                                          \
   if (L == n) {                                                                       \
     using RT = RowTile<T, n>;                                                         \
-    if (nt)                                                                           \
-      k1_xr2c<T, n, true><<<ceil_div(pairs, RT::NTEAM), RT::NT, RT::lds + n * sizeof(Cplx<T>), s>>>(g, in, H, tw); \
-    else                                                                              \
-      k1_xr2c<T, n, false><<<ceil_div(pairs, RT::NTEAM), RT::NT, RT::lds + n * sizeof(Cplx<T>), s>>>(g, in, H, tw); \
+    k1_xr2c<T, n><<<ceil_div(pairs, RT::NTEAM), RT::NT, RT::lds + n * sizeof(Cplx<T>), s>>>(g, in, H, tw); \
     SOPHT_CHECK_LAUNCH();                                                             \
     return SOPHT_OK;                                                                  \
   }
@@ -596,7 +557,7 @@ int launch_ycols(int64_t L, SlabGeom g, Cplx<T>* H, const Cplx<T>* tw, hipStream
 #define X(n)                                                                          \
   if (L == n) {                                                                       \
     using CT = ColTile<T, n>;                                                         \
-    const int64_t nb = col_tile_blocks(g.nx + 1, int64_t(g.ncomp) * g.zn, CT::WC, g.nl); \
+    const int64_t nb = col_tile_blocks(g.nx + 1, int64_t(g.ncomp) * g.nz, CT::WC);    \
     k_ycols<T, n, INV><<<nb, CT::NT, CT::lds + n * sizeof(Cplx<T>), s>>>(g, H, tw);   \
     SOPHT_CHECK_LAUNCH();                                                             \
     return SOPHT_OK;                                                                  \
@@ -612,7 +573,7 @@ int launch_k3(int64_t L, SlabGeom g, Cplx<T>* H, const T* ghat, const Cplx<T>* t
 #define X(n)                                                                          \
   if (L == n) {                                                                       \
     using CT = ColTile<T, n>;                                                         \
-    const int64_t nb = col_tile_blocks(g.nx + 1, g.ny + 1, CT::WC, g.nl);             \
+    const int64_t nb = col_tile_blocks(g.nx + 1, g.ny + 1, CT::WC);                   \
     const unsigned nch = unsigned((2 * g.ncomp + kZconvJobs - 1) / kZconvJobs);        \
     k3_zconv<T, n><<<dim3(unsigned(nb), nch), CT::NT, CT::lds, s>>>(g, H, ghat, tw);   \
     SOPHT_CHECK_LAUNCH();                                                             \
@@ -626,7 +587,7 @@ int launch_k3(int64_t L, SlabGeom g, Cplx<T>* H, const T* ghat, const Cplx<T>* t
 template <typename T>
 int launch_k5(int64_t L, SlabGeom g, const Cplx<T>* H, CompPtrsW out, const Cplx<T>* tw,
               hipStream_t s, bool acc) {
-  const int64_t rows = int64_t(g.ncomp) * g.zn * g.ny;
+  const int64_t rows = int64_t(g.ncomp) * g.nz * g.ny;
   const int64_t pairs = (rows + 1) / 2;
 #define X(n)                                                                          \
   if (L == n) {                                                                       \
@@ -646,7 +607,7 @@ int launch_k5(int64_t L, SlabGeom g, const Cplx<T>* H, CompPtrsW out, const Cplx
 template <typename T>
 SlabGeom slab_geom(const sopht_poisson3d* P, int ncomp) {
   return SlabGeom{P->nz, P->ny, P->nx, kRowPad ? round_up(P->nx + 1, kLineCplx<T>) : P->nx + 1,
-                  P->gz, ncomp, 0, P->nz, 0, 0};
+                  P->gz, ncomp};
 }
 
 // ncomp <= 3 components by value (sol, rhs), or any number from device pointer tables
@@ -679,85 +640,17 @@ int solve_impl(sopht_poisson3d* P, int ncomp, void* const sol[3], const int64_t*
   auto mark = [&](int i) {
     if (timer) timer->mark(first_stage + i, s);
   };
-  if (P->chunk_mode == 2 && !sol_tab && !rhs_tab) {
-    // kx chunks: the y pass of chunk j on the second stream, its z pass and inverse y pass on
-    // s once it is done; the y pass of chunk j + 1 waits for the z pass of chunk j - 1, so at
-    // most two chunks of the expanded slab are live in the cache
-    hipStream_t a = P->aux;
-    if ((rc = launch_k1<T>(2 * P->nx, g, in, H, twx, s))) return rc;
-    mark(0);
-    SOPHT_CHECK_HIP(hipEventRecord(P->ev[2 * 64], s));
-    SOPHT_CHECK_HIP(hipStreamWaitEvent(a, P->ev[2 * 64], 0));
-    const int nch = P->chunks;
-    for (int j = 0; j < nch; ++j) {
-      SlabGeom c = g;
-      c.l0 = int64_t(j) * P->lchunk;
-      c.nl = std::min<int64_t>(P->lchunk, P->lines - c.l0);
-      if (j >= 2) SOPHT_CHECK_HIP(hipStreamWaitEvent(a, P->ev[64 + (j - 2) % 64], 0));
-      if ((rc = launch_ycols<T, false>(2 * P->ny, c, H, twy, a))) return rc;
-      SOPHT_CHECK_HIP(hipEventRecord(P->ev[j % 64], a));
-      SOPHT_CHECK_HIP(hipStreamWaitEvent(s, P->ev[j % 64], 0));
-      if ((rc = launch_k3<T>(2 * P->nz, c, H, static_cast<const T*>(P->ghat), twz, s))) return rc;
-      SOPHT_CHECK_HIP(hipEventRecord(P->ev[64 + j % 64], s));
-      if ((rc = launch_ycols<T, true>(2 * P->ny, c, H, twy, s))) return rc;
-    }
-    // the y, z and inverse y passes overlap: their time is marked as the z pass
-    mark(1);
-    mark(2);
-    mark(3);
-    rc = launch_k5<T>(2 * P->nx, g, H, out, twx, s, accumulate);
-    mark(4);
-    return rc;
-  }
-  if (P->chunk_mode != 1 || P->chunks <= 1 || sol_tab || rhs_tab) {
-    if ((rc = launch_k1<T>(2 * P->nx, g, in, H, twx, s))) return rc;
-    mark(0);
-    if ((rc = launch_ycols<T, false>(2 * P->ny, g, H, twy, s))) return rc;
-    mark(1);
-    if ((rc = launch_k3<T>(2 * P->nz, g, H, static_cast<const T*>(P->ghat), twz, s))) return rc;
-    mark(2);
-    if ((rc = launch_ycols<T, true>(2 * P->ny, g, H, twy, s))) return rc;
-    mark(3);
-    rc = launch_k5<T>(2 * P->nx, g, H, out, twx, s, accumulate);
-    mark(4);
-    return rc;
-  }
-  // z chunks: the x pass of chunk i on s, its y pass on the second stream once the x pass is
-  // done (event i); s joins the second stream before the z pass and at the end. The stage
-  // marks on s then time the x passes, and the y passes' remainder.
-  hipStream_t a = P->aux;
-  const int nch = P->chunks;
-  auto chunk = [&](int i) {
-    SlabGeom c = g;
-    c.z0 = int64_t(i) * P->zchunk;
-    c.zn = std::min<int64_t>(P->zchunk, P->nz - c.z0);
-    return c;
-  };
-  for (int i = 0; i < nch; ++i) {
-    const SlabGeom c = chunk(i);
-    if ((rc = launch_k1<T>(2 * P->nx, c, in, H, twx, s, false))) return rc;
-    SOPHT_CHECK_HIP(hipEventRecord(P->ev[i], s));
-    SOPHT_CHECK_HIP(hipStreamWaitEvent(a, P->ev[i], 0));
-    if ((rc = launch_ycols<T, false>(2 * P->ny, c, H, twy, a))) return rc;
-  }
+  if ((rc = launch_k1<T>(2 * P->nx, g, in, H, twx, s))) return rc;
   mark(0);
-  SOPHT_CHECK_HIP(hipEventRecord(P->ev[2 * 64], a));
-  SOPHT_CHECK_HIP(hipStreamWaitEvent(s, P->ev[2 * 64], 0));
+  if ((rc = launch_ycols<T, false>(2 * P->ny, g, H, twy, s))) return rc;
   mark(1);
   if ((rc = launch_k3<T>(2 * P->nz, g, H, static_cast<const T*>(P->ghat), twz, s))) return rc;
   mark(2);
-  for (int i = 0; i < nch; ++i) {
-    const SlabGeom c = chunk(i);
-    if ((rc = launch_ycols<T, true>(2 * P->ny, c, H, twy, s))) return rc;
-    SOPHT_CHECK_HIP(hipEventRecord(P->ev[64 + i], s));
-    SOPHT_CHECK_HIP(hipStreamWaitEvent(a, P->ev[64 + i], 0));
-    if ((rc = launch_k5<T>(2 * P->nx, c, H, out, twx, a, accumulate))) return rc;
-  }
+  if ((rc = launch_ycols<T, true>(2 * P->ny, g, H, twy, s))) return rc;
   mark(3);
-  SOPHT_CHECK_HIP(hipEventRecord(P->ev[2 * 64 + 1], a));
-  SOPHT_CHECK_HIP(hipStreamWaitEvent(s, P->ev[2 * 64 + 1], 0));
+  rc = launch_k5<T>(2 * P->nx, g, H, out, twx, s, accumulate);
   mark(4);
-  return SOPHT_OK;
+  return rc;
 }
 
 // exp(-2 pi i m / L) in FP64 with exact argument reduction to the first octant.
@@ -918,54 +811,6 @@ int poisson_create(int dtype, int64_t nz, int64_t ny, int64_t nx, double x_range
     if (rc) return fail(rc);
   }
   if ((rc = build_ghat(P, s))) return fail(rc);
-  // Chunked passes when a pass's output would not stay in the 256 MiB Infinity Cache for the
-  // next one (single-field plans only; batched plans use tables). SOPHT_POISSON_CHUNK = none,
-  // z (x and y passes by z chunks) or kx (y, z and inverse y passes by kx chunks).
-  {
-    const int64_t half = int64_t(comps) * nz * ny * rs * int64_t(cs);
-    P->chunks = 1;
-    P->zchunk = nz;
-    P->chunk_mode = 0;
-    const char* env = std::getenv("SOPHT_POISSON_CHUNK");
-    int mode = kDefaultChunkMode;
-    if (env && std::strcmp(env, "none") == 0) mode = 0;
-    if (env && std::strcmp(env, "z") == 0) mode = 1;
-    if (env && std::strcmp(env, "kx") == 0) mode = 2;
-    // the thresholds can be lowered (tests exercise the chunked passes on small grids)
-    int64_t min_bytes = kChunkMinBytes, chunk_bytes = kChunkBytes;
-    if (const char* e = std::getenv("SOPHT_POISSON_CHUNK_MIN_BYTES")) min_bytes = std::atoll(e);
-    if (const char* e = std::getenv("SOPHT_POISSON_CHUNK_BYTES")) chunk_bytes = std::max<int64_t>(1, std::atoll(e));
-    if (comps <= 3 && half > min_bytes && mode == 1) {
-      const int64_t want = std::min<int64_t>(64, (half + chunk_bytes - 1) / chunk_bytes);
-      P->zchunk = (nz + want - 1) / want;
-      P->chunks = int((nz + P->zchunk - 1) / P->zchunk);
-      P->chunk_mode = P->chunks > 1 ? 1 : 0;
-    }
-    if (comps <= 3 && 2 * half > min_bytes && mode == 2) {
-      const int64_t line_cplx = 2 * (kColHalf ? (128 / int64_t(cs)) / 2 : 128 / int64_t(cs));
-      P->lines = (nx + 1 + line_cplx - 1) / line_cplx;
-      const int64_t line_bytes = int64_t(comps) * nz * 2 * ny * line_cplx * int64_t(cs);
-      P->lchunk = std::max<int64_t>(1, chunk_bytes / line_bytes);
-      P->chunks = int((P->lines + P->lchunk - 1) / P->lchunk);
-      P->chunk_mode = P->chunks > 1 ? 2 : 0;
-      if (P->chunks > 64) {  // the event ring holds 64
-        P->lchunk = (P->lines + 63) / 64;
-        P->chunks = int((P->lines + P->lchunk - 1) / P->lchunk);
-      }
-    }
-    if (P->chunks > 1) {
-      if (hipStreamCreateWithFlags(&P->aux, hipStreamNonBlocking) != hipSuccess) {
-        P->aux = nullptr;
-        set_last_error("hipStreamCreate of the Poisson plan's second stream failed", __FILE__, __LINE__);
-        return fail(SOPHT_ERR_HIP);
-      }
-      for (int i = 0; i < 2 * 64 + 2; ++i)
-        if (hipEventCreateWithFlags(&P->ev[i], hipEventDisableTiming) != hipSuccess) {
-          set_last_error("hipEventCreate of the Poisson plan failed", __FILE__, __LINE__);
-          return fail(SOPHT_ERR_HIP);
-        }
-    }
-  }
   *out = P;
   return SOPHT_OK;
 }
@@ -976,12 +821,6 @@ extern "C" {
 
 int sopht_poisson3d_destroy(sopht_poisson3d* P) {
   if (!P) return SOPHT_OK;
-  if (P->aux) {
-    (void)hipStreamSynchronize(P->aux);
-    for (int i = 0; i < 2 * 64 + 2; ++i)
-      if (P->ev[i]) (void)hipEventDestroy(P->ev[i]);
-    (void)hipStreamDestroy(P->aux);
-  }
   if (P->slab) (void)hipFree(P->slab);
   if (P->ghat) (void)hipFree(P->ghat);
   for (int a = 0; a < 3; ++a)

@@ -101,61 +101,6 @@ def test_linearity_at_256(dev):
     assert err.item() < 1e-5
 
 
-@pytest.mark.parametrize("mode", ["z", "kx"])
-@pytest.mark.parametrize("precision", ["single", "double"])
-@pytest.mark.parametrize("shape", [(24, 24, 24), (20, 12, 40), (48, 8, 8)])
-def test_chunked_passes_bit_identical(dev, mode, precision, shape, monkeypatch, rng):
-    """The chunked pass orders (x and y passes by z chunks, or the y, z and inverse y passes
-    by kx chunks, on two streams) do the same arithmetic per element: bit-identical to the
-    unchunked passes. The chunk thresholds are lowered so small grids split into several
-    chunks (down to one plane or one kx line per chunk)."""
-    import torch
-
-    real_t = PREC[precision]
-    rhs = to_dev(rng.standard_normal((3, *shape)).astype(real_t), dev)
-    outs = {}
-    for m in ("none", mode):
-        monkeypatch.setenv("SOPHT_POISSON_CHUNK", m)
-        monkeypatch.setenv("SOPHT_POISSON_CHUNK_MIN_BYTES", "0")
-        monkeypatch.setenv("SOPHT_POISSON_CHUNK_BYTES", "1")
-        s = solver_cls()(*shape, x_range=1.0, real_t=real_t)
-        out = torch.zeros_like(rhs)
-        for _ in range(2):  # a second solve reuses the plan's streams and events
-            s.vector_field_solve(out, rhs)
-        s1 = torch.zeros_like(rhs[0])
-        s.solve(s1, rhs[1])
-        torch.cuda.synchronize()
-        outs[m] = (out.clone(), s1.clone())
-        del s
-    assert torch.equal(outs["none"][0], outs[mode][0])
-    assert torch.equal(outs["none"][1], outs[mode][1])
-    assert torch.equal(outs[mode][1], outs[mode][0][1])
-
-
-@pytest.mark.parametrize("mode", ["z", "kx"])
-def test_chunked_passes_in_the_step_and_adjoint(dev, mode, monkeypatch):
-    """The flow step, the velocity recovery and the adjoint rollout (whose Poisson passes
-    accumulate into the vorticity adjoint) with chunked passes: bit-identical to unchunked."""
-    import torch
-
-    from test_gpu_adjoint import build_rollout
-
-    results = {}
-    for m in ("none", mode):
-        monkeypatch.setenv("SOPHT_POISSON_CHUNK", m)
-        monkeypatch.setenv("SOPHT_POISSON_CHUNK_MIN_BYTES", "0")
-        monkeypatch.setenv("SOPHT_POISSON_CHUNK_BYTES", "4096")
-        ro, *_ = build_rollout((16, 16, 24), np.float64, 60, (0.2, 0.12, 0.12), 1e-2, 0.02, 6, 3)
-        theta = torch.tensor(np.random.default_rng(7).normal(0.0, 0.5, 16), device=dev, requires_grad=True)
-        from sopht_amd import simulator as S
-
-        forces = S.body_forces(theta, ro)
-        S.time_averaged_force(forces, ro.dts).backward()
-        results[m] = (forces.detach().clone(), theta.grad.clone(), ro.flow.vorticity_field.clone())
-    for a, b in zip(results["none"], results[mode]):
-        assert torch.equal(a, b)
-
-
 @pytest.mark.parametrize("precision", ["single", "double"])
 @pytest.mark.parametrize("shape", [(192, 6, 8), (8, 192, 6), (6, 8, 192), (4, 6, 384), (384, 4, 6)])
 def test_solve_on_384_and_768_point_lines(dev, precision, shape, rng):
