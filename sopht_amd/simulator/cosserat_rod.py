@@ -320,14 +320,22 @@ class CosseratRodFlowInteraction(ImmersedBodyFlowInteraction):
         )
 
 
-class FlowForces:
+try:  # a pyelastica forcing must derive from NoForces; without pyelastica, a plain base
+    from elastica import NoForces as _ForcingBase
+except ImportError:  # pragma: no cover - pyelastica is optional
+    _ForcingBase = object
+
+
+class FlowForces(_ForcingBase):
     """The flow's forcing on a body, as a pyelastica forcing (``flow_forces.py:12-25``):
     each call evaluates the interactor's forces and torques on the body (interpolation and
     Lagrangian forcing on the device, no spreading) and adds them to the system's external
-    loads. It has the ``apply_forces`` / ``apply_torques`` interface of pyelastica's
-    ``NoForces``, which is its base class there."""
+    loads. With pyelastica installed it derives from ``NoForces``, so
+    ``add_forcing_to(rod).using(FlowForces, interactor)`` accepts it."""
 
     def __init__(self, body_flow_interactor):
+        if _ForcingBase is not object:
+            super().__init__()
         self.body_flow_interactor = body_flow_interactor
 
     def apply_forces(self, system, time=0.0):
