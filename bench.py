@@ -63,6 +63,9 @@ def stage_words(nz, ny, nx):
         "poisson_z_conv": 3 * (4 * c + 4 * c) + ghat,  # z < nz in and out, G^ octant
         "poisson_y_inv": 3 * (4 * c + 2 * c),
         "poisson_x_c2r": 3 * (2 * c + 1),
+        # three-pass schedule (nx = ny): x rows with the inner y sums, and their inverse
+        "poisson_xy_fwd": 3 * (1 + 4 * c),     # real rows in, all 2ny rows of the slab out
+        "poisson_yx_inv": 3 * (4 * c + 1),
         "velocity_curl": 3 + 3,                # read psi; write u
     }
 
@@ -382,7 +385,7 @@ def plain_rate(n, real_t, steps=20, warmup=3):
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     nst = lib.sopht_ns3d_stage_times(flow._plan, ms, 16, C.byref(cnt), 0)
-    names = [lib.sopht_ns3d_stage_name(i).decode() for i in range(nst)]
+    names = [lib.sopht_ns3d_stage_name(flow._plan, i).decode() for i in range(nst)]
     avg = {names[i]: ms[i] / max(cnt.value, 1) for i in range(nst)}
     word = np.dtype(real_t).itemsize
     words = stage_words(n, n, n)
@@ -615,7 +618,7 @@ def main():
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     nst = lib.sopht_ns3d_stage_times(flow._plan, ms, 16, C.byref(cnt), 0)
-    names = [lib.sopht_ns3d_stage_name(i).decode() for i in range(nst)]
+    names = [lib.sopht_ns3d_stage_name(flow._plan, i).decode() for i in range(nst)]
     avg_ms = {names[i]: ms[i] / max(cnt.value, 1) for i in range(nst)}
     inter.check_support_in_domain()
     if not torch.isfinite(flow.vorticity_field).all():

@@ -349,10 +349,11 @@ int sopht_ns3d_last_velocity_max(sopht_ns3d* plan, const void* velocity, void* o
  * counterpart in the reference, which has no profiling hooks). enable_timing creates the
  * events; stage_times harvests (synchronising on the last recorded step), writes the
  * accumulated milliseconds of up to n stages into ms, the number of timed steps into
- * *steps, optionally resets, and returns the stage count. stage_name names stage i. */
+ * *steps, optionally resets, and returns the stage count. stage_name names stage i of the
+ * plan (its Poisson solver sets how many stages a step has). */
 int sopht_ns3d_enable_timing(sopht_ns3d* plan, int enable);
 int sopht_ns3d_stage_times(sopht_ns3d* plan, double* ms, int n, int64_t* steps, int reset);
-const char* sopht_ns3d_stage_name(int stage);
+const char* sopht_ns3d_stage_name(const sopht_ns3d* plan, int stage);
 
 /* ---- batched step (no counterpart in the reference: many independent simulators, the
  * reference driver's loop body flow_past_sphere_case.py:191-197 per member, advanced together

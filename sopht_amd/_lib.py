@@ -111,7 +111,7 @@ SIGNATURES = {
     "sopht_laplacian_filter_3d": [_i, _i, _i, _vp, _pi64, _i64, _i64, _i64, _vp, _pi64, _vp, _pi64, _i, _vp],
     "sopht_ns3d_enable_timing": [_vp, _i],
     "sopht_ns3d_stage_times": [_vp, _pd, _i, _pi64, _i],
-    "sopht_ns3d_stage_name": [_i],
+    "sopht_ns3d_stage_name": [_vp, _i],
     "sopht_ns3d_batch_create": [_i, _i, C.POINTER(BatchMember), _vp, C.POINTER(_vp)],
     "sopht_ns3d_batch_step": [_vp, _i, _i, _vp],
     "sopht_ns3d_batch_state": [_vp, _pd, _pd, _vp],

@@ -169,6 +169,8 @@ int fused_vorticity_filter(int dtype, int order, int type, void* w, void* buf, i
 int fastdiag_vector_solve_timed(sopht_fastdiag3d* P, void* const sol[3], const int64_t* ss,
                                 const void* const rhs[3], const int64_t* rs, hipStream_t s,
                                 StageTimer* tm, int first);
+// Launches (timed stages) of one Poisson solve: 3 (x and y passes fused) or 5.
+int poisson_stage_count(const sopht_poisson3d* P);
 // As the vector solve, adding the solution into sol (sol += G rhs).
 int poisson_vector_solve_accumulate(sopht_poisson3d* P, void* const sol[3], const int64_t* ss,
                                     const void* const rhs[3], const int64_t* rs, hipStream_t s);

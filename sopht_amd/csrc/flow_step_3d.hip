@@ -18,18 +18,26 @@
 #include "ns3d_plan.h"
 #include "sweeps_3d.h"
 
-// Stage order of one step, for sopht_ns3d_stage_times.
-static const char* const kStageNames[] = {
-    "vorticity_forcing_advect",    // S1
-    "vorticity_diffuse_penalise",  // SC
-    "poisson_x_r2c",            // K1
-    "poisson_y_fwd",            // K2
-    "poisson_z_conv",           // K3
-    "poisson_y_inv",            // K4
-    "poisson_x_c2r",            // K5
-    "velocity_curl",            // S2
-};
-static constexpr int kNumStages = 8;
+// Stage order of one step, for sopht_ns3d_stage_times: the two vorticity sweeps, the
+// Poisson solver's launches (five or three passes, or the fast-diagonalisation stages), the
+// velocity sweep.
+static const char* const kStagesFivePass[] = {
+    "vorticity_forcing_advect", "vorticity_diffuse_penalise", "poisson_x_r2c", "poisson_y_fwd",
+    "poisson_z_conv", "poisson_y_inv", "poisson_x_c2r", "velocity_curl"};
+static const char* const kStagesThreePass[] = {
+    "vorticity_forcing_advect", "vorticity_diffuse_penalise", "poisson_xy_fwd", "poisson_z_conv",
+    "poisson_yx_inv", "velocity_curl"};
+static const char* const kStagesFastdiag[] = {
+    "vorticity_forcing_advect", "vorticity_diffuse_penalise", "fastdiag_z_fwd", "fastdiag_y_fwd",
+    "fastdiag_x_fwd_scale", "fastdiag_z_bwd", "fastdiag_yx_bwd", "velocity_curl"};
+
+static int stage_count(const sopht_ns3d* P) {
+  return P->fastdiag ? 8 : 3 + sopht::poisson_stage_count(P->poisson);
+}
+static const char* const* stage_names(const sopht_ns3d* P) {
+  if (P->fastdiag) return kStagesFastdiag;
+  return sopht::poisson_stage_count(P->poisson) == 3 ? kStagesThreePass : kStagesFivePass;
+}
 
 namespace sopht {
 namespace {
@@ -121,7 +129,7 @@ int step_impl(sopht_ns3d* P, const void* vort_in, void* vort_out, const void* ve
   SOPHT_CHECK_LAUNCH();
   P->s2_partials = fused_max ? int64_t(ml.blocks) : 0;
   P->s2_velocity = u;
-  tm.mark(7, s);
+  tm.mark(stage_count(P) - 1, s);
   tm.end();
   return SOPHT_OK;
 }
@@ -237,7 +245,7 @@ int sopht_ns3d_create_with_poisson_solver(int dtype, int64_t nz, int64_t ny, int
 int sopht_ns3d_enable_timing(sopht_ns3d* P, int enable) {
   SOPHT_REQUIRE(P != nullptr, "plan is null");
   if (enable) {
-    SOPHT_CHECK_HIP(P->timer.enable(kNumStages) == SOPHT_OK ? hipSuccess : hipErrorUnknown);
+    SOPHT_CHECK_HIP(P->timer.enable(stage_count(P)) == SOPHT_OK ? hipSuccess : hipErrorUnknown);
   } else {
     P->timer.disable();
   }
@@ -248,17 +256,18 @@ int sopht_ns3d_stage_times(sopht_ns3d* P, double* ms, int n, int64_t* steps, int
   SOPHT_REQUIRE(P != nullptr, "plan is null");
   P->timer.harvest(0);
   P->timer.harvest(1);
-  for (int i = 0; i < n && i < kNumStages; ++i) ms[i] = P->timer.acc[i];
+  const int ns = stage_count(P);
+  for (int i = 0; i < n && i < ns; ++i) ms[i] = P->timer.acc[i];
   if (steps) *steps = P->timer.steps;
   if (reset) {
     for (double& a : P->timer.acc) a = 0.0;
     P->timer.steps = 0;
   }
-  return kNumStages;
+  return ns;
 }
 
-const char* sopht_ns3d_stage_name(int i) {
-  return (i >= 0 && i < kNumStages) ? kStageNames[i] : "";
+const char* sopht_ns3d_stage_name(const sopht_ns3d* P, int i) {
+  return (P && i >= 0 && i < stage_count(P)) ? stage_names(P)[i] : "";
 }
 
 int sopht_ns3d_destroy(sopht_ns3d* P) {

@@ -33,6 +33,7 @@ struct sopht_poisson3d {
   int64_t gz;       // G^ line stride (elements): nz + 1 rounded up to a 128-byte line
   int slab_comps;   // components the slab holds (3, or 3 per instance of a batched class)
   int64_t slab_bytes, ghat_bytes;
+  int passes;       // 3: x and y passes fused (three-pass schedule), or 5
 };
 
 namespace sopht {
@@ -85,6 +86,24 @@ __device__ __forceinline__ bool col_tile(int64_t b, int64_t ncols, int64_t nrows
   row = b / tiles;
   kx0 = (b % tiles) * wc;
   return row < nrows;
+}
+
+// The same with a line split in PARTS tiles: the parts of a line go to workgroups
+// b + 8 p (p < PARTS), which the dispatcher sends to one XCD.
+template <int PARTS>
+__device__ __forceinline__ bool col_tile_parts(int64_t b, int64_t ncols, int64_t nrows, int wc,
+                                               int64_t& row, int64_t& kx0) {
+  const int64_t lines = (ncols + PARTS * wc - 1) / (PARTS * wc);
+  const int64_t line = (b / (8 * PARTS)) * 8 + b % 8;
+  const int64_t part = (b / 8) % PARTS;
+  row = line / lines;
+  kx0 = ((line % lines) * PARTS + part) * wc;
+  return row < nrows;
+}
+
+inline int64_t col_tile_parts_blocks(int parts, int64_t ncols, int64_t nrows, int wc) {
+  const int64_t lines = (ncols + parts * wc - 1) / (parts * wc);
+  return 8 * parts * ((lines * nrows + 7) / 8);
 }
 
 inline int64_t col_tile_blocks(int64_t ncols, int64_t nrows, int wc) {
@@ -480,6 +499,498 @@ __global__ void __launch_bounds__((RowTile<T, L>::NT)) k5_xc2r(SlabGeom g, const
   });
 }
 
+// ------------------------------------------------------------------ three-pass schedule
+// The y transform (length LY = 2 ny = 4 L2) is split by decimation in time, y = 4 y2 + y1 and
+// ky = k2 + L2 k1:
+//   Y[k2 + L2 k1] = sum_{y1<4} w4^{y1 k1} * [ w_LY^{y1 k2} * sum_{y2} x[4 y2 + y1] w_L2^{y2 k2} ]
+// and its inverse the same way with conjugate roots. Pass A (ka_xy_fwd) runs per (c, z, y1):
+// the x transforms of the rows y = 4 y2 + y1, then the inner L2-point sums over y2 and the
+// twiddle, stored in slab row y1 L2 + k2. The z pass (k3_zconv4) applies the 4-point sums
+// over y1 before its convolution and their inverse after it. Pass C (kc_yx_inv) undoes the
+// twiddle and the inner sums and runs the inverse x transforms. So the x-pass outputs
+// (K1 -> K2) and the inverse y-pass outputs (K4 -> K5) of the five-pass schedule never reach
+// HBM: 24 words per cell fewer.
+//
+// A workgroup of pass A or C holds the ny / 4 rows of its (c, z, y1) as row pairs (two real
+// rows per complex x transform), PPT pairs per team of TX threads. The y sums run on chunks of
+// TX columns (kx = TX ch + t, the ch-th element of every x thread; the last chunk is kx = nx
+// alone), one team of T2 threads per column, staged through LDS.
+template <typename T, int LX, int LY>
+struct XYPass {
+  static constexpr int EX = LinePlan<LX>::E, TX = LinePlan<LX>::T;
+  static constexpr int L2 = LY / 4;
+  static constexpr int E2 = LinePlan<L2>::E, T2 = LinePlan<L2>::T;
+  static constexpr int NY2 = L2 / 2;      // rows of a workgroup (y2 < ny / 4)
+  static constexpr int NPAIR = NY2 / 2;
+  static constexpr int NTEAM = NPAIR * TX <= 1024 ? NPAIR : 1024 / TX;
+  static constexpr int PPT = NPAIR / NTEAM;  // row pairs per x team
+  static constexpr int NT = NTEAM * TX;
+  static constexpr int CW = TX;              // columns per chunk
+  static constexpr int NCH = EX / 2 + 1;
+  static constexpr int SX = region_stride<LX, false>();
+  static constexpr int S2 = region_stride<L2, true>();
+  // staging tile [row][P]: the column teams' accesses (T2 rows of 32 / T2 columns per half
+  // wavefront) fall in distinct banks when P = 32 / T2 (mod 32)
+  static constexpr int P = (CW + 31) / 32 * 32 + 32 / T2;
+  static constexpr int XPH = NTEAM * SX + LX;  // x regions and the x twiddle table
+  static constexpr int CREG = CW * S2;         // column regions
+  static constexpr int STG = L2 * P;           // staging tile
+  static constexpr int UNION = XPH > CREG ? (XPH > STG ? XPH : STG) : (CREG > STG ? CREG : STG);
+  static constexpr size_t lds = size_t(2 * L2 + UNION) * sizeof(Cplx<T>);
+  // the staging tile moves to and from HBM as rows of CW columns: thread tid takes column
+  // tid % CW of the rows tid / CW + q NT / CW
+  static constexpr int RSTEP = NT / CW, NQ = L2 / RSTEP;
+  static_assert(LY % 8 == 0 && NPAIR % NTEAM == 0 && NTEAM >= T2 && T2 <= 32, "three-pass geometry");
+  static_assert(NT % CW == 0 && L2 % RSTEP == 0, "whole staging rows per thread");
+};
+
+// Pass A: x real-to-complex of the rows y = 4 y2 + y1 of (c, z) (two rows per complex FFT,
+// separated as in k1_xr2c), then per column kx the zero-padded L2-point FFT over y2 times
+// w_LY^{y1 k2}, stored in slab row y1 L2 + k2.
+template <typename T, int LX, int LY>
+__global__ void __launch_bounds__((XYPass<T, LX, LY>::NT)) ka_xy_fwd(SlabGeom g, CompPtrs in, Cplx<T>* H,
+                                                                    const Cplx<T>* twx, const Cplx<T>* twy) {
+  using F = XYPass<T, LX, LY>;
+  constexpr int EX = F::EX, TX = F::TX, L2 = F::L2, E2 = F::E2, T2 = F::T2, PPT = F::PPT, P = F::P;
+  constexpr int CW = F::CW, NT = F::NT;
+  extern __shared__ unsigned char smem_raw[];
+  Cplx<T>* const tw2 = reinterpret_cast<Cplx<T>*>(smem_raw);  // w_L2^m = w_LY^{4m}
+  Cplx<T>* const twr = tw2 + L2;                               // w_LY^{y1 k2}
+  Cplx<T>* const U = twr + L2;
+  const int tid = threadIdx.x;
+  const int y1 = int(blockIdx.x & 3);
+  const int cz = int(blockIdx.x >> 2);
+  const int c = cz / int(g.nz), z = cz % int(g.nz);
+  const int team = tid / TX, t = tid % TX;
+  for (int i = tid; i < L2; i += NT) {
+    tw2[i] = twy[4 * i];
+    twr[i] = twy[(y1 * i) % LY];
+  }
+  Cplx<T>* const twxl = U + F::NTEAM * F::SX;
+  for (int i = tid; i < LX; i += NT) twxl[i] = twx[i];
+  const T* base = static_cast<const T*>(in.get(c)) + int64_t(z) * in.sz;
+  Cplx<T> v[PPT][EX];
+  static_for<0, PPT>([&](auto p_) {
+    constexpr int p = decltype(p_)::value;
+    const int y2 = 2 * (team * PPT + p);
+    const T* ra = base + int64_t(4 * y2 + y1) * in.sy + int64_t(t) * in.sx;
+    const T* rb = ra + 4 * in.sy;
+    static_for<0, EX>([&](auto e_) {
+      constexpr int e = decltype(e_)::value;
+      if constexpr (e < EX / 2) {
+        v[p][e] = Cplx<T>{ra[int64_t(TX * e) * in.sx], rb[int64_t(TX * e) * in.sx]};
+      } else {
+        v[p][e] = Cplx<T>{T(0), T(0)};
+      }
+    });
+  });
+  __syncthreads();  // tables complete
+  Cplx<T>* const xline = U + team * F::SX;
+  // half spectra kx = TX e + t (e <= EX / 2; e = EX / 2 is kx = nx, lane 0 only)
+  Cplx<T> A[PPT][EX / 2 + 1], B[PPT][EX / 2 + 1];
+  const int mirror_lane = (TX - t) % TX;
+  static_for<0, PPT>([&](auto p_) {
+    constexpr int p = decltype(p_)::value;
+    rfft::team_fft<T, LX, false, true>(v[p], xline, twxl, t);
+    static_for<0, EX / 2 + 1>([&](auto e_) {
+      constexpr int e = decltype(e_)::value;
+      // Z[LX - k], k = TX e + t: lane TX - t, element EX - 1 - e (t > 0); lane 0: EX - e
+      const Cplx<T> zk = v[p][e];
+      Cplx<T> zm;
+      zm.re = __shfl(v[p][EX - 1 - e].re, mirror_lane, TX);
+      zm.im = __shfl(v[p][EX - 1 - e].im, mirror_lane, TX);
+      if (t == 0) zm = v[p][(EX - e) % EX];
+      A[p][e] = Cplx<T>{T(0.5) * (zk.re + zm.re), T(0.5) * (zk.im - zm.im)};
+      B[p][e] = Cplx<T>{T(0.5) * (zk.im + zm.im), T(0.5) * (zm.re - zk.re)};
+    });
+  });
+  __syncthreads();  // x regions free: the chunk phase reuses the area
+  const int col = tid / T2, t2 = tid % T2;
+  const bool col_role = tid < CW * T2;
+  Cplx<T>* const creg = U + col * F::S2;
+  // staging rows to HBM: column sc of rows sr + q RSTEP
+  const int sc = tid % CW, sr = tid / CW;
+  Cplx<T>* const hrow = H + g.hi(c, z, y1 * L2 + sr, sc);
+  const int rstride = F::RSTEP * int(g.rs);
+  static_for<0, F::NCH>([&](auto ch_) {
+    constexpr int ch = decltype(ch_)::value;
+    constexpr bool last = ch == EX / 2;  // kx = nx alone
+    if (!last || t == 0) {
+      static_for<0, PPT>([&](auto p_) {
+        constexpr int p = decltype(p_)::value;
+        const int y2 = 2 * (team * PPT + p);
+        U[y2 * P + t] = A[p][ch];
+        U[(y2 + 1) * P + t] = B[p][ch];
+      });
+    }
+    __syncthreads();
+    const bool live = col_role && (!last || col == 0);
+    Cplx<T> w[E2];
+    if (live) {
+      static_for<0, E2>([&](auto e_) {
+        constexpr int e = decltype(e_)::value;
+        if constexpr (e < E2 / 2) w[e] = U[(t2 + T2 * e) * P + col];
+        else w[e] = Cplx<T>{T(0), T(0)};
+      });
+    }
+    __syncthreads();
+    if (live) {
+      rfft::team_fft<T, L2, false, true>(w, creg, tw2, t2);
+      static_for<0, E2>([&](auto e_) {
+        constexpr int e = decltype(e_)::value;
+        w[e] = cmul(w[e], twr[t2 + T2 * e]);
+      });
+    }
+    __syncthreads();
+    if (live) {
+      static_for<0, E2>([&](auto e_) {
+        constexpr int e = decltype(e_)::value;
+        U[(t2 + T2 * e) * P + col] = w[e];
+      });
+    }
+    __syncthreads();
+    if (!last || sc == 0) {
+      static_for<0, F::NQ>([&](auto q_) {
+        constexpr int q = decltype(q_)::value;
+        store_stream(hrow + q * rstride + TX * ch, U[(sr + q * F::RSTEP) * P + sc]);
+      });
+    }
+    __syncthreads();
+  });
+}
+
+// Pass C: per column kx, w_LY^{-y1 k2} times the slab rows y1 L2 + k2, the inverse L2-point
+// FFT over k2 (outputs y2 < ny / 4), then the Hermitian x half spectra of each row pair to one
+// complex inverse FFT (as in k5_xc2r), rows y = 4 y2 + y1 stored (ACC: added). The next
+// chunk's rows are loaded while the current one is transformed.
+template <typename T, int LX, int LY, bool ACC>
+__global__ void __launch_bounds__((XYPass<T, LX, LY>::NT)) kc_yx_inv(SlabGeom g, const Cplx<T>* H, CompPtrsW out,
+                                                                    const Cplx<T>* twx, const Cplx<T>* twy) {
+  using F = XYPass<T, LX, LY>;
+  constexpr int EX = F::EX, TX = F::TX, L2 = F::L2, E2 = F::E2, T2 = F::T2, PPT = F::PPT, P = F::P;
+  constexpr int CW = F::CW, NT = F::NT;
+  extern __shared__ unsigned char smem_raw[];
+  Cplx<T>* const tw2 = reinterpret_cast<Cplx<T>*>(smem_raw);
+  Cplx<T>* const twr = tw2 + L2;
+  Cplx<T>* const U = twr + L2;
+  const int tid = threadIdx.x;
+  const int y1 = int(blockIdx.x & 3);
+  const int cz = int(blockIdx.x >> 2);
+  const int c = cz / int(g.nz), z = cz % int(g.nz);
+  const int team = tid / TX, t = tid % TX;
+  for (int i = tid; i < L2; i += NT) {
+    tw2[i] = twy[4 * i];
+    twr[i] = twy[(y1 * i) % LY];
+  }
+  const int col = tid / T2, t2 = tid % T2;
+  const bool col_role = tid < CW * T2;
+  Cplx<T>* const creg = U + col * F::S2;
+  const int mirror_lane = (TX - t) % TX;
+  // the rows of a chunk: thread tid holds elements tid + NT q of the chunk's L2 x ncol block
+  // (addresses formed per load: a hoisted row pointer costs the registers of the prefetch)
+  Cplx<T> pre[F::NQ];
+  auto load_chunk = [&](auto ch_) {
+    constexpr int ch = decltype(ch_)::value;
+    constexpr int ncol = ch == EX / 2 ? 1 : CW;
+    static_for<0, F::NQ>([&](auto q_) {
+      constexpr int q = decltype(q_)::value;
+      const int i = tid + q * NT;
+      if (i < L2 * ncol) pre[q] = H[g.hi(c, z, y1 * L2 + i / ncol, TX * ch + i % ncol)];
+    });
+  };
+  load_chunk(std::integral_constant<int, 0>{});
+  Cplx<T> v[PPT][EX];
+  static_for<0, F::NCH>([&](auto ch_) {
+    constexpr int ch = decltype(ch_)::value;
+    constexpr bool last = ch == EX / 2;
+    constexpr int ncol = last ? 1 : CW;
+    static_for<0, F::NQ>([&](auto q_) {
+      constexpr int q = decltype(q_)::value;
+      const int i = tid + q * NT;
+      if (i < L2 * ncol) U[(i / ncol) * P + i % ncol] = pre[q];
+    });
+    if constexpr (ch + 1 < F::NCH) load_chunk(std::integral_constant<int, ch + 1>{});
+    __syncthreads();  // (first chunk: the tables too)
+    const bool live = col_role && (!last || col == 0);
+    Cplx<T> w[E2];
+    if (live) {
+      static_for<0, E2>([&](auto e_) {
+        constexpr int e = decltype(e_)::value;
+        w[e] = cmulc(U[(t2 + T2 * e) * P + col], twr[t2 + T2 * e]);
+      });
+    }
+    __syncthreads();
+    if (live) rfft::team_fft<T, L2, true>(w, creg, tw2, t2);
+    __syncthreads();
+    if (live) {
+      static_for<0, E2 / 2>([&](auto e_) {
+        constexpr int e = decltype(e_)::value;
+        U[(t2 + T2 * e) * P + col] = w[e];
+      });
+    }
+    __syncthreads();
+    static_for<0, PPT>([&](auto p_) {
+      constexpr int p = decltype(p_)::value;
+      const int y2 = 2 * (team * PPT + p);
+      Cplx<T> a = U[y2 * P + t], b = U[(y2 + 1) * P + t];
+      if ((ch == 0 && t == 0) || last) {  // kx = 0 and kx = nx: imaginary parts dropped
+        a.im = T(0);
+        b.im = T(0);
+      }
+      const Cplx<T> zk = {a.re - b.im, a.im + b.re};  // Z[k] = A + i B
+      const Cplx<T> zm = {a.re + b.im, b.re - a.im};  // Z[LX - k] = conj(A) + i conj(B)
+      if constexpr (last) {
+        if (t == 0) v[p][EX / 2] = zk;
+      } else {
+        Cplx<T> r;
+        r.re = __shfl(zm.re, mirror_lane, TX);
+        r.im = __shfl(zm.im, mirror_lane, TX);
+        v[p][ch] = zk;
+        // Z[LX - k'] of lane TX - t (k' = TX ch + TX - t) is element EX - 1 - ch of lane t;
+        // lane 0 keeps its own mirror, element EX - ch
+        if (t != 0) v[p][EX - 1 - ch] = r;
+        if constexpr (ch > 0) {
+          if (t == 0) v[p][EX - ch] = zm;
+        }
+      }
+    });
+    __syncthreads();
+  });
+  Cplx<T>* const twxl = U + F::NTEAM * F::SX;
+  for (int i = tid; i < LX; i += NT) twxl[i] = twx[i];
+  __syncthreads();
+  Cplx<T>* const xline = U + team * F::SX;
+  T* const base = static_cast<T*>(out.get(c)) + int64_t(z) * out.sz;
+  static_for<0, PPT>([&](auto p_) {
+    constexpr int p = decltype(p_)::value;
+    rfft::team_fft<T, LX, true>(v[p], xline, twxl, t);
+    const int y2 = 2 * (team * PPT + p);
+    T* const ra = base + int64_t(4 * y2 + y1) * out.sy + int64_t(t) * out.sx;
+    T* const rb = ra + 4 * out.sy;
+    static_for<0, EX / 2>([&](auto e_) {
+      constexpr int e = decltype(e_)::value;
+      const int64_t x = int64_t(TX * e) * out.sx;
+      if constexpr (ACC) {
+        ra[x] += v[p][e].re;
+        rb[x] += v[p][e].im;
+      } else {
+        __builtin_nontemporal_store(v[p][e].re, ra + x);
+        __builtin_nontemporal_store(v[p][e].im, rb + x);
+      }
+    });
+  });
+}
+
+// Build-time choices of the three-pass z pass (measured: ordinary stores beat streaming ones
+// there, quarter-line tiles beat half-line ones, six jobs per workgroup beat two or three).
+#ifndef SOPHT_ZROW_STREAM
+#define SOPHT_ZROW_STREAM 0
+#endif
+#ifndef SOPHT_ZROW_PARTS
+#define SOPHT_ZROW_PARTS 4
+#endif
+#ifndef SOPHT_ZROW_JOBS
+#define SOPHT_ZROW_JOBS 6
+#endif
+constexpr bool kZrowStream = SOPHT_ZROW_STREAM;
+constexpr int kZrowJobs = SOPHT_ZROW_JOBS;
+
+// The z pass of the three-pass schedule. A workgroup owns one kx tile (WC columns) of the row
+// class k2 (slab rows y1 L2 + k2, y1 < 4) and of its mirror class L2 - k2. Per job (component,
+// class) the four rows are loaded, the 4-point DFT over y1 gives the pencils ky = k2 + L2 k1,
+// each pencil is convolved along z as in k3_zconv, and the inverse 4-point DFT returns the rows.
+// The workgroup has two teams per column and transforms two pencils at a time (two LDS tiles),
+// so a thread holds E / 4 values of each row: the next job's rows are loaded during the
+// transforms within 128 VGPRs. The G^ lines of the class, |k2 + L2 k1| folded to <= ny for
+// k1 < 4, are staged in LDS once per workgroup. Slot s of a column takes pencil k1 = s + 2r in
+// round r of a job, and so G^ line s + 2r; in a mirror job it takes pencil 3 - s - 2r, whose
+// ky' = L2 - k2 + L2 (3 - s - 2r) has the line of 2 ny - ky' = k2 + L2 (s + 2r): the same.
+template <typename T, int L>
+struct ZRowTile {
+  static constexpr int E = LinePlan<L>::E, TT = LinePlan<L>::T;
+  // a quarter of a 128-byte line per workgroup (two workgroups per CU); the four quarters of a
+  // line go to workgroups b, b + 8, b + 16, b + 24 (the same XCD), so three come from its L2
+  static constexpr int PARTS = SOPHT_ZROW_PARTS;
+  static constexpr int WC = kLineCplx<T> / PARTS;
+  static constexpr int NT = 2 * WC * TT;
+  // tiles [slot][column]: the load role's lanes (column, z) hit distinct banks when the
+  // column stride is 8 (mod 32) slots
+  static constexpr int S = TT < 32 ? L + 16 : L + 8;
+  static constexpr int KL = E / 4;  // z values per row per thread (z = lr + 2 TT k)
+  static constexpr int GL = L / 2 + 1;
+  static constexpr size_t lds = size_t(2 * WC * S) * sizeof(Cplx<T>) + size_t(4 * WC * GL) * sizeof(T);
+  static_assert(E % 4 == 0, "the z rows split in quarters");
+};
+
+template <typename T, int L>
+__global__ void __launch_bounds__((ZRowTile<T, L>::NT)) __attribute__((amdgpu_waves_per_eu(kZconvWavesFor<T, L>))) k3_zconv4(SlabGeom g, Cplx<T>* H, const T* ghat,
+                                                                  const Cplx<T>* tw) {
+  using ZT = ZRowTile<T, L>;
+  constexpr int E = ZT::E, TT = ZT::TT, WC = ZT::WC, S = ZT::S, KL = ZT::KL, GL = ZT::GL;
+  extern __shared__ unsigned char smem_raw[];
+  Cplx<T>* xb = reinterpret_cast<Cplx<T>*>(smem_raw);  // tiles [slot][column][z]
+  T* const gl = reinterpret_cast<T*>(xb + 2 * WC * S);  // G^ lines [k1][column][kz]
+  const int tid = threadIdx.x;
+  const int lw = tid % WC, lr = tid / WC;  // load role: column, first z
+  const int team = tid / TT, t = tid % TT;  // transform role: team = slot * WC + column
+  const int tcol = team % WC, slot = team / WC;
+  const int64_t ncols = g.nx + 1;
+  const int L2 = int(g.ny) / 2;
+  int64_t k2, kx0;
+  if (!col_tile_parts<ZT::PARTS>(blockIdx.x, ncols, L2 / 2 + 1, WC, k2, kx0)) return;
+  const bool self = k2 == 0 || 2 * k2 == L2;
+  const int alljobs = self ? g.ncomp : 2 * g.ncomp;
+  const int jbeg = int(blockIdx.y) * kZrowJobs;
+  if (jbeg >= alljobs) return;
+  const int njobs = alljobs - jbeg < kZrowJobs ? alljobs - jbeg : kZrowJobs;
+  const int64_t kx = kx0 + lw;
+  const bool live = kx < ncols;
+  const int first_col = 64 / TT >= WC ? 0 : ((tid / 64) * (64 / TT)) % WC;
+  const bool wave_live = kx0 + first_col < ncols;
+  {
+    // every load in flight before the first LDS write (a loop would wait on each in turn)
+    constexpr int NG = 4 * WC * GL, NIT = (NG + ZT::NT - 1) / ZT::NT;
+    T gt[NIT];
+    static_for<0, NIT>([&](auto i_) {
+      constexpr int it = decltype(i_)::value;
+      const int i = tid + it * ZT::NT;
+      const int kz = i % GL, rest = i / GL;
+      const int w = rest % WC, k1 = (rest / WC) & 3;
+      const int ky = int(k2) + L2 * k1;
+      const int kyo = ky <= int(g.ny) ? ky : 2 * int(g.ny) - ky;
+      const int kxt = kx0 + w < ncols ? int(kx0) + w : 0;
+      gt[it] = (it + 1 < NIT || i < NG) ? ghat[(kyo * int(ncols) + kxt) * int(g.gz) + kz] : T(0);
+    });
+    static_for<0, NIT>([&](auto i_) {
+      constexpr int it = decltype(i_)::value;
+      const int i = tid + it * ZT::NT;
+      if (it + 1 < NIT || i < NG) gl[i] = gt[it];
+    });
+  }
+  const int zs = 2 * int(g.ny) * int(g.rs);
+  using P3 = Plan3<typename LinePlan<L>::R>;
+  constexpr int R1 = at_least_1(P3::r1), R2 = at_least_1(P3::r2);
+  constexpr int PER2 = at_least_1(E / R1), PER3 = at_least_1(E / R2);
+  Cplx<T> w2[PER2][R1], w3[PER3][R2];
+  if constexpr (last_twiddles_hoisted<L>()) {
+    static_for<0, PER2>([&](auto p_) {
+      constexpr int p = decltype(p_)::value;
+      rfft::stage_twiddles<T, L, R1, P3::r0>(w2[p], tw, (t + p * TT) % P3::r0);
+    });
+    static_for<0, PER3>([&](auto p_) {
+      constexpr int p = decltype(p_)::value;
+      rfft::stage_twiddles<T, L, R2, P3::r0 * R1>(w3[p], tw, (t + p * TT) % (P3::r0 * R1));
+    });
+  }
+  auto row_col = [&](int jl, int y1) -> int {
+    const int j = jbeg + jl;
+    const int cc = j % g.ncomp;
+    const int cls = j < g.ncomp ? int(k2) : L2 - int(k2);
+    return g.hi(cc, 0, y1 * L2 + cls, live ? int(kx) : 0);
+  };
+  Cplx<T> pf[4][KL];
+  auto load = [&](int j) {
+    static_for<0, 4>([&](auto y_) {
+      constexpr int y1 = decltype(y_)::value;
+      const int colp = row_col(j, y1);
+      static_for<0, KL>([&](auto k_) {
+        constexpr int k = decltype(k_)::value;
+        pf[y1][k] = live ? H[colp + (lr + 2 * TT * k) * zs] : Cplx<T>{T(0), T(0)};
+      });
+    });
+  };
+  Cplx<T>* const line = xb + team * S;
+  load(0);
+  __syncthreads();  // G^ lines staged
+  for (int j = 0; j < njobs; ++j) {
+    const bool mirror = jbeg + j >= g.ncomp;
+    Cplx<T> y[4][KL];
+    static_for<0, KL>([&](auto k_) {
+      constexpr int k = decltype(k_)::value;
+      Cplx<T> u[4] = {pf[0][k], pf[1][k], pf[2][k], pf[3][k]};
+      rfft::dft4<T, false>(u);
+      static_for<0, 4>([&](auto q_) { y[decltype(q_)::value][k] = u[decltype(q_)::value]; });
+    });
+    if (j + 1 < njobs) load(j + 1);
+    static_for<0, 2>([&](auto r_) {
+      constexpr int r = decltype(r_)::value;
+      // slot 0 takes pencil 2r (a mirror job: 3 - 2r), slot 1 pencil 2r + 1 (mirror: 2 - 2r)
+      static_for<0, 2>([&](auto s_) {
+        constexpr int sl = decltype(s_)::value;
+        static_for<0, KL>([&](auto k_) {
+          constexpr int k = decltype(k_)::value;
+          // selects on values (a select of two array elements' addresses would leave y in
+          // scratch memory)
+          const Cplx<T> a = y[3 - sl - 2 * r][k], b = y[sl + 2 * r][k];
+          xb[(sl * WC + lw) * S + lr + 2 * TT * k] = Cplx<T>{mirror ? a.re : b.re, mirror ? a.im : b.im};
+        });
+      });
+      __syncthreads();
+      Cplx<T> v[E];
+      static_for<0, E>([&](auto e_) {
+        constexpr int e = decltype(e_)::value;
+        if constexpr (e < E / 2) v[e] = line[t + TT * e];
+        else v[e] = Cplx<T>{T(0), T(0)};
+      });
+      rfft::wave_sync();
+      if (wave_live) {
+        const T* const gq = gl + ((slot + 2 * r) * WC + tcol) * GL;
+        if constexpr (last_twiddles_hoisted<L>()) {
+          constexpr int R0 = P3::r0, NS3 = R0 * R1;
+          rfft::stockham_stage<T, L, R0, 1, false, true>(v, line, tw, t);
+          rfft::stockham_stage_given<T, L, R1, R0, false>(v, line, w2, t);
+          rfft::stockham_stage_given<T, L, R2, NS3, false>(v, line, w3, t);
+        } else {
+          rfft::team_fft<T, L, false, true>(v, line, tw, t);
+        }
+        static_for<0, E>([&](auto e_) {
+          constexpr int e = decltype(e_)::value;
+          const int kz = t + TT * e;
+          const T gv = gq[kz <= L / 2 ? kz : L - kz];
+          v[e] = Cplx<T>{v[e].re * gv, v[e].im * gv};
+        });
+        if constexpr (last_twiddles_hoisted<L>()) {
+          constexpr int R0 = P3::r0, NS3 = R0 * R1;
+          rfft::stockham_stage<T, L, R0, 1, true, false>(v, line, tw, t);
+          rfft::stockham_stage_given<T, L, R1, R0, true>(v, line, w2, t);
+          rfft::stockham_stage_given<T, L, R2, NS3, true>(v, line, w3, t);
+        } else {
+          rfft::team_fft<T, L, true>(v, line, tw, t);
+        }
+      }
+      static_for<0, E / 2>([&](auto e_) {
+        constexpr int e = decltype(e_)::value;
+        line[t + TT * e] = v[e];
+      });
+      __syncthreads();
+      static_for<0, 2>([&](auto s_) {
+        constexpr int sl = decltype(s_)::value;
+        static_for<0, KL>([&](auto k_) {
+          constexpr int k = decltype(k_)::value;
+          const Cplx<T> val = xb[(sl * WC + lw) * S + lr + 2 * TT * k];
+          Cplx<T>& a = y[3 - sl - 2 * r][k];
+          Cplx<T>& b = y[sl + 2 * r][k];
+          a = Cplx<T>{mirror ? val.re : a.re, mirror ? val.im : a.im};
+          b = Cplx<T>{mirror ? b.re : val.re, mirror ? b.im : val.im};
+        });
+      });
+      __syncthreads();
+    });
+    if (live) {
+      static_for<0, KL>([&](auto k_) {
+        constexpr int k = decltype(k_)::value;
+        Cplx<T> u[4] = {y[0][k], y[1][k], y[2][k], y[3][k]};
+        rfft::dft4<T, true>(u);
+        static_for<0, 4>([&](auto q_) {
+          constexpr int y1 = decltype(q_)::value;
+          if constexpr (kZrowStream) store_stream(H + row_col(j, y1) + (lr + 2 * TT * k) * zs, u[y1]);
+          else H[row_col(j, y1) + (lr + 2 * TT * k) * zs] = u[y1];
+        });
+      });
+    }
+  }
+}
+
 // ------------------------------------------------------------------ Green's function octant
 // g[z][y][x] = 1/(4 pi r dx) on integer offsets, g[0] = 1/(4 pi dx), in FP64.
 __global__ void k_green_octant(double* g, int64_t nz1, int64_t ny1, int64_t nx1, double dx) {
@@ -604,6 +1115,96 @@ int launch_k5(int64_t L, SlabGeom g, const Cplx<T>* H, CompPtrsW out, const Cplx
   return SOPHT_ERR_UNSUPPORTED;
 }
 
+// Line lengths 2 nx = 2 ny of the three-pass schedule, and 2 nz (longer z lines leave the
+// z pass without registers for its next rows).
+#define SOPHT_XY_LENGTHS(X) X(128) X(256) X(384) X(512)
+#define SOPHT_ZROW_LENGTHS(X) \
+  X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512)
+
+bool zrow_length_supported(int64_t L) {
+#define X(n) if (L == n) return true;
+  SOPHT_ZROW_LENGTHS(X)
+#undef X
+  return false;
+}
+
+bool xy_length_supported(int64_t L) {
+#define X(n) if (L == n) return true;
+  SOPHT_XY_LENGTHS(X)
+#undef X
+  return false;
+}
+
+// Workgroups above 64 KB of LDS declare it once per kernel.
+template <typename K>
+int allow_lds(K* kernel, size_t bytes) {
+  if (bytes <= 65536) return SOPHT_OK;
+  SOPHT_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes)));
+  return SOPHT_OK;
+}
+
+template <typename T>
+int launch_xy_fwd(int64_t L, SlabGeom g, CompPtrs in, Cplx<T>* H, const Cplx<T>* twx,
+                  const Cplx<T>* twy, hipStream_t s) {
+  const unsigned blocks = unsigned(int64_t(g.ncomp) * g.nz * 4);
+#define X(n)                                                                          \
+  if (L == n) {                                                                       \
+    using F = XYPass<T, n, n>;                                                        \
+    static const int lds_ok = allow_lds(ka_xy_fwd<T, n, n>, F::lds);                  \
+    if (lds_ok) return lds_ok;                                                        \
+    ka_xy_fwd<T, n, n><<<blocks, F::NT, F::lds, s>>>(g, in, H, twx, twy);             \
+    SOPHT_CHECK_LAUNCH();                                                             \
+    return SOPHT_OK;                                                                  \
+  }
+  SOPHT_XY_LENGTHS(X)
+#undef X
+  return SOPHT_ERR_UNSUPPORTED;
+}
+
+template <typename T>
+int launch_yx_inv(int64_t L, SlabGeom g, const Cplx<T>* H, CompPtrsW out, const Cplx<T>* twx,
+                  const Cplx<T>* twy, hipStream_t s, bool acc) {
+  const unsigned blocks = unsigned(int64_t(g.ncomp) * g.nz * 4);
+#define X(n)                                                                          \
+  if (L == n) {                                                                       \
+    using F = XYPass<T, n, n>;                                                        \
+    if (acc) {                                                                        \
+      static const int lds_ok = allow_lds(kc_yx_inv<T, n, n, true>, F::lds);          \
+      if (lds_ok) return lds_ok;                                                      \
+      kc_yx_inv<T, n, n, true><<<blocks, F::NT, F::lds, s>>>(g, H, out, twx, twy);    \
+    } else {                                                                          \
+      static const int lds_ok = allow_lds(kc_yx_inv<T, n, n, false>, F::lds);         \
+      if (lds_ok) return lds_ok;                                                      \
+      kc_yx_inv<T, n, n, false><<<blocks, F::NT, F::lds, s>>>(g, H, out, twx, twy);   \
+    }                                                                                 \
+    SOPHT_CHECK_LAUNCH();                                                             \
+    return SOPHT_OK;                                                                  \
+  }
+  SOPHT_XY_LENGTHS(X)
+#undef X
+  return SOPHT_ERR_UNSUPPORTED;
+}
+
+template <typename T>
+int launch_k3_rows(int64_t L, SlabGeom g, Cplx<T>* H, const T* ghat, const Cplx<T>* tw,
+                   hipStream_t s) {
+#define X(n)                                                                          \
+  if (L == n) {                                                                       \
+    using ZT = ZRowTile<T, n>;                                                        \
+    const int64_t nb = col_tile_parts_blocks(ZT::PARTS, g.nx + 1, g.ny / 4 + 1, ZT::WC); \
+    const unsigned nch = unsigned((2 * g.ncomp + kZrowJobs - 1) / kZrowJobs);          \
+    static const int lds_ok = allow_lds(k3_zconv4<T, n>, ZT::lds);                    \
+    if (lds_ok) return lds_ok;                                                        \
+    k3_zconv4<T, n><<<dim3(unsigned(nb), nch), ZT::NT, ZT::lds, s>>>(g, H, ghat, tw);  \
+    SOPHT_CHECK_LAUNCH();                                                             \
+    return SOPHT_OK;                                                                  \
+  }
+  SOPHT_ZROW_LENGTHS(X)
+#undef X
+  return SOPHT_ERR_UNSUPPORTED;
+}
+
 template <typename T>
 SlabGeom slab_geom(const sopht_poisson3d* P, int ncomp) {
   return SlabGeom{P->nz, P->ny, P->nx, kRowPad ? round_up(P->nx + 1, kLineCplx<T>) : P->nx + 1,
@@ -640,6 +1241,17 @@ int solve_impl(sopht_poisson3d* P, int ncomp, void* const sol[3], const int64_t*
   auto mark = [&](int i) {
     if (timer) timer->mark(first_stage + i, s);
   };
+  if constexpr (sizeof(T) == 4) {  // the three-pass kernels are built for FP32
+    if (P->passes == 3) {
+      if ((rc = launch_xy_fwd<T>(2 * P->nx, g, in, H, twx, twy, s))) return rc;
+      mark(0);
+      if ((rc = launch_k3_rows<T>(2 * P->nz, g, H, static_cast<const T*>(P->ghat), twz, s))) return rc;
+      mark(1);
+      rc = launch_yx_inv<T>(2 * P->nx, g, H, out, twx, twy, s, accumulate);
+      mark(2);
+      return rc;
+    }
+  }
   if ((rc = launch_k1<T>(2 * P->nx, g, in, H, twx, s))) return rc;
   mark(0);
   if ((rc = launch_ycols<T, false>(2 * P->ny, g, H, twy, s))) return rc;
@@ -723,6 +1335,8 @@ int build_ghat(sopht_poisson3d* P, hipStream_t s) {
 
 }  // namespace
 
+int poisson_stage_count(const sopht_poisson3d* P) { return P->passes; }
+
 int poisson_vector_solve_timed(sopht_poisson3d* P, void* const sol[3], const int64_t* ss,
                                const void* const rhs[3], const int64_t* rs, hipStream_t s,
                                StageTimer* timer, int first_stage) {
@@ -791,6 +1405,13 @@ int poisson_create(int dtype, int64_t nz, int64_t ny, int64_t nx, double x_range
     return SOPHT_ERR_UNSUPPORTED;
   }
   P->ghat_bytes = (ny + 1) * (nx + 1) * P->gz * int64_t(rsz);
+  // the three-pass schedule where its kernels are built (FP32, nx = ny, 2 nx in
+  // SOPHT_XY_LENGTHS, 2 nz in SOPHT_ZROW_LENGTHS); SOPHT_POISSON_PASSES=5 in the environment
+  // keeps the five-pass one
+  const char* forced = std::getenv("SOPHT_POISSON_PASSES");
+  const bool five = forced != nullptr && std::atoi(forced) == 5;
+  P->passes = (!five && dtype == SOPHT_F32 && nx == ny && xy_length_supported(2 * nx) &&
+               zrow_length_supported(2 * nz)) ? 3 : 5;
   int rc = SOPHT_OK;
   auto fail = [&](int code) {
     sopht_poisson3d_destroy(P);

@@ -115,3 +115,47 @@ def test_solve_on_384_and_768_point_lines(dev, precision, shape, rng):
     s.vector_field_solve(solution_vector_field=out, rhs_vector_field=to_dev(rhs, dev))
     bound = 1e-10 if real_t == np.float64 else 2e-5
     assert rel_l2(to_host(out), ref) <= bound, rel_l2(to_host(out), ref)
+
+
+@pytest.mark.parametrize("shape", [(64, 64, 64), (48, 64, 64), (32, 128, 128), (12, 192, 192)])
+def test_three_pass_schedule_matches_oracle(dev, shape, rng):
+    """FP32 grids with nx = ny and 2 nx in {128, 256, 384, 512} run the three-pass schedule
+    (the x transform fused with the inner sums of the y transform, and the outer 4-point sums
+    in the z pass); against the FP64 oracle at the five-pass schedule's FP32 bound."""
+    rhs = rng.standard_normal((3, *shape)).astype(np.float32)
+    ref = np.zeros_like(rhs, dtype=np.float64)
+    P.UnboundedPoissonSolver3D(*shape, x_range=1.0, real_t=np.float64).vector_field_solve(ref, rhs.astype(np.float64))
+    s = solver_cls()(*shape, x_range=1.0, real_t=np.float32)
+    out = to_dev(np.zeros_like(rhs), dev)
+    s.vector_field_solve(solution_vector_field=out, rhs_vector_field=to_dev(rhs, dev))
+    got = to_host(out)
+    assert rel_l2(got, ref) <= 2e-5, rel_l2(got, ref)
+    assert rel_linf(got, ref) <= 1e-4, rel_linf(got, ref)
+    # the scalar solve takes the same passes with one component
+    one = to_dev(np.zeros_like(rhs[0]), dev)
+    s.solve(solution_field=one, rhs_field=to_dev(rhs[1], dev))
+    assert rel_l2(to_host(one), ref[1]) <= 2e-5
+
+
+@pytest.mark.parametrize("n", [192, 256])
+def test_three_pass_against_five_pass(dev, n, monkeypatch):
+    """At the workloads' sizes, the three-pass solution against the five-pass one on the same
+    FP32 right-hand side (SOPHT_POISSON_PASSES=5 at plan creation selects the five passes)."""
+    import torch
+
+    gen = torch.Generator(device=dev).manual_seed(7)
+    rhs = torch.randn((3, n, n, n), dtype=torch.float32, device=dev, generator=gen)
+    outs = []
+    for passes in ("5", None):
+        if passes:
+            monkeypatch.setenv("SOPHT_POISSON_PASSES", passes)
+        else:
+            monkeypatch.delenv("SOPHT_POISSON_PASSES", raising=False)
+        s = solver_cls()(n, n, n, x_range=1.0, real_t=np.float32)
+        out = torch.zeros_like(rhs)
+        s.vector_field_solve(solution_vector_field=out, rhs_vector_field=rhs)
+        outs.append(out)
+        del s
+    err = (torch.linalg.vector_norm(outs[1] - outs[0]) / torch.linalg.vector_norm(outs[0])).item()
+    print(f"{n}^3 FP32 three-pass vs five-pass rel-L2 {err:.3e}")
+    assert err <= 1e-5, err
