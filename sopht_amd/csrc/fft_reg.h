@@ -272,9 +272,17 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Twiddle tables copied to LDS are stored padded, element m at m + m / 32: a stage reads the
+// table at stride r L / (NS R) across its lanes, and without the padding those reads fall on
+// 2 to 8 lanes per bank (the stage with NS = 8 of a 512-point line); with it, on one for
+// every plan of 128, 256, 512 and 1024 points (two for 384 and 768).
+__host__ __device__ constexpr int tw_slot(int m) { return m + (m >> 5); }
+template <int L>
+constexpr int tw_lds_size() { return L + L / 32 + 1; }
+
 // HALF_IN (first stage only): the line's upper half x[L/2 .. L) is zero, i.e. the inputs
-// r >= R/2 of every butterfly.
-template <typename T, int L, int R, int NS, bool INV, bool HALF_IN>
+// r >= R/2 of every butterfly. PADTW: tw is a padded LDS table (tw_slot).
+template <typename T, int L, int R, int NS, bool INV, bool HALF_IN, bool PADTW = false>
 __device__ __forceinline__ void stockham_stage(Cplx<T>* v, Cplx<T>* xb, const Cplx<T>* tw,
                                                int t) {
   constexpr int E = LinePlan<L>::E, TT = LinePlan<L>::T;
@@ -294,7 +302,7 @@ __device__ __forceinline__ void stockham_stage(Cplx<T>* v, Cplx<T>* xb, const Cp
       static_for<1, R>([&](auto r_) {
         constexpr int r = decltype(r_)::value;
         constexpr int hb = kTwiddleProducts ? 1 << ilog2(r) : r;
-        if constexpr (hb == r) wr[r] = tw[r * k * step];
+        if constexpr (hb == r) wr[r] = tw[PADTW ? tw_slot(r * k * step) : r * k * step];
         else wr[r] = cmul(wr[hb], wr[r - hb]);
       });
       static_for<1, R>([&](auto r_) {
@@ -381,27 +389,34 @@ __device__ __forceinline__ void stockham_stage_given(Cplx<T>* v, Cplx<T>* xb,
   }
 }
 
-template <typename T, int L, bool INV, bool HALF_IN, int NS, int R0, int... Rest>
+template <typename T, int L, bool INV, bool HALF_IN, bool PADTW, int NS, int R0, int... Rest>
 __device__ __forceinline__ void stockham_run(Cplx<T>* v, Cplx<T>* xb, const Cplx<T>* tw, int t) {
-  stockham_stage<T, L, R0, NS, INV, HALF_IN && NS == 1>(v, xb, tw, t);
+  stockham_stage<T, L, R0, NS, INV, HALF_IN && NS == 1, PADTW>(v, xb, tw, t);
   if constexpr (sizeof...(Rest) > 0)
-    stockham_run<T, L, INV, HALF_IN, NS * R0, Rest...>(v, xb, tw, t);
+    stockham_run<T, L, INV, HALF_IN, PADTW, NS * R0, Rest...>(v, xb, tw, t);
 }
 
-template <typename T, int L, bool INV, bool HALF_IN, int... Rs>
+template <typename T, int L, bool INV, bool HALF_IN, bool PADTW, int... Rs>
 __device__ __forceinline__ void stockham_plan(Cplx<T>* v, Cplx<T>* xb, const Cplx<T>* tw, int t,
                                               Radices<Rs...>) {
-  stockham_run<T, L, INV, HALF_IN, 1, Rs...>(v, xb, tw, t);
+  stockham_run<T, L, INV, HALF_IN, PADTW, 1, Rs...>(v, xb, tw, t);
 }
 
 // In-place transform of a line held as v[e] = x[t + T e] by thread t of its team; on return
 // v[e] = X[t + T e] (forward: exp(-), inverse: exp(+), unnormalised). `region` (L elements
 // of LDS) belongs to the team alone; its contents are undefined on return. HALF_IN: the
 // caller guarantees x[L/2 .. L) = 0 (v[e] = 0 for e >= E/2), which prunes the first stage.
-template <typename T, int L, bool INV, bool HALF_IN = false>
+// PADTW: tw is a padded LDS table (tw_slot).
+template <typename T, int L, bool INV, bool HALF_IN = false, bool PADTW = false>
 __device__ __forceinline__ void team_fft(Cplx<T>* v, Cplx<T>* region, const Cplx<T>* tw, int t) {
   static_assert(64 % LinePlan<L>::T == 0, "a team must sit inside one wavefront");
-  stockham_plan<T, L, INV, HALF_IN>(v, region, tw, t, typename LinePlan<L>::R{});
+  stockham_plan<T, L, INV, HALF_IN, PADTW>(v, region, tw, t, typename LinePlan<L>::R{});
+}
+
+// Copy the twiddle table tw[0, L) into the padded LDS table twl (threads i0, i0 + step, ...).
+template <typename T, int L>
+__device__ __forceinline__ void load_tw_lds(Cplx<T>* twl, const Cplx<T>* tw, int i0, int step) {
+  for (int i = i0; i < L; i += step) twl[tw_slot(i)] = tw[i];
 }
 
 }  // namespace rfft

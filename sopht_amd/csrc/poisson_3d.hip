@@ -208,9 +208,9 @@ __global__ void __launch_bounds__((RowTile<T, L>::NT)) k1_xr2c(SlabGeom g, CompP
   Cplx<T>* const line = xb + team * RT::S;
   // the twiddle table in LDS, shared by the workgroup's teams
   Cplx<T>* const twl = xb + RT::NTEAM * RT::S;
-  for (int i = threadIdx.x; i < L; i += RT::NT) twl[i] = tw[i];
+  rfft::load_tw_lds<T, L>(twl, tw, threadIdx.x, RT::NT);
   __syncthreads();
-  team_fft<T, L, false, true>(v, line, twl, t);
+  team_fft<T, L, false, true, true>(v, line, twl, t);
   static_for<0, E>([&](auto e_) {
     constexpr int e = decltype(e_)::value;
     line[t + TT * e] = v[e];
@@ -251,7 +251,7 @@ __global__ void __launch_bounds__((ColTile<T, L>::NT)) k_ycols(SlabGeom g, Cplx<
   const int ys = int(g.rs);  // stride between consecutive y (or ky)
   // the twiddle table in LDS (one load per thread), read by every stage of the 8 transforms
   Cplx<T>* const twl = xb + WC * S;
-  for (int i = tid; i < L; i += CT::NT) twl[i] = tw[i];
+  rfft::load_tw_lds<T, L>(twl, tw, tid, CT::NT);
   static_for<0, KIN>([&](auto k_) {
     constexpr int k = decltype(k_)::value;
     const int y = lr + TT * k;
@@ -266,7 +266,7 @@ __global__ void __launch_bounds__((ColTile<T, L>::NT)) k_ycols(SlabGeom g, Cplx<
     else v[e] = Cplx<T>{T(0), T(0)};
   });
   rfft::wave_sync();
-  if (wave_live) team_fft<T, L, INV, !INV>(v, line, twl, t);
+  if (wave_live) team_fft<T, L, INV, !INV, true>(v, line, twl, t);
   static_for<0, KOUT>([&](auto e_) {
     constexpr int e = decltype(e_)::value;
     line[t + TT * e] = v[e];
@@ -459,7 +459,7 @@ __global__ void __launch_bounds__((RowTile<T, L>::NT)) k5_xc2r(SlabGeom g, const
   Cplx<T>* const line = xb + team * RT::S;
   // the twiddle table in LDS, shared by the workgroup's teams (one barrier, below)
   Cplx<T>* const twl = xb + RT::NTEAM * RT::S;
-  for (int i = threadIdx.x; i < L; i += RT::NT) twl[i] = tw[i];
+  rfft::load_tw_lds<T, L>(twl, tw, threadIdx.x, RT::NT);
   for (int k = t; k <= nx; k += TT) {
     Cplx<T> a = {T(0), T(0)}, bb = {T(0), T(0)};
     if (ha) a = ha[k];
@@ -478,7 +478,7 @@ __global__ void __launch_bounds__((RowTile<T, L>::NT)) k5_xc2r(SlabGeom g, const
     v[e] = line[t + TT * e];
   });
   __syncthreads();  // the twiddle table is complete
-  team_fft<T, L, true>(v, line, twl, t);
+  team_fft<T, L, true, false, true>(v, line, twl, t);
   auto row_ptr = [&](int64_t r) -> T* {
     const int c = int(r / (g.nz * g.ny));
     const int64_t rem = r % (g.nz * g.ny);
@@ -532,11 +532,12 @@ struct XYPass {
   // staging tile [row][P]: the column teams' accesses (T2 rows of 32 / T2 columns per half
   // wavefront) fall in distinct banks when P = 32 / T2 (mod 32)
   static constexpr int P = (CW + 31) / 32 * 32 + 32 / T2;
-  static constexpr int XPH = NTEAM * SX + LX;  // x regions and the x twiddle table
+  static constexpr int TW2 = rfft::tw_lds_size<L2>();                // padded w_L2 table
+  static constexpr int XPH = NTEAM * SX + rfft::tw_lds_size<LX>();  // x regions, x table
   static constexpr int CREG = CW * S2;         // column regions
   static constexpr int STG = L2 * P;           // staging tile
   static constexpr int UNION = XPH > CREG ? (XPH > STG ? XPH : STG) : (CREG > STG ? CREG : STG);
-  static constexpr size_t lds = size_t(2 * L2 + UNION) * sizeof(Cplx<T>);
+  static constexpr size_t lds = size_t(TW2 + L2 + UNION) * sizeof(Cplx<T>);
   // the staging tile moves to and from HBM as rows of CW columns: thread tid takes column
   // tid % CW of the rows tid / CW + q NT / CW
   static constexpr int RSTEP = NT / CW, NQ = L2 / RSTEP;
@@ -554,8 +555,8 @@ __global__ void __launch_bounds__((XYPass<T, LX, LY>::NT)) ka_xy_fwd(SlabGeom g,
   constexpr int EX = F::EX, TX = F::TX, L2 = F::L2, E2 = F::E2, T2 = F::T2, PPT = F::PPT, P = F::P;
   constexpr int CW = F::CW, NT = F::NT;
   extern __shared__ unsigned char smem_raw[];
-  Cplx<T>* const tw2 = reinterpret_cast<Cplx<T>*>(smem_raw);  // w_L2^m = w_LY^{4m}
-  Cplx<T>* const twr = tw2 + L2;                               // w_LY^{y1 k2}
+  Cplx<T>* const tw2 = reinterpret_cast<Cplx<T>*>(smem_raw);  // w_L2^m = w_LY^{4m}, padded
+  Cplx<T>* const twr = tw2 + F::TW2;                           // w_LY^{y1 k2}
   Cplx<T>* const U = twr + L2;
   const int tid = threadIdx.x;
   const int y1 = int(blockIdx.x & 3);
@@ -563,11 +564,11 @@ __global__ void __launch_bounds__((XYPass<T, LX, LY>::NT)) ka_xy_fwd(SlabGeom g,
   const int c = cz / int(g.nz), z = cz % int(g.nz);
   const int team = tid / TX, t = tid % TX;
   for (int i = tid; i < L2; i += NT) {
-    tw2[i] = twy[4 * i];
+    tw2[rfft::tw_slot(i)] = twy[4 * i];  // padded (w_L2^m = w_LY^{4m})
     twr[i] = twy[(y1 * i) % LY];
   }
   Cplx<T>* const twxl = U + F::NTEAM * F::SX;
-  for (int i = tid; i < LX; i += NT) twxl[i] = twx[i];
+  rfft::load_tw_lds<T, LX>(twxl, twx, tid, NT);
   const T* base = static_cast<const T*>(in.get(c)) + int64_t(z) * in.sz;
   Cplx<T> v[PPT][EX];
   static_for<0, PPT>([&](auto p_) {
@@ -591,7 +592,7 @@ __global__ void __launch_bounds__((XYPass<T, LX, LY>::NT)) ka_xy_fwd(SlabGeom g,
   const int mirror_lane = (TX - t) % TX;
   static_for<0, PPT>([&](auto p_) {
     constexpr int p = decltype(p_)::value;
-    rfft::team_fft<T, LX, false, true>(v[p], xline, twxl, t);
+    rfft::team_fft<T, LX, false, true, true>(v[p], xline, twxl, t);
     static_for<0, EX / 2 + 1>([&](auto e_) {
       constexpr int e = decltype(e_)::value;
       // Z[LX - k], k = TX e + t: lane TX - t, element EX - 1 - e (t > 0); lane 0: EX - e
@@ -635,7 +636,7 @@ __global__ void __launch_bounds__((XYPass<T, LX, LY>::NT)) ka_xy_fwd(SlabGeom g,
     }
     __syncthreads();
     if (live) {
-      rfft::team_fft<T, L2, false, true>(w, creg, tw2, t2);
+      rfft::team_fft<T, L2, false, true, true>(w, creg, tw2, t2);
       static_for<0, E2>([&](auto e_) {
         constexpr int e = decltype(e_)::value;
         w[e] = cmul(w[e], twr[t2 + T2 * e]);
@@ -671,7 +672,7 @@ __global__ void __launch_bounds__((XYPass<T, LX, LY>::NT)) kc_yx_inv(SlabGeom g,
   constexpr int CW = F::CW, NT = F::NT;
   extern __shared__ unsigned char smem_raw[];
   Cplx<T>* const tw2 = reinterpret_cast<Cplx<T>*>(smem_raw);
-  Cplx<T>* const twr = tw2 + L2;
+  Cplx<T>* const twr = tw2 + F::TW2;
   Cplx<T>* const U = twr + L2;
   const int tid = threadIdx.x;
   const int y1 = int(blockIdx.x & 3);
@@ -679,7 +680,7 @@ __global__ void __launch_bounds__((XYPass<T, LX, LY>::NT)) kc_yx_inv(SlabGeom g,
   const int c = cz / int(g.nz), z = cz % int(g.nz);
   const int team = tid / TX, t = tid % TX;
   for (int i = tid; i < L2; i += NT) {
-    tw2[i] = twy[4 * i];
+    tw2[rfft::tw_slot(i)] = twy[4 * i];  // padded (w_L2^m = w_LY^{4m})
     twr[i] = twy[(y1 * i) % LY];
   }
   const int col = tid / T2, t2 = tid % T2;
@@ -720,7 +721,7 @@ __global__ void __launch_bounds__((XYPass<T, LX, LY>::NT)) kc_yx_inv(SlabGeom g,
       });
     }
     __syncthreads();
-    if (live) rfft::team_fft<T, L2, true>(w, creg, tw2, t2);
+    if (live) rfft::team_fft<T, L2, true, false, true>(w, creg, tw2, t2);
     __syncthreads();
     if (live) {
       static_for<0, E2 / 2>([&](auto e_) {
@@ -757,13 +758,13 @@ __global__ void __launch_bounds__((XYPass<T, LX, LY>::NT)) kc_yx_inv(SlabGeom g,
     __syncthreads();
   });
   Cplx<T>* const twxl = U + F::NTEAM * F::SX;
-  for (int i = tid; i < LX; i += NT) twxl[i] = twx[i];
+  rfft::load_tw_lds<T, LX>(twxl, twx, tid, NT);
   __syncthreads();
   Cplx<T>* const xline = U + team * F::SX;
   T* const base = static_cast<T*>(out.get(c)) + int64_t(z) * out.sz;
   static_for<0, PPT>([&](auto p_) {
     constexpr int p = decltype(p_)::value;
-    rfft::team_fft<T, LX, true>(v[p], xline, twxl, t);
+    rfft::team_fft<T, LX, true, false, true>(v[p], xline, twxl, t);
     const int y2 = 2 * (team * PPT + p);
     T* const ra = base + int64_t(4 * y2 + y1) * out.sy + int64_t(t) * out.sx;
     T* const rb = ra + 4 * out.sy;
@@ -1054,7 +1055,7 @@ int launch_k1(int64_t L, SlabGeom g, CompPtrs in, Cplx<T>* H, const Cplx<T>* tw,
 #define X(n)                                                                          \
   if (L == n) {                                                                       \
     using RT = RowTile<T, n>;                                                         \
-    k1_xr2c<T, n><<<ceil_div(pairs, RT::NTEAM), RT::NT, RT::lds + n * sizeof(Cplx<T>), s>>>(g, in, H, tw); \
+    k1_xr2c<T, n><<<ceil_div(pairs, RT::NTEAM), RT::NT, RT::lds + rfft::tw_lds_size<n>() * sizeof(Cplx<T>), s>>>(g, in, H, tw); \
     SOPHT_CHECK_LAUNCH();                                                             \
     return SOPHT_OK;                                                                  \
   }
@@ -1069,7 +1070,7 @@ int launch_ycols(int64_t L, SlabGeom g, Cplx<T>* H, const Cplx<T>* tw, hipStream
   if (L == n) {                                                                       \
     using CT = ColTile<T, n>;                                                         \
     const int64_t nb = col_tile_blocks(g.nx + 1, int64_t(g.ncomp) * g.nz, CT::WC);    \
-    k_ycols<T, n, INV><<<nb, CT::NT, CT::lds + n * sizeof(Cplx<T>), s>>>(g, H, tw);   \
+    k_ycols<T, n, INV><<<nb, CT::NT, CT::lds + rfft::tw_lds_size<n>() * sizeof(Cplx<T>), s>>>(g, H, tw);   \
     SOPHT_CHECK_LAUNCH();                                                             \
     return SOPHT_OK;                                                                  \
   }
@@ -1104,9 +1105,9 @@ int launch_k5(int64_t L, SlabGeom g, const Cplx<T>* H, CompPtrsW out, const Cplx
   if (L == n) {                                                                       \
     using RT = RowTile<T, n>;                                                         \
     if (acc)                                                                          \
-      k5_xc2r<T, n, true><<<ceil_div(pairs, RT::NTEAM), RT::NT, RT::lds + n * sizeof(Cplx<T>), s>>>(g, H, out, tw); \
+      k5_xc2r<T, n, true><<<ceil_div(pairs, RT::NTEAM), RT::NT, RT::lds + rfft::tw_lds_size<n>() * sizeof(Cplx<T>), s>>>(g, H, out, tw); \
     else                                                                              \
-      k5_xc2r<T, n, false><<<ceil_div(pairs, RT::NTEAM), RT::NT, RT::lds + n * sizeof(Cplx<T>), s>>>(g, H, out, tw); \
+      k5_xc2r<T, n, false><<<ceil_div(pairs, RT::NTEAM), RT::NT, RT::lds + rfft::tw_lds_size<n>() * sizeof(Cplx<T>), s>>>(g, H, out, tw); \
     SOPHT_CHECK_LAUNCH();                                                             \
     return SOPHT_OK;                                                                  \
   }
