@@ -793,7 +793,14 @@ __global__ void __launch_bounds__((XYPass<T, LX, LY>::NT)) kc_yx_inv(SlabGeom g,
 #ifndef SOPHT_ZROW_JOBS
 #define SOPHT_ZROW_JOBS 6
 #endif
+#ifndef SOPHT_ZROW_SLOTS
+#define SOPHT_ZROW_SLOTS 2
+#endif
+#ifndef SOPHT_ZROW_PREFETCH
+#define SOPHT_ZROW_PREFETCH 1
+#endif
 constexpr bool kZrowStream = SOPHT_ZROW_STREAM;
+constexpr bool kZrowPrefetch = SOPHT_ZROW_PREFETCH;
 constexpr int kZrowJobs = SOPHT_ZROW_JOBS;
 
 // The z pass of the three-pass schedule. A workgroup owns one kx tile (WC columns) of the row
@@ -812,15 +819,16 @@ struct ZRowTile {
   // a quarter of a 128-byte line per workgroup (two workgroups per CU); the four quarters of a
   // line go to workgroups b, b + 8, b + 16, b + 24 (the same XCD), so three come from its L2
   static constexpr int PARTS = SOPHT_ZROW_PARTS;
+  static constexpr int SLOTS = SOPHT_ZROW_SLOTS;  // pencils per column transformed at once
   static constexpr int WC = kLineCplx<T> / PARTS;
-  static constexpr int NT = 2 * WC * TT;
+  static constexpr int NT = SLOTS * WC * TT;
   // tiles [slot][column]: the load role's lanes (column, z) hit distinct banks when the
   // column stride is 8 (mod 32) slots
-  static constexpr int S = TT < 32 ? L + 16 : L + 8;
-  static constexpr int KL = E / 4;  // z values per row per thread (z = lr + 2 TT k)
+  static constexpr int S = TT < 32 ? L + 16 : (WC == 4 ? L + 8 : L + 4);
+  static constexpr int KL = E / (2 * SLOTS);  // z values per row per thread (z = lr + SLOTS TT k)
   static constexpr int GL = L / 2 + 1;
-  static constexpr size_t lds = size_t(2 * WC * S) * sizeof(Cplx<T>) + size_t(4 * WC * GL) * sizeof(T);
-  static_assert(E % 4 == 0, "the z rows split in quarters");
+  static constexpr size_t lds = size_t(SLOTS * WC * S) * sizeof(Cplx<T>) + size_t(4 * WC * GL) * sizeof(T);
+  static_assert(E % (2 * SLOTS) == 0 && 4 % SLOTS == 0, "the z rows split among the slots");
 };
 
 template <typename T, int L>
@@ -828,9 +836,10 @@ __global__ void __launch_bounds__((ZRowTile<T, L>::NT)) __attribute__((amdgpu_wa
                                                                   const Cplx<T>* tw) {
   using ZT = ZRowTile<T, L>;
   constexpr int E = ZT::E, TT = ZT::TT, WC = ZT::WC, S = ZT::S, KL = ZT::KL, GL = ZT::GL;
+  constexpr int SL = ZT::SLOTS, ZSTEP = SL * TT;
   extern __shared__ unsigned char smem_raw[];
   Cplx<T>* xb = reinterpret_cast<Cplx<T>*>(smem_raw);  // tiles [slot][column][z]
-  T* const gl = reinterpret_cast<T*>(xb + 2 * WC * S);  // G^ lines [k1][column][kz]
+  T* const gl = reinterpret_cast<T*>(xb + SL * WC * S);  // G^ lines [k1][column][kz]
   const int tid = threadIdx.x;
   const int lw = tid % WC, lr = tid / WC;  // load role: column, first z
   const int team = tid / TT, t = tid % TT;  // transform role: team = slot * WC + column
@@ -896,15 +905,16 @@ __global__ void __launch_bounds__((ZRowTile<T, L>::NT)) __attribute__((amdgpu_wa
       const int colp = row_col(j, y1);
       static_for<0, KL>([&](auto k_) {
         constexpr int k = decltype(k_)::value;
-        pf[y1][k] = live ? H[colp + (lr + 2 * TT * k) * zs] : Cplx<T>{T(0), T(0)};
+        pf[y1][k] = live ? H[colp + (lr + ZSTEP * k) * zs] : Cplx<T>{T(0), T(0)};
       });
     });
   };
   Cplx<T>* const line = xb + team * S;
-  load(0);
+  if constexpr (kZrowPrefetch) load(0);
   __syncthreads();  // G^ lines staged
   for (int j = 0; j < njobs; ++j) {
     const bool mirror = jbeg + j >= g.ncomp;
+    if constexpr (!kZrowPrefetch) load(j);
     Cplx<T> y[4][KL];
     static_for<0, KL>([&](auto k_) {
       constexpr int k = decltype(k_)::value;
@@ -912,18 +922,20 @@ __global__ void __launch_bounds__((ZRowTile<T, L>::NT)) __attribute__((amdgpu_wa
       rfft::dft4<T, false>(u);
       static_for<0, 4>([&](auto q_) { y[decltype(q_)::value][k] = u[decltype(q_)::value]; });
     });
-    if (j + 1 < njobs) load(j + 1);
-    static_for<0, 2>([&](auto r_) {
+    if constexpr (kZrowPrefetch) {
+      if (j + 1 < njobs) load(j + 1);
+    }
+    static_for<0, 4 / SL>([&](auto r_) {
       constexpr int r = decltype(r_)::value;
-      // slot 0 takes pencil 2r (a mirror job: 3 - 2r), slot 1 pencil 2r + 1 (mirror: 2 - 2r)
-      static_for<0, 2>([&](auto s_) {
+      // slot sl takes pencil sl + SL r (a mirror job: 3 - sl - SL r)
+      static_for<0, SL>([&](auto s_) {
         constexpr int sl = decltype(s_)::value;
         static_for<0, KL>([&](auto k_) {
           constexpr int k = decltype(k_)::value;
           // selects on values (a select of two array elements' addresses would leave y in
           // scratch memory)
-          const Cplx<T> a = y[3 - sl - 2 * r][k], b = y[sl + 2 * r][k];
-          xb[(sl * WC + lw) * S + lr + 2 * TT * k] = Cplx<T>{mirror ? a.re : b.re, mirror ? a.im : b.im};
+          const Cplx<T> a = y[3 - sl - SL * r][k], b = y[sl + SL * r][k];
+          xb[(sl * WC + lw) * S + lr + ZSTEP * k] = Cplx<T>{mirror ? a.re : b.re, mirror ? a.im : b.im};
         });
       });
       __syncthreads();
@@ -935,7 +947,7 @@ __global__ void __launch_bounds__((ZRowTile<T, L>::NT)) __attribute__((amdgpu_wa
       });
       rfft::wave_sync();
       if (wave_live) {
-        const T* const gq = gl + ((slot + 2 * r) * WC + tcol) * GL;
+        const T* const gq = gl + ((slot + SL * r) * WC + tcol) * GL;
         if constexpr (last_twiddles_hoisted<L>()) {
           constexpr int R0 = P3::r0, NS3 = R0 * R1;
           rfft::stockham_stage<T, L, R0, 1, false, true>(v, line, tw, t);
@@ -964,13 +976,13 @@ __global__ void __launch_bounds__((ZRowTile<T, L>::NT)) __attribute__((amdgpu_wa
         line[t + TT * e] = v[e];
       });
       __syncthreads();
-      static_for<0, 2>([&](auto s_) {
+      static_for<0, SL>([&](auto s_) {
         constexpr int sl = decltype(s_)::value;
         static_for<0, KL>([&](auto k_) {
           constexpr int k = decltype(k_)::value;
-          const Cplx<T> val = xb[(sl * WC + lw) * S + lr + 2 * TT * k];
-          Cplx<T>& a = y[3 - sl - 2 * r][k];
-          Cplx<T>& b = y[sl + 2 * r][k];
+          const Cplx<T> val = xb[(sl * WC + lw) * S + lr + ZSTEP * k];
+          Cplx<T>& a = y[3 - sl - SL * r][k];
+          Cplx<T>& b = y[sl + SL * r][k];
           a = Cplx<T>{mirror ? val.re : a.re, mirror ? val.im : a.im};
           b = Cplx<T>{mirror ? b.re : val.re, mirror ? b.im : val.im};
         });
@@ -984,8 +996,8 @@ __global__ void __launch_bounds__((ZRowTile<T, L>::NT)) __attribute__((amdgpu_wa
         rfft::dft4<T, true>(u);
         static_for<0, 4>([&](auto q_) {
           constexpr int y1 = decltype(q_)::value;
-          if constexpr (kZrowStream) store_stream(H + row_col(j, y1) + (lr + 2 * TT * k) * zs, u[y1]);
-          else H[row_col(j, y1) + (lr + 2 * TT * k) * zs] = u[y1];
+          if constexpr (kZrowStream) store_stream(H + row_col(j, y1) + (lr + ZSTEP * k) * zs, u[y1]);
+          else H[row_col(j, y1) + (lr + ZSTEP * k) * zs] = u[y1];
         });
       });
     }
