@@ -25,8 +25,9 @@ against forward with checkpoints plus the adjoint sweep, 128^3, 200 steps) and
 "fast_diagonalisation" (the Neumann fast-diagonalisation vector solve at 256^3 FP32 against
 the FP32 MFMA peak, the flow step with that solver, and the numpy oracle of the solve),
 "rod_coupling" (the flexible-filament driver loop at 128^3 with a heaving rod: rod substeps with
-the flow forces, one spreading per flow step) and "other_sizes" (the sphere workload at 192^3
-fp32 and fp64 and at 256^3 fp64, each with its roofline).
+the flow forces, one spreading per flow step), "other_sizes" (the sphere workload at 192^3
+fp32 and fp64 and at 256^3 fp64, each with its roofline) and "filtered_step" (the 256^3 step
+with the order-5 convolution vorticity filter of the flexible-filament example).
 """
 
 from __future__ import annotations
@@ -84,7 +85,7 @@ def hbm_traffic(stage):
     return None
 
 
-def build(n, real_t, seed=42, n_lag=3000, radius=0.08, nu=5e-4, K=8):
+def build(n, real_t, seed=42, n_lag=3000, radius=0.08, nu=5e-4, K=8, flow_kw=None):
     from sopht_amd import simulator as sps
     from sopht_amd.utils import synthetic
 
@@ -92,7 +93,7 @@ def build(n, real_t, seed=42, n_lag=3000, radius=0.08, nu=5e-4, K=8):
 
     flow = sps.UnboundedNavierStokesFlowSimulator3D(
         grid_size=(n, n, n), x_range=1.0, kinematic_viscosity=nu, cfl=0.1, real_t=real_t,
-        with_forcing=True, with_free_stream_flow=True,
+        with_forcing=True, with_free_stream_flow=True, **(flow_kw or {}),
     )
     w0 = synthetic.random_phase_vorticity((n, n, n), 1.0, K, seed, 10.0)
     flow.vorticity_field.copy_(torch.from_numpy(w0.astype(real_t)))
@@ -360,17 +361,17 @@ def rod_coupled(n=128, steps=20, warmup=2):
     }
 
 
-def plain_rate(n, real_t, steps=20, warmup=3):
+def plain_rate(n, real_t, steps=20, warmup=3, flow_kw=None):
     """The coupled sphere loop at another size or precision (the 192^3 / 256^3 workload
-    family): ms per step, cell-updates/s and the step's fraction of the 76-word HBM model,
-    with the dominant kernel's roofline."""
+    family), or with other simulator options (flow_kw): ms per step, cell-updates/s and the
+    step's fraction of the 76-word HBM model, with the dominant kernel's roofline."""
     import ctypes as C
 
     import torch
 
     from sopht_amd import _lib as L
 
-    flow, inter, U = build(n, real_t)
+    flow, inter, U = build(n, real_t, flow_kw=flow_kw)
     for _ in range(warmup):
         step(flow, inter, U)
     lib = L.lib()
@@ -389,6 +390,10 @@ def plain_rate(n, real_t, steps=20, warmup=3):
     avg = {names[i]: ms[i] / max(cnt.value, 1) for i in range(nst)}
     word = np.dtype(real_t).itemsize
     words = stage_words(n, n, n)
+    if flow_kw and flow_kw.get("filter_vorticity"):
+        # the filter runs inside the second stage: order-k convolution, one read and one write
+        # of the three components per axis (penalisation planes neglected)
+        words["vorticity_diffuse_penalise"] += 3 * 6
     dom = max(avg, key=avg.get)
     gbs = words[dom] * n**3 * word / (avg[dom] * 1e-3) / 1e9
     ms_step = 1e3 * el / steps
@@ -481,6 +486,19 @@ def fastdiag_solve(n=256, reps=20, cpu=True):
     return result
 
 
+FILTER_WORKLOAD = ("coupled IB flow step, 256^3 fp32, with the vorticity filter of the flexible-filament "
+                   "example (order 5, convolution)")
+
+
+def filtered_step(steps):
+    """The 256^3 FP32 sphere step with the Laplacian vorticity filter on, order 5
+    convolution (flow_past_rod_case.py:103-104): ms per step against the same step without it."""
+    kw = {"filter_vorticity": True, "filter_setting_dict": {"order": 5, "type": "convolution"}}
+    out = plain_rate(256, np.float32, steps, flow_kw=kw)
+    out["workload"] = FILTER_WORKLOAD
+    return out
+
+
 def other_sizes(steps):
     """The sphere workload at 192^3 (fp32, fp64) and 256^3 fp64."""
     import torch
@@ -552,6 +570,9 @@ def main():
     ap.add_argument("--sizes-steps", type=int, default=20,
                     help="steps of the 192^3 fp32/fp64 and 256^3 fp64 lines (0 = skip)")
     ap.add_argument("--sizes-only", action="store_true", help="run only the other sizes and precisions")
+    ap.add_argument("--filter-steps", type=int, default=20,
+                    help="steps of the 256^3 step with the order-5 convolution filter (0 = skip)")
+    ap.add_argument("--filter-only", action="store_true", help="run only the filtered step")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -585,6 +606,12 @@ def main():
 
         torch.cuda.set_device(0)
         print(json.dumps(other_sizes(max(1, args.sizes_steps))))
+        return 0
+    if args.filter_only:
+        import torch
+
+        torch.cuda.set_device(0)
+        print(json.dumps(filtered_step(max(1, args.filter_steps))))
         return 0
     if args.batch_only:
         import torch
@@ -670,6 +697,10 @@ def main():
     if args.sizes_steps > 0:
         torch.cuda.empty_cache()
         sizes = other_sizes(args.sizes_steps)
+    filtered = None
+    if args.filter_steps > 0:
+        torch.cuda.empty_cache()
+        filtered = filtered_step(args.filter_steps)
     cpu = None
     if args.cpu_steps > 0:
         try:
@@ -705,6 +736,7 @@ def main():
         "fast_diagonalisation": fastdiag,
         "rod_coupling": rod,
         "other_sizes": sizes,
+        "filtered_step": filtered,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -45,7 +45,9 @@ struct FilterComps {
   T* f[3];  // the filtered field (kMultFinal: f <- f - P in)
 };
 
-// x passes. Row r = blockIdx.x * 4 + wave of component blockIdx.y; VEC chunks of 64 cells.
+// x passes. Row r = blockIdx.x * 4 + wave of component blockIdx.y; lane l holds the VEC
+// consecutive cells x = VEC l + i, so a pass exchanges only the two cells at the lane's edges
+// with its neighbours. Rows of a multiple of 4 cells load and store 16 bytes per access.
 template <typename T, int VEC, int MODE>
 __global__ void __launch_bounds__(256) k_filter_rows(FilterComps<T> c, int nz, int ny, int nx, int order) {
   const int lane = threadIdx.x & 63;
@@ -57,51 +59,87 @@ __global__ void __launch_bounds__(256) k_filter_rows(FilterComps<T> c, int nz, i
   const int64_t base = row * nx;
   if (MODE == kConv && !line_in) return;  // the flux is 0 along the whole row
   const T* in = c.in[comp] + base;
+  const int x0 = VEC * lane;
+  // 16-byte accesses need 4 | nx (the row base is then 16-byte aligned) and 4 | VEC
+  const bool vec4 = sizeof(T) == 4 && VEC % 4 == 0 && nx % 4 == 0;
   T g[VEC];
+  if (vec4) {
+    typedef T quad __attribute__((ext_vector_type(4)));
 #pragma unroll
-  for (int v = 0; v < VEC; ++v) {
-    const int x = v * 64 + lane;
-    g[v] = x < nx ? in[x] : T(0);
+    for (int i = 0; i < VEC; i += 4) {
+      if (x0 + i < nx) {
+        const quad q = *reinterpret_cast<const quad*>(in + x0 + i);
+        g[i] = q.x; g[i + 1] = q.y; g[i + 2] = q.z; g[i + 3] = q.w;
+      } else {
+        g[i] = g[i + 1] = g[i + 2] = g[i + 3] = T(0);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < VEC; ++i) g[i] = x0 + i < nx ? in[x0 + i] : T(0);
   }
-  const int passes = MODE == kConv ? order : 1;
-  T h[VEC];
-  for (int p = 0; p < passes; ++p) {
+  T g0[VEC];  // the row's input (kConv updates it in place: no second read)
 #pragma unroll
-    for (int v = 0; v < VEC; ++v) {
-      const int x = v * 64 + lane;
-      // neighbours x - 1 and x + 1: the adjacent lane, or across the 64-cell chunk edge
-      T left = __shfl_up(g[v], 1, 64);
-      T right = __shfl_down(g[v], 1, 64);
-      if (v > 0) {
-        const T prev_last = __shfl(g[v > 0 ? v - 1 : 0], 63, 64);
-        if (lane == 0) left = prev_last;
-      }
-      if (v < VEC - 1) {
-        const T next_first = __shfl(g[v < VEC - 1 ? v + 1 : 0], 0, 64);
-        if (lane == 63) right = next_first;
-      }
-      h[v] = (line_in && x > 0 && x < nx - 1) ? lap_pass(left, g[v], right) : T(0);
+  for (int i = 0; i < VEC; ++i) g0[i] = g[i];
+  const int passes = MODE == kConv ? order : 1;
+  for (int p = 0; p < passes; ++p) {
+    // x0 - 1 from the lane below (its last cell), x0 + VEC from the lane above (its first)
+    const T left = __shfl_up(g[VEC - 1], 1, 64);
+    const T right = __shfl_down(g[0], 1, 64);
+    T h[VEC];
+#pragma unroll
+    for (int i = 0; i < VEC; ++i) {
+      const int x = x0 + i;
+      const T l = i > 0 ? g[i > 0 ? i - 1 : 0] : left;
+      const T r = i < VEC - 1 ? g[i < VEC - 1 ? i + 1 : 0] : right;
+      h[i] = (line_in && x > 0 && x < nx - 1) ? lap_pass(l, g[i], r) : T(0);
     }
 #pragma unroll
-    for (int v = 0; v < VEC; ++v) g[v] = h[v];
+    for (int i = 0; i < VEC; ++i) g[i] = h[i];
   }
+  if (MODE == kMultPass) {
+    T* o = c.out[comp] + base;
 #pragma unroll
-  for (int v = 0; v < VEC; ++v) {
-    const int x = v * 64 + lane;
-    if (x >= nx) continue;
-    if (MODE == kMultPass) {
-      c.out[comp][base + x] = g[v];
+    for (int i = 0; i < VEC; ++i)
+      if (x0 + i < nx) o[x0 + i] = g[i];
+  } else {
+    T* f = (MODE == kConv ? c.out[comp] : c.f[comp]) + base;
+    if (vec4) {
+      typedef T quad __attribute__((ext_vector_type(4)));
+#pragma unroll
+      for (int i = 0; i < VEC; i += 4) {
+        if (x0 + i < nx) {
+          quad q;
+          if (MODE == kConv) {
+            q.x = g0[i]; q.y = g0[i + 1]; q.z = g0[i + 2]; q.w = g0[i + 3];
+          } else {
+            q = *reinterpret_cast<const quad*>(f + x0 + i);
+          }
+          q.x = T(1) * q.x + T(-1) * g[i];
+          q.y = T(1) * q.y + T(-1) * g[i + 1];
+          q.z = T(1) * q.z + T(-1) * g[i + 2];
+          q.w = T(1) * q.w + T(-1) * g[i + 3];
+          *reinterpret_cast<quad*>(f + x0 + i) = q;
+        }
+      }
     } else {
-      T* f = (MODE == kConv ? c.out[comp] : c.f[comp]) + base;
-      f[x] = T(1) * f[x] + T(-1) * g[v];
+#pragma unroll
+      for (int i = 0; i < VEC; ++i)
+        if (x0 + i < nx) f[x0 + i] = T(1) * (MODE == kConv ? g0[i] : f[x0 + i]) + T(-1) * g[i];
     }
   }
 }
 
 // y (AXIS 1) or z (AXIS 2) passes: one thread per line (the other two coordinates), lanes
-// along x for coalesced accesses; K passes pipelined along the line.
+// along x for coalesced accesses; K passes pipelined along the line. The line is read in
+// batches of kMarchBatch values, all in flight together; the convolution keeps the last K + 1
+// inputs in registers for its update f - P^K f (the value at q = p - K is still the input
+// there: this thread writes the line only behind its reads).
+constexpr int kMarchBatch = 8;
+
 template <typename T, int AXIS, int K, int MODE>
 __global__ void __launch_bounds__(256) k_filter_march(FilterComps<T> c, int nz, int ny, int nx) {
+  constexpr int D = kMarchBatch;
   const int comp = blockIdx.z;
   const int x = blockIdx.x * 256 + threadIdx.x;
   const int other = blockIdx.y;  // z for the y march, y for the z march
@@ -118,26 +156,51 @@ __global__ void __launch_bounds__(256) k_filter_march(FilterComps<T> c, int nz, 
   T w[K + 1][3];
 #pragma unroll
   for (int j = 0; j <= K; ++j) w[j][0] = w[j][1] = w[j][2] = T(0);
-  for (int p = 0; p < n + K; ++p) {
-    w[0][0] = w[0][1];
-    w[0][1] = w[0][2];
-    w[0][2] = p < n ? in[p * stride] : T(0);
+  T hist[K + 1];  // inputs at p - K .. p (kConv)
 #pragma unroll
-    for (int j = 1; j <= K; ++j) {
-      const int q = p - j;  // pass j's position this iteration; its inputs q - 1, q, q + 1
-      T v = T(0);
-      if (line_in && q > 0 && q < n - 1) v = lap_pass(w[j - 1][0], w[j - 1][1], w[j - 1][2]);
-      w[j][0] = w[j][1];
-      w[j][1] = w[j][2];
-      w[j][2] = v;
+  for (int j = 0; j <= K; ++j) hist[j] = T(0);
+  for (int p0 = 0; p0 < n + K; p0 += D) {
+    T buf[D], fq[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const int p = p0 + d;
+      buf[d] = p < n ? in[p * stride] : T(0);
     }
-    const int q = p - K;
-    if (q >= 0) {
-      const T flux = w[K][2];
-      if (MODE == kMultPass) {
-        out[q * stride] = flux;
-      } else {
-        out[q * stride] = T(1) * out[q * stride] + T(-1) * flux;
+    if (MODE == kMultFinal) {
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        const int q = p0 + d - K;
+        fq[d] = (q >= 0 && q < n) ? out[q * stride] : T(0);
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const int p = p0 + d;
+      w[0][0] = w[0][1];
+      w[0][1] = w[0][2];
+      w[0][2] = buf[d];
+#pragma unroll
+      for (int j = 0; j < K; ++j) hist[j] = hist[j + 1];
+      hist[K] = buf[d];
+#pragma unroll
+      for (int j = 1; j <= K; ++j) {
+        const int q = p - j;  // pass j's position this iteration; its inputs q - 1, q, q + 1
+        T v = T(0);
+        if (line_in && q > 0 && q < n - 1) v = lap_pass(w[j - 1][0], w[j - 1][1], w[j - 1][2]);
+        w[j][0] = w[j][1];
+        w[j][1] = w[j][2];
+        w[j][2] = v;
+      }
+      const int q = p - K;
+      if (q >= 0 && q < n) {
+        const T flux = w[K][2];
+        if (MODE == kMultPass) {
+          out[q * stride] = flux;
+        } else if (MODE == kConv) {
+          out[q * stride] = T(1) * hist[0] + T(-1) * flux;
+        } else {
+          out[q * stride] = T(1) * fq[d] + T(-1) * flux;
+        }
       }
     }
   }

@@ -62,6 +62,49 @@ int launch_boundary_slabs(int64_t nz, int64_t ny, int64_t nx, int64_t w, Fn fn, 
   return launch_box(0, 0, nx - wx, nz, ny, wx, fn, s);
 }
 
+// The surface cells of the box [z0, z0 + bz) x [y0, y0 + by) x [x0, x0 + bx) (bz, by, bx >= 2),
+// one thread each: the two z faces, then the two y faces between them, then the two x faces.
+template <typename Fn>
+__global__ void __launch_bounds__(256) shell_kernel(int64_t z0, int64_t y0, int64_t x0, int64_t bz,
+                                                    int64_t by, int64_t bx, Fn fn) {
+  const int64_t nzf = by * bx, nyf = (bz - 2) * bx, nxf = (bz - 2) * (by - 2);
+  const int64_t total = 2 * (nzf + nyf + nxf);
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < total;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    int64_t z, y, x, r = i;
+    if (r < 2 * nzf) {
+      z = z0 + (r < nzf ? 0 : bz - 1);
+      r %= nzf;
+      y = y0 + r / bx;
+      x = x0 + r % bx;
+    } else if ((r -= 2 * nzf) < 2 * nyf) {
+      y = y0 + (r < nyf ? 0 : by - 1);
+      r %= nyf;
+      z = z0 + 1 + r / bx;
+      x = x0 + r % bx;
+    } else {
+      r -= 2 * nyf;
+      x = x0 + (r < nxf ? 0 : bx - 1);
+      r %= nxf;
+      z = z0 + 1 + r / (by - 2);
+      y = y0 + 1 + r % (by - 2);
+    }
+    fn(z, y, x);
+  }
+}
+
+template <typename Fn>
+int launch_shell(int64_t z0, int64_t y0, int64_t x0, int64_t bz, int64_t by, int64_t bx, Fn fn,
+                 hipStream_t s) {
+  if (bz < 2 || by < 2 || bx < 2) return launch_box(z0, y0, x0, bz, by, bx, fn, s);
+  const int64_t total = 2 * (by * bx + (bz - 2) * bx + (bz - 2) * (by - 2));
+  int blocks = ceil_div(total, 256);
+  if (blocks > 65536) blocks = 65536;
+  shell_kernel<<<blocks, 256, 0, s>>>(z0, y0, x0, bz, by, bx, fn);
+  SOPHT_CHECK_LAUNCH();
+  return SOPHT_OK;
+}
+
 inline hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
 
 // ------------------------------------------------------------------ curl-type operators
@@ -222,7 +265,8 @@ int penalise_impl(void* field, const int64_t* fs, int64_t nz, int64_t ny, int64_
   View3<T> f = make_view<T>(field, fs);
   const int64_t cz0 = w - 1, cz1 = nz - w, cy0 = w - 1, cy1 = ny - w, cx0 = w - 1, cx1 = nx - w;
   const int64_t bz = cz1 - cz0 + 1, by = cy1 - cy0 + 1, bx = cx1 - cx0 + 1;
-  return launch_box(
+  // only the surface of the core box [w-1, n-w]^3 has work: one thread per surface cell
+  return launch_shell(
       cz0, cy0, cx0, bz, by, bx,
       [=] __device__(int64_t z, int64_t y, int64_t x) {
         const bool onz = (z == cz0) || (z == cz1), ony = (y == cy0) || (y == cy1),
