@@ -208,6 +208,7 @@ def batched_vs_sequential(steps, seq_steps, warmup=2):
     torch.cuda.synchronize()
     seq = time.perf_counter() - t0
     sim = BatchedFlowSimulation(members)
+    schedules = sim.poisson_schedules()
     sim.step(1 + warmup)
     sim.synchronize()
     torch.cuda.synchronize()
@@ -229,6 +230,7 @@ def batched_vs_sequential(steps, seq_steps, warmup=2):
         "speedup": round(bat_rate / seq_rate, 2),
         "steps": steps,
         "sequential_steps": seq_steps,
+        "poisson_schedules": schedules,
     }
 
 
@@ -647,6 +649,7 @@ def main():
     nst = lib.sopht_ns3d_stage_times(flow._plan, ms, 16, C.byref(cnt), 0)
     names = [lib.sopht_ns3d_stage_name(flow._plan, i).decode() for i in range(nst)]
     avg_ms = {names[i]: ms[i] / max(cnt.value, 1) for i in range(nst)}
+    passes, zjobs = L.schedule("sopht_ns3d_poisson_schedule", flow._plan)
     inter.check_support_in_domain()
     if not torch.isfinite(flow.vorticity_field).all():
         print("non-finite vorticity after the timed steps", file=sys.stderr)
@@ -730,6 +733,7 @@ def main():
         "step_hbm_frac_76w": round(step_gbs / HBM_PEAK_GBS, 4),
         "stage_ms": {k: round(v, 4) for k, v in avg_ms.items()},
         "stage_gbs": kernels_gbs,
+        "poisson_schedule": {"passes": passes, "zpass_jobs": zjobs},
         "roofline": roof,
         "batched": batched,
         "adjoint": adjoint,

@@ -182,6 +182,12 @@ int sopht_poisson3d_vector_solve(sopht_poisson3d* plan, void* const solution[3],
 int sopht_poisson3d_destroy(sopht_poisson3d* plan);
 /* Bytes of device scratch the plan holds (for capacity planning). */
 int64_t sopht_poisson3d_scratch_bytes(const sopht_poisson3d* plan);
+/* The plan's FFT schedule (no counterpart in the reference, which leaves it to FFTW's
+ * planner): *passes is 3 (x transform fused with the inner y sums; FP32, nx = ny) or 5, and
+ * *zpass_jobs the jobs per workgroup of the three-pass z pass, timed over {1, 2, 3, 6} when
+ * the plan was created (SOPHT_POISSON_ZJOBS in the environment fixes it; 0 with five passes).
+ * Results do not depend on it. Either pointer may be null. */
+int sopht_poisson3d_schedule(const sopht_poisson3d* plan, int* passes, int* zpass_jobs);
 
 /* ---------------------------------------------------------------- Neumann Poisson (fast diagonalisation) */
 
@@ -353,6 +359,9 @@ int sopht_ns3d_last_velocity_max(sopht_ns3d* plan, const void* velocity, void* o
  * plan (its Poisson solver sets how many stages a step has). */
 int sopht_ns3d_enable_timing(sopht_ns3d* plan, int enable);
 int sopht_ns3d_stage_times(sopht_ns3d* plan, double* ms, int n, int64_t* steps, int reset);
+/* sopht_poisson3d_schedule of the step's Poisson plan (0 and 0 with the fast-diagonalisation
+ * solver; no counterpart in the reference). */
+int sopht_ns3d_poisson_schedule(const sopht_ns3d* plan, int* passes, int* zpass_jobs);
 const char* sopht_ns3d_stage_name(const sopht_ns3d* plan, int stage);
 
 /* ---- batched step (no counterpart in the reference: many independent simulators, the
@@ -393,6 +402,11 @@ int sopht_ns3d_batch_step(sopht_ns3d_batch* batch, int steps, int use_graph, voi
 /* Synchronises and reads each member's last dt and its time (either pointer may be NULL). */
 int sopht_ns3d_batch_state(sopht_ns3d_batch* batch, double* dt, double* time, void* stream);
 int sopht_ns3d_batch_destroy(sopht_ns3d_batch* batch);
+/* The batch's size classes (one Poisson plan each; no counterpart in the reference): returns
+ * their number; for 0 <= i < that number also writes class i's grid (nz, ny, nx), its
+ * component count and its plan's schedule (sopht_poisson3d_schedule). Pointers may be null. */
+int sopht_ns3d_batch_class_schedule(const sopht_ns3d_batch* batch, int i, int64_t* shape, int* ncomp,
+                                    int* passes, int* zpass_jobs);
 
 /* ---- reverse-mode adjoint of the coupled step (no counterpart in the reference, which has
  * no differentiation; the step it differentiates is navier_stokes_flow_simulators.py:449-498

@@ -31,6 +31,7 @@ _d = C.c_double
 _p3 = _vp * 3
 _pi64 = C.POINTER(C.c_int64)
 _pd = C.POINTER(C.c_double)
+_pi = C.POINTER(C.c_int)
 
 
 
@@ -82,6 +83,7 @@ SIGNATURES = {
     "sopht_poisson3d_vector_solve": [_vp, _p3, _pi64, _p3, _pi64, _vp],
     "sopht_poisson3d_destroy": [_vp],
     "sopht_poisson3d_scratch_bytes": [_vp],
+    "sopht_poisson3d_schedule": [_vp, _pi, _pi],
     "sopht_fastdiag3d_create": [_i, _i64, _i64, _i64, _d, _vp, C.POINTER(_vp)],
     "sopht_fastdiag3d_solve": [_vp, _vp, _pi64, _vp, _pi64, _vp],
     "sopht_fastdiag3d_vector_solve": [_vp, _p3, _pi64, _p3, _pi64, _vp],
@@ -112,6 +114,8 @@ SIGNATURES = {
     "sopht_ns3d_enable_timing": [_vp, _i],
     "sopht_ns3d_stage_times": [_vp, _pd, _i, _pi64, _i],
     "sopht_ns3d_stage_name": [_vp, _i],
+    "sopht_ns3d_poisson_schedule": [_vp, _pi, _pi],
+    "sopht_ns3d_batch_class_schedule": [_vp, _i, _pi64, _pi, _pi, _pi],
     "sopht_ns3d_batch_create": [_i, _i, C.POINTER(BatchMember), _vp, C.POINTER(_vp)],
     "sopht_ns3d_batch_step": [_vp, _i, _i, _vp],
     "sopht_ns3d_batch_state": [_vp, _pd, _pd, _vp],
@@ -231,3 +235,11 @@ def written(*tensors) -> None:
     live = [t for t in tensors if t is not None]
     if live:
         torch.autograd.graph.increment_version(live)
+
+
+def schedule(fn: str, handle) -> tuple[int, int]:
+    """(passes, zpass_jobs) of a Poisson plan (``sopht_poisson3d_schedule``) or of a step plan
+    (``sopht_ns3d_poisson_schedule``)."""
+    passes, jobs = C.c_int(0), C.c_int(0)
+    check(getattr(lib(), fn)(handle, C.byref(passes), C.byref(jobs)))
+    return passes.value, jobs.value

@@ -241,6 +241,7 @@ using namespace sopht;
 // one Poisson plan per distinct grid of the batch, with device tables of its members' fields
 struct BatchClass {
   sopht_poisson3d* plan = nullptr;
+  int64_t shape[3] = {0, 0, 0};  // (nz, ny, nx)
   int ncomp = 0;
   void** sol_tab = nullptr;  // device: psi components
   void** rhs_tab = nullptr;  // device: omega components
@@ -391,6 +392,9 @@ int batch_build(sopht_ns3d_batch* B, const sopht_batch_member* mem, hipStream_t 
     const sopht_batch_member& q = mem[ms[0]];
     BatchClass C;
     C.ncomp = 3 * int(ms.size());
+    C.shape[0] = q.nz;
+    C.shape[1] = q.ny;
+    C.shape[2] = q.nx;
     int rc = poisson_create(B->dtype, q.nz, q.ny, q.nx, q.x_range, C.ncomp, s, &C.plan);
     if (rc) return rc;
     std::vector<void*> sol(C.ncomp), rhs(C.ncomp);
@@ -582,6 +586,19 @@ int sopht_ns3d_batch_state(sopht_ns3d_batch* B, double* dt, double* time, void* 
     }
   }
   return SOPHT_OK;
+}
+
+int sopht_ns3d_batch_class_schedule(const sopht_ns3d_batch* B, int i, int64_t* shape, int* ncomp,
+                                    int* passes, int* zpass_jobs) {
+  SOPHT_REQUIRE(B != nullptr, "batch is null");
+  const int n = int(B->classes.size());
+  if (i < 0 || i >= n) return n;
+  const BatchClass& cl = B->classes[size_t(i)];
+  if (shape)
+    for (int a = 0; a < 3; ++a) shape[a] = cl.shape[a];
+  if (ncomp) *ncomp = cl.ncomp;
+  const int rc = sopht_poisson3d_schedule(cl.plan, passes, zpass_jobs);
+  return rc ? rc : n;
 }
 
 int sopht_ns3d_batch_destroy(sopht_ns3d_batch* B) {

@@ -266,6 +266,16 @@ int sopht_ns3d_stage_times(sopht_ns3d* P, double* ms, int n, int64_t* steps, int
   return ns;
 }
 
+int sopht_ns3d_poisson_schedule(const sopht_ns3d* P, int* passes, int* zpass_jobs) {
+  SOPHT_REQUIRE(P != nullptr, "plan is null");
+  if (!P->poisson) {  // the fast-diagonalisation solver
+    if (passes) *passes = 0;
+    if (zpass_jobs) *zpass_jobs = 0;
+    return SOPHT_OK;
+  }
+  return sopht_poisson3d_schedule(P->poisson, passes, zpass_jobs);
+}
+
 const char* sopht_ns3d_stage_name(const sopht_ns3d* P, int i) {
   return (P && i >= 0 && i < stage_count(P)) ? stage_names(P)[i] : "";
 }

@@ -34,6 +34,7 @@ struct sopht_poisson3d {
   int slab_comps;   // components the slab holds (3, or 3 per instance of a batched class)
   int64_t slab_bytes, ghat_bytes;
   int passes;       // 3: x and y passes fused (three-pass schedule), or 5
+  int zjobs;        // three-pass z pass: jobs per workgroup, chosen at plan creation
 };
 
 namespace sopht {
@@ -833,7 +834,7 @@ struct ZRowTile {
 
 template <typename T, int L>
 __global__ void __launch_bounds__((ZRowTile<T, L>::NT)) __attribute__((amdgpu_waves_per_eu(kZconvWavesFor<T, L>))) k3_zconv4(SlabGeom g, Cplx<T>* H, const T* ghat,
-                                                                  const Cplx<T>* tw) {
+                                                                  const Cplx<T>* tw, int jobs) {
   using ZT = ZRowTile<T, L>;
   constexpr int E = ZT::E, TT = ZT::TT, WC = ZT::WC, S = ZT::S, KL = ZT::KL, GL = ZT::GL;
   constexpr int SL = ZT::SLOTS, ZSTEP = SL * TT;
@@ -850,9 +851,9 @@ __global__ void __launch_bounds__((ZRowTile<T, L>::NT)) __attribute__((amdgpu_wa
   if (!col_tile_parts<ZT::PARTS>(blockIdx.x, ncols, L2 / 2 + 1, WC, k2, kx0)) return;
   const bool self = k2 == 0 || 2 * k2 == L2;
   const int alljobs = self ? g.ncomp : 2 * g.ncomp;
-  const int jbeg = int(blockIdx.y) * kZrowJobs;
+  const int jbeg = int(blockIdx.y) * jobs;
   if (jbeg >= alljobs) return;
-  const int njobs = alljobs - jbeg < kZrowJobs ? alljobs - jbeg : kZrowJobs;
+  const int njobs = alljobs - jbeg < jobs ? alljobs - jbeg : jobs;
   const int64_t kx = kx0 + lw;
   const bool live = kx < ncols;
   const int first_col = 64 / TT >= WC ? 0 : ((tid / 64) * (64 / TT)) % WC;
@@ -1130,7 +1131,7 @@ int launch_k5(int64_t L, SlabGeom g, const Cplx<T>* H, CompPtrsW out, const Cplx
 
 // Line lengths 2 nx = 2 ny of the three-pass schedule, and 2 nz (longer z lines leave the
 // z pass without registers for its next rows).
-#define SOPHT_XY_LENGTHS(X) X(128) X(256) X(384) X(512)
+#define SOPHT_XY_LENGTHS(X) X(96) X(128) X(192) X(256) X(384) X(512)
 #define SOPHT_ZROW_LENGTHS(X) \
   X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512)
 
@@ -1201,15 +1202,15 @@ int launch_yx_inv(int64_t L, SlabGeom g, const Cplx<T>* H, CompPtrsW out, const 
 
 template <typename T>
 int launch_k3_rows(int64_t L, SlabGeom g, Cplx<T>* H, const T* ghat, const Cplx<T>* tw,
-                   hipStream_t s) {
+                   hipStream_t s, int jobs) {
 #define X(n)                                                                          \
   if (L == n) {                                                                       \
     using ZT = ZRowTile<T, n>;                                                        \
     const int64_t nb = col_tile_parts_blocks(ZT::PARTS, g.nx + 1, g.ny / 4 + 1, ZT::WC); \
-    const unsigned nch = unsigned((2 * g.ncomp + kZrowJobs - 1) / kZrowJobs);          \
+    const unsigned nch = unsigned((2 * g.ncomp + jobs - 1) / jobs);                    \
     static const int lds_ok = allow_lds(k3_zconv4<T, n>, ZT::lds);                    \
     if (lds_ok) return lds_ok;                                                        \
-    k3_zconv4<T, n><<<dim3(unsigned(nb), nch), ZT::NT, ZT::lds, s>>>(g, H, ghat, tw);  \
+    k3_zconv4<T, n><<<dim3(unsigned(nb), nch), ZT::NT, ZT::lds, s>>>(g, H, ghat, tw, jobs); \
     SOPHT_CHECK_LAUNCH();                                                             \
     return SOPHT_OK;                                                                  \
   }
@@ -1258,7 +1259,8 @@ int solve_impl(sopht_poisson3d* P, int ncomp, void* const sol[3], const int64_t*
     if (P->passes == 3) {
       if ((rc = launch_xy_fwd<T>(2 * P->nx, g, in, H, twx, twy, s))) return rc;
       mark(0);
-      if ((rc = launch_k3_rows<T>(2 * P->nz, g, H, static_cast<const T*>(P->ghat), twz, s))) return rc;
+      if ((rc = launch_k3_rows<T>(2 * P->nz, g, H, static_cast<const T*>(P->ghat), twz, s, P->zjobs)))
+        return rc;
       mark(1);
       rc = launch_yx_inv<T>(2 * P->nx, g, H, out, twx, twy, s, accumulate);
       mark(2);
@@ -1344,6 +1346,55 @@ int build_ghat(sopht_poisson3d* P, hipStream_t s) {
   SOPHT_CHECK_HIP(hipFree(b));
   SOPHT_CHECK_HIP(hipFree(ct));
   return SOPHT_OK;
+}
+
+// Jobs per workgroup of the three-pass z pass: the fewer, the more workgroups (and the more
+// G^ staging). Each candidate is timed on the plan's own slab (zeroed) and stream, and the
+// fastest kept; SOPHT_POISSON_ZJOBS in the environment fixes the choice instead. Results do
+// not depend on it: a pencil's arithmetic is the same in any workgroup.
+int tune_zpass(sopht_poisson3d* P, hipStream_t s) {
+  P->zjobs = kZrowJobs;
+  if (P->passes != 3 || P->dtype != SOPHT_F32) return SOPHT_OK;
+  if (const char* forced = std::getenv("SOPHT_POISSON_ZJOBS")) {
+    const int j = std::atoi(forced);
+    if (j >= 1) P->zjobs = j;
+    return SOPHT_OK;
+  }
+  const SlabGeom g = slab_geom<float>(P, P->slab_comps);
+  Cplx<float>* H = static_cast<Cplx<float>*>(P->slab);
+  const float* gh = static_cast<const float*>(P->ghat);
+  const Cplx<float>* twz = static_cast<const Cplx<float>*>(P->tw[2]);
+  SOPHT_CHECK_HIP(hipMemsetAsync(P->slab, 0, size_t(P->slab_bytes), s));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  SOPHT_CHECK_HIP(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    set_last_error("hipEventCreate failed", __FILE__, __LINE__);
+    return SOPHT_ERR_HIP;
+  }
+  int rc = SOPHT_OK;
+  float best = -1.f;
+  for (const int jobs : {1, 2, 3, 6}) {
+    if (jobs > 2 * g.ncomp) continue;
+    if ((rc = launch_k3_rows<float>(2 * P->nz, g, H, gh, twz, s, jobs))) break;  // warm-up
+    if (hipEventRecord(e0, s) != hipSuccess) { rc = SOPHT_ERR_HIP; break; }
+    for (int r = 0; r < 3 && rc == SOPHT_OK; ++r) rc = launch_k3_rows<float>(2 * P->nz, g, H, gh, twz, s, jobs);
+    if (rc) break;
+    float ms = 0.f;
+    if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+      rc = SOPHT_ERR_HIP;
+      break;
+    }
+    if (best < 0.f || ms < best) {
+      best = ms;
+      P->zjobs = jobs;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc == SOPHT_ERR_HIP) set_last_error("timing the z pass at plan creation failed", __FILE__, __LINE__);
+  return rc;
 }
 
 }  // namespace
@@ -1445,6 +1496,7 @@ int poisson_create(int dtype, int64_t nz, int64_t ny, int64_t nx, double x_range
     if (rc) return fail(rc);
   }
   if ((rc = build_ghat(P, s))) return fail(rc);
+  if ((rc = tune_zpass(P, s))) return fail(rc);
   *out = P;
   return SOPHT_OK;
 }
@@ -1465,6 +1517,13 @@ int sopht_poisson3d_destroy(sopht_poisson3d* P) {
 
 int64_t sopht_poisson3d_scratch_bytes(const sopht_poisson3d* P) {
   return P ? P->slab_bytes + P->ghat_bytes : 0;
+}
+
+int sopht_poisson3d_schedule(const sopht_poisson3d* P, int* passes, int* zpass_jobs) {
+  SOPHT_REQUIRE(P != nullptr, "plan is null");
+  if (passes) *passes = P->passes;
+  if (zpass_jobs) *zpass_jobs = P->passes == 3 ? P->zjobs : 0;
+  return SOPHT_OK;
 }
 
 int sopht_poisson3d_solve(sopht_poisson3d* P, void* sol, const int64_t* ss, const void* rhs,

@@ -53,6 +53,13 @@ class UnboundedPoissonSolverPYFFTW3D:
     def scratch_bytes(self) -> int:
         return int(self._lib.sopht_poisson3d_scratch_bytes(self._plan))
 
+    @property
+    def fft_schedule(self) -> dict:
+        """The plan's FFT schedule: ``passes`` (3 or 5) and ``zpass_jobs`` (the three-pass z
+        pass's jobs per workgroup, timed when the plan was created; 0 with five passes)."""
+        passes, jobs = L.schedule("sopht_poisson3d_schedule", self._plan)
+        return {"passes": passes, "zpass_jobs": jobs}
+
     def __del__(self):
         plan = getattr(self, "_plan", None)
         if plan is not None and plan.value:

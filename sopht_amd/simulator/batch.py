@@ -115,6 +115,22 @@ class BatchedFlowSimulation:
                 L.written(inter.lag_grid_flow_velocity_field, inter.lag_grid_velocity_mismatch_field,
                           inter.lag_grid_position_mismatch_field, inter.lag_grid_forcing_field)
 
+    def poisson_schedules(self) -> list:
+        """Per size class: its grid, component count and its Poisson plan's FFT schedule
+        (passes, z-pass jobs per workgroup chosen when the plan was created)."""
+        out = []
+        lib = L.lib()
+        n = lib.sopht_ns3d_batch_class_schedule(self._batch, -1, None, None, None, None)
+        L.check(min(0, n))
+        for i in range(n):
+            shape = (C.c_int64 * 3)()
+            nc, passes, jobs = C.c_int(0), C.c_int(0), C.c_int(0)
+            L.check(min(0, lib.sopht_ns3d_batch_class_schedule(
+                self._batch, i, shape, C.byref(nc), C.byref(passes), C.byref(jobs))))
+            out.append({"grid": list(shape), "components": nc.value, "passes": passes.value,
+                        "zpass_jobs": jobs.value})
+        return out
+
     def synchronize(self):
         """Wait for the device; set each member's ``time`` (and its interactor's) and return
         the last dt of every member."""

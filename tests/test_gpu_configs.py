@@ -183,6 +183,9 @@ def test_c4_member_sizes_batched_bit_identical(dev):
     dts = [step_alone(f, i, U, steps) for f, i, U in alone]
     batch = [member(s) for s in specs]
     sim = BatchedFlowSimulation(batch)
+    sched = sim.poisson_schedules()
+    assert sorted(c["grid"][0] for c in sched) == [48, 64, 96, 128], sched
+    assert all(c["components"] == 3 and c["passes"] == 3 and c["zpass_jobs"] in (1, 2, 3, 6) for c in sched), sched
     sim.step(1)
     sim.step(steps - 1)
     last = sim.synchronize()

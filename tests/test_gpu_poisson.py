@@ -117,15 +117,18 @@ def test_solve_on_384_and_768_point_lines(dev, precision, shape, rng):
     assert rel_l2(to_host(out), ref) <= bound, rel_l2(to_host(out), ref)
 
 
-@pytest.mark.parametrize("shape", [(64, 64, 64), (48, 64, 64), (32, 128, 128), (12, 192, 192)])
+@pytest.mark.parametrize("shape", [(64, 64, 64), (48, 64, 64), (32, 128, 128), (12, 192, 192), (48, 48, 48),
+                                   (24, 96, 96)])
 def test_three_pass_schedule_matches_oracle(dev, shape, rng):
-    """FP32 grids with nx = ny and 2 nx in {128, 256, 384, 512} run the three-pass schedule
-    (the x transform fused with the inner sums of the y transform, and the outer 4-point sums
-    in the z pass); against the FP64 oracle at the five-pass schedule's FP32 bound."""
+    """FP32 grids with nx = ny and 2 nx in {96, 128, 192, 256, 384, 512} run the three-pass
+    schedule (the x transform fused with the inner sums of the y transform, and the outer
+    4-point sums in the z pass); against the FP64 oracle at the five-pass schedule's FP32 bound."""
     rhs = rng.standard_normal((3, *shape)).astype(np.float32)
     ref = np.zeros_like(rhs, dtype=np.float64)
     P.UnboundedPoissonSolver3D(*shape, x_range=1.0, real_t=np.float64).vector_field_solve(ref, rhs.astype(np.float64))
     s = solver_cls()(*shape, x_range=1.0, real_t=np.float32)
+    sched = s.fft_schedule
+    assert sched["passes"] == 3 and sched["zpass_jobs"] in (1, 2, 3, 6), sched
     out = to_dev(np.zeros_like(rhs), dev)
     s.vector_field_solve(solution_vector_field=out, rhs_vector_field=to_dev(rhs, dev))
     got = to_host(out)
@@ -159,3 +162,25 @@ def test_three_pass_against_five_pass(dev, n, monkeypatch):
     err = (torch.linalg.vector_norm(outs[1] - outs[0]) / torch.linalg.vector_norm(outs[0])).item()
     print(f"{n}^3 FP32 three-pass vs five-pass rel-L2 {err:.3e}")
     assert err <= 1e-5, err
+
+
+def test_schedule_choice_and_zpass_jobs(dev, monkeypatch):
+    """FP64 plans and grids with nx != ny take five passes; the z pass's jobs per workgroup
+    (timed at plan creation, or fixed by SOPHT_POISSON_ZJOBS) do not change the result."""
+    import torch
+
+    assert solver_cls()(16, 16, 16, x_range=1.0, real_t=np.float64).fft_schedule == {"passes": 5, "zpass_jobs": 0}
+    assert solver_cls()(64, 32, 64, x_range=1.0, real_t=np.float32).fft_schedule["passes"] == 5
+    n = 96
+    gen = torch.Generator(device=dev).manual_seed(3)
+    rhs = torch.randn((3, n, n, n), dtype=torch.float32, device=dev, generator=gen)
+    outs = []
+    for jobs in ("1", "6", "2"):
+        monkeypatch.setenv("SOPHT_POISSON_ZJOBS", jobs)
+        s = solver_cls()(n, n, n, x_range=1.0, real_t=np.float32)
+        assert s.fft_schedule == {"passes": 3, "zpass_jobs": int(jobs)}
+        out = torch.zeros_like(rhs)
+        s.vector_field_solve(solution_vector_field=out, rhs_vector_field=rhs)
+        outs.append(out)
+        del s
+    assert torch.equal(outs[0], outs[1]) and torch.equal(outs[0], outs[2])
