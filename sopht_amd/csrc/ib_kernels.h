@@ -6,6 +6,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <type_traits>
 
 #include "common.h"
 
@@ -333,9 +334,113 @@ __global__ void k_spread_keys(const int64_t* idx, int64_t n, int64_t padded, int
   if (i < padded) spread_key(idx, n, nz, ny, nx, prev, flag, i);
 }
 
-// Stable sort of the packed keys (bitonic on unique 64-bit keys) in one workgroup, only
-// when some node changed its support block since the plan was built; clears the plan's fill
-// counter for the rebuild.
+// Sort of the packed keys (unique 64-bit keys: the order is stable by construction) in one
+// workgroup, only when some node changed its support block since the plan was built; clears
+// the plan's fill counter for the rebuild.
+//
+// Bitonic sort of the padded keys (a power of two, at most 8192) by one workgroup of 1024
+// threads: thread t holds the E = max(1, padded / 1024) keys t E .. t E + E - 1 in registers.
+// Compare-exchange steps of stride j < E run inside the thread, strides below 64 E between the
+// lanes of a wavefront (shuffles), and only the longer strides through LDS with workgroup
+// barriers: 10 such steps for 8192 keys (and for 2048) instead of the 91 (66) of a sort that
+// takes every step through LDS. The sorted keys end in `keys` and in the LDS copy sk (read
+// by the box table).
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for_sort(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for_sort<B + 1, E>(f);
+  }
+}
+
+constexpr int kSortThreads = 1024;
+
+template <int J, int E>
+__device__ __forceinline__ void sort_in_thread(uint64_t (&v)[E], int base, int k) {
+  static_for_sort<0, E>([&](auto e_) {
+    constexpr int e = decltype(e_)::value;
+    if constexpr ((e & J) == 0 && e + J < E) {
+      const bool up = ((base + e) & k) == 0;
+      const uint64_t a = v[e], b = v[e + J];
+      if ((a > b) == up) {
+        v[e] = b;
+        v[e + J] = a;
+      }
+    }
+  });
+}
+
+// x of lane ^ M: an immediate swizzle inside each 32-lane half for M < 32 (bit mask mode:
+// and 0x1f, or 0, xor M), a permute across the halves for M = 32
+template <int M>
+__device__ __forceinline__ int xor_lane(int x) {
+  if constexpr (M < 32) return __builtin_amdgcn_ds_swizzle(x, (M << 10) | 0x1f);
+  else return __shfl_xor(x, M, 64);
+}
+
+template <int M>
+__device__ __forceinline__ uint64_t xor_lane_u64(uint64_t x) {
+  const int lo = int(unsigned(x)), hi = int(unsigned(x >> 32));
+  return (uint64_t(unsigned(xor_lane<M>(hi))) << 32) | unsigned(xor_lane<M>(lo));
+}
+
+template <int E>
+__device__ __forceinline__ void bitonic_regs(const uint64_t* prev, uint64_t* keys, int padded,
+                                             uint64_t* sk) {
+  const int t = int(threadIdx.x);
+  const bool act = t * E < padded;
+  const int base = t * E;
+  uint64_t v[E];
+  static_for_sort<0, E>([&](auto e_) {
+    constexpr int e = decltype(e_)::value;
+    v[e] = act ? prev[base + e] : ~uint64_t(0);
+  });
+  for (int k = 2; k <= padded; k <<= 1) {
+    // strides j = k / 2, k / 4, ... in decreasing order: through LDS while j >= 64 E ...
+    for (int j = k >> 1; j >= 64 * E; j >>= 1) {
+      if (act)
+        static_for_sort<0, E>([&](auto e_) { sk[base + decltype(e_)::value] = v[decltype(e_)::value]; });
+      __syncthreads();
+      if (act) {
+        const bool lower = (base & j) == 0;
+        static_for_sort<0, E>([&](auto e_) {
+          constexpr int e = decltype(e_)::value;
+          const bool up = ((base + e) & k) == 0;
+          const uint64_t a = v[e], b = sk[(base + e) ^ j];
+          v[e] = (lower == up) ? (a < b ? a : b) : (a < b ? b : a);
+        });
+      }
+      __syncthreads();
+    }
+    // ... between the lanes of a wavefront for j = M E, M = 32, 16, ..., 1 ...
+    static_for_sort<0, 6>([&](auto s_) {
+      constexpr int M = 32 >> decltype(s_)::value;
+      if (M * E < k) {  // j = M E <= k / 2
+        const bool lower = (t & M) == 0;
+        static_for_sort<0, E>([&](auto e_) {
+          constexpr int e = decltype(e_)::value;
+          const bool up = ((base + e) & k) == 0;
+          const uint64_t a = v[e], b = xor_lane_u64<M>(v[e]);
+          v[e] = (lower == up) ? (a < b ? a : b) : (a < b ? b : a);
+        });
+      }
+    });
+    // ... and inside the thread for j < E
+    if constexpr (E > 4) if (4 < k) sort_in_thread<4, E>(v, base, k);
+    if constexpr (E > 2) if (2 < k) sort_in_thread<2, E>(v, base, k);
+    if constexpr (E > 1) sort_in_thread<1, E>(v, base, k);
+  }
+  if (act) {
+    static_for_sort<0, E>([&](auto e_) {
+      constexpr int e = decltype(e_)::value;
+      sk[base + e] = v[e];
+      keys[base + e] = v[e];
+    });
+  }
+  __syncthreads();
+}
+
+// blockDim.x must be kSortThreads and padded <= 8 kSortThreads (the launches ensure both)
 __device__ __forceinline__ void bitonic_block(const uint64_t* prev, uint64_t* keys, int padded,
                                               int* flag, uint64_t* sk) {
   if (flag_load(flag + 1) == 0) return;  // uniform for the workgroup
@@ -343,25 +448,10 @@ __device__ __forceinline__ void bitonic_block(const uint64_t* prev, uint64_t* ke
     atomicExch(flag + 2, 0);
     atomicExch(flag + 3, 0);
   }
-  for (int i = threadIdx.x; i < padded; i += blockDim.x) sk[i] = prev[i];
-  __syncthreads();
-  for (int k = 2; k <= padded; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = threadIdx.x; i < padded; i += blockDim.x) {
-        const int p = i ^ j;
-        if (p > i) {
-          const bool up = (i & k) == 0;
-          const uint64_t a = sk[i], b = sk[p];
-          if ((a > b) == up) {
-            sk[i] = b;
-            sk[p] = a;
-          }
-        }
-      }
-      __syncthreads();
-    }
-  }
-  for (int i = threadIdx.x; i < padded; i += blockDim.x) keys[i] = sk[i];
+  if (padded >= 8 * kSortThreads) bitonic_regs<8>(prev, keys, padded, sk);
+  else if (padded >= 4 * kSortThreads) bitonic_regs<4>(prev, keys, padded, sk);
+  else if (padded >= 2 * kSortThreads) bitonic_regs<2>(prev, keys, padded, sk);
+  else bitonic_regs<1>(prev, keys, padded, sk);
 }
 
 // Box table of the sorted keys (see sopht_spread3d::table), built by the sorting workgroup
@@ -432,15 +522,18 @@ __device__ void cell_table_block(const uint64_t* sk, int padded, int64_t ny, int
   const int le = lb + chunk < v32 ? lb + chunk : v32;
   int sum = 0;
   for (int l = lb; l < le; ++l) sum += cnt[l];
-  scan[tid] = sum;
-  __syncthreads();
-  for (int off = 1; off < nt; off <<= 1) {  // inclusive scan of the chunk sums
-    const int v = tid >= off ? scan[tid - off] : 0;
-    __syncthreads();
-    scan[tid] += v;
-    __syncthreads();
+  // exclusive scan of the chunk sums: inside each wavefront by shuffles, then over the
+  // wavefront totals (one barrier)
+  const int lane = tid & 63, wave = tid >> 6;
+  int inc = sum;
+  for (int sft = 1; sft < 64; sft <<= 1) {
+    const int v = __shfl_up(inc, sft, 64);
+    if (lane >= sft) inc += v;
   }
-  int run = scan[tid] - sum;
+  if (lane == 63) scan[wave] = inc;
+  __syncthreads();
+  int run = inc - sum;
+  for (int w2 = 0; w2 < wave; ++w2) run += scan[w2];
   for (int l = lb; l < le; ++l) {
     table[l] = run;
     run += cnt[l];

@@ -73,7 +73,8 @@ def test_interp_arange_weights_closed_form(dev, real_t):
     np.testing.assert_allclose(to_host(lag1), 64 * 63 / 2 * dx**3, atol=tol(real_t))
 
 
-@pytest.mark.parametrize("m,cluster", [(3, False), (700, False), (500, True), (1500, True), (9000, False)])
+@pytest.mark.parametrize("m,cluster", [(3, False), (700, False), (500, True), (1500, True), (3000, False), (6000, False),
+                                       (9000, False)])
 def test_spread_bit_exact_vs_serial_order(dev, real_t, m, cluster, rng):
     n = 40
     dx = real_t(1.0 / n)
