@@ -250,8 +250,16 @@ int sopht_lag_to_eul_spread_3d(int dtype, int n_components, sopht_spread3d* ws,
                                int64_t ny, int64_t nx, const void* lag_field,
                                const void* weights, const int64_t* nearest_index,
                                int64_t num_lag_nodes, void* stream);
-/* Reads back (synchronising `stream`) and clears the out-of-domain flag of ws. */
+/* Reads back (synchronising `stream`) and clears the out-of-domain flag of ws; returns
+ * SOPHT_ERR_INVALID (and clears it) if a call declared by sopht_spread3d_reuse_plan found
+ * changed support indices. */
 int sopht_spread3d_check(sopht_spread3d* ws, void* stream);
+/* No counterpart in the reference (its spreading has no plan): declares that the next
+ * sopht_lag_to_eul_spread_3d through ws has the node count, grid and support indices of the
+ * previous one (a body at rest), so that call launches only the key check and the replay,
+ * not the sort and plan kernels. A key that changed anyway is detected on the device: that
+ * call spreads nothing and the next sopht_spread3d_check returns SOPHT_ERR_INVALID. */
+int sopht_spread3d_reuse_plan(sopht_spread3d* ws);
 
 /* No counterpart in the reference: sopht_lag_to_eul_spread_3d for callers that cannot
  * allocate (code under HIP graph capture, tight serving loops, harnesses that forbid

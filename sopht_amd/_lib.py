@@ -96,6 +96,7 @@ SIGNATURES = {
     "sopht_spread3d_destroy": [_vp],
     "sopht_lag_to_eul_spread_3d": [_i, _i, _vp, _p3, _pi64, _i64, _i64, _i64, _vp, _vp, _vp, _i64, _vp],
     "sopht_spread3d_check": [_vp, _vp],
+    "sopht_spread3d_reuse_plan": [_vp],
     "sopht_lag_to_eul_spread_3d_noalloc": [_i, _i, _p3, _pi64, _i64, _i64, _i64, _vp, _vp, _vp, _i64, _vp, _vp],
     "sopht_vbf_lag_force_3d": [
         _i, _i, _p3, _pi64, _i64, _i64, _i64, _vp, _vp, _i64, _d, _d, _d, _d,

@@ -145,7 +145,7 @@ __global__ void __launch_bounds__(1024) k_spread_sort_batch(const Member<T>* M) 
   const Member<T>& S = M[blockIdx.x];
   if (S.vb.n == 0) return;
   for (int64_t i = threadIdx.x; i < S.padded; i += blockDim.x)
-    spread_key(S.vb.idx, S.vb.n, S.g.nz, S.g.ny, S.g.nx, S.prev, S.flag, i);
+    spread_key(S.vb.idx, S.vb.n, S.g.nz, S.g.ny, S.g.nx, S.prev, S.flag, i, 0);
   __syncthreads();
   bitonic_block(S.prev, S.keys, S.padded, S.flag, sk);
 }

@@ -140,12 +140,19 @@ int spread_impl(int dtype, int ncomp, sopht_spread3d* ws, void* const eul[3], co
                 const int64_t* idx, int64_t n, hipStream_t s, int force) {
   int64_t padded = 1;
   while (padded < n) padded <<= 1;
+  // the caller's declaration (sopht_spread3d_reuse_plan) holds for this call only, and only
+  // for the same node count and grid as the call that left the plan
+  const int reuse = (ws->reuse_next && !force && ws->last_n == n) ? 1 : 0;
+  ws->reuse_next = 0;
+  ws->last_n = n;
   // keys of this call into ws->prev (and the "changed" flag) over the workspace's padded
   // length, so a later call with fewer nodes still sees its stale tail as changed
   k_spread_keys<<<blocks_for(ws->padded, 256), 256, 0, s>>>(idx, n, ws->padded, nz, ny, nx,
-                                                            ws->prev, ws->flag, force);
+                                                            ws->prev, ws->flag, force, reuse);
   SOPHT_CHECK_LAUNCH();
-  if (padded <= 8192) {
+  if (reuse) {
+    // sort, owners and merge would find the keys unchanged and return: not launched
+  } else if (padded <= 8192) {
     k_bitonic_lds<<<1, 1024, bitonic_lds_bytes(padded), s>>>(ws->prev, ws->keys, int(padded),
                                                              ws->flag, ny, nx, ws->table, ws->bbox,
                                                              table_cap_for(padded));
@@ -161,7 +168,8 @@ int spread_impl(int dtype, int ncomp, sopht_spread3d* ws, void* const eul[3], co
   }
   // plan rebuild (a no-op unless the keys changed): owners, then the merged contributor
   // lists; sorted keys in LDS for the owners' binary searches when they fit (128 KiB)
-  if (n <= 16384)
+  if (reuse) {
+  } else if (n <= 16384)
     k_spread_owner<true><<<blocks_for(n * 64, 512), 512, n * sizeof(uint64_t), s>>>(
         nz, ny, nx, ws->keys, n, ws->flag, ws->pcount, ws->poff, ws->owners, ws->table, ws->bbox);
   else
@@ -171,11 +179,13 @@ int spread_impl(int dtype, int ncomp, sopht_spread3d* ws, void* const eul[3], co
   SOPHT_CHECK_LAUNCH();
   // about one wavefront per node (an owned cell has ~16 contributors, so a node owns ~4 cells
   // on average): the owners' latency chains then overlap instead of queueing in a few waves
-  const int mblocks = std::min(8192, blocks_for(n * 64, 64 * kMergeWaves / 4));
-  k_spread_merge<<<mblocks, 64 * kMergeWaves, 0, s>>>(nz, ny, nx, ws->keys, n, ws->flag, ws->pcount,
-                                                     ws->poff, ws->owners, ws->plist, ws->table,
-                                                     ws->bbox);
-  SOPHT_CHECK_LAUNCH();
+  if (!reuse) {
+    const int mblocks = std::min(8192, blocks_for(n * 64, 64 * kMergeWaves / 4));
+    k_spread_merge<<<mblocks, 64 * kMergeWaves, 0, s>>>(nz, ny, nx, ws->keys, n, ws->flag, ws->pcount,
+                                                       ws->poff, ws->owners, ws->plist, ws->table,
+                                                       ws->bbox);
+    SOPHT_CHECK_LAUNCH();
+  }
   IB_DISPATCH(dtype, ([&] {
                 EulW3<T> F;
                 for (int c = 0; c < 3; ++c) F.p[c] = c < ncomp ? static_cast<T*>(eul[c]) : nullptr;
@@ -320,14 +330,28 @@ int sopht_lag_to_eul_spread_3d_noalloc(int dtype, int ncomp, void* const eul[3],
 int sopht_spread3d_check(sopht_spread3d* ws, void* stream) {
   SOPHT_REQUIRE(ws != nullptr, "spread workspace is null");
   hipStream_t s = static_cast<hipStream_t>(stream);
-  int h = 0;
-  SOPHT_CHECK_HIP(hipMemcpyAsync(&h, ws->flag, sizeof(int), hipMemcpyDeviceToHost, s));
+  int h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  SOPHT_CHECK_HIP(hipMemcpyAsync(h, ws->flag, sizeof(h), hipMemcpyDeviceToHost, s));
   SOPHT_CHECK_HIP(hipStreamSynchronize(s));
-  if (h) {
+  if (h[5]) {  // a call declared its plan reusable although its keys changed: nothing spread
+    const int zero = 0;
+    SOPHT_CHECK_HIP(hipMemcpyAsync(ws->flag + 5, &zero, sizeof(int), hipMemcpyHostToDevice, s));
+    SOPHT_CHECK_HIP(hipStreamSynchronize(s));
+    set_last_error("sopht_spread3d_reuse_plan was declared for a call whose support indices changed; "
+                   "that call spread nothing", __FILE__, __LINE__);
+    return SOPHT_ERR_INVALID;
+  }
+  if (h[0]) {
     SOPHT_CHECK_HIP(hipMemsetAsync(ws->flag, 0, sizeof(int), s));
     set_last_error("a Lagrangian node's 4^3 support leaves the Eulerian grid", __FILE__, __LINE__);
     return SOPHT_ERR_OUT_OF_DOMAIN;
   }
+  return SOPHT_OK;
+}
+
+int sopht_spread3d_reuse_plan(sopht_spread3d* ws) {
+  SOPHT_REQUIRE(ws != nullptr, "spread workspace is null");
+  ws->reuse_next = 1;
   return SOPHT_OK;
 }
 

@@ -32,6 +32,8 @@ struct sopht_spread3d {
   int* table;
   int* bbox;
   int64_t plan_nz, plan_ny, plan_nx;  // grid the plan was built for (host copy)
+  int64_t last_n;     // node count of the last call (host copy)
+  int reuse_next;     // the caller declared the next call's support indices unchanged
 };
 
 namespace sopht {
@@ -309,7 +311,8 @@ __global__ void k_vbf_time_step(T* dX, const T* dV, int64_t m, T dt) {
 
 // -------------------------------------------------------------------- spreading
 __device__ __forceinline__ void spread_key(const int64_t* idx, int64_t n, int64_t nz, int64_t ny,
-                                           int64_t nx, uint64_t* prev, int* flag, int64_t i) {
+                                           int64_t nx, uint64_t* prev, int* flag, int64_t i,
+                                           int guard) {
   uint64_t key = ~uint64_t(0);
   if (i < n) {
     const int64_t base[3] = {idx[i] - 1, idx[n + i] - 1, idx[2 * n + i] - 1};
@@ -323,15 +326,19 @@ __device__ __forceinline__ void spread_key(const int64_t* idx, int64_t n, int64_
   // the sorted order and the gather plan are reused while every node keeps its support block
   if (prev[i] != key) {
     atomicOr(flag + 1, 1);
+    if (guard) atomicOr(flag + 5, 1);  // declared unchanged, but it changed
     prev[i] = key;
   }
 }
 
+// guard: the caller declared the keys unchanged and the plan kernels are not launched; a key
+// that changed anyway sets flag[5], which stops the replay and is reported by the check
 __global__ void k_spread_keys(const int64_t* idx, int64_t n, int64_t padded, int64_t nz,
-                              int64_t ny, int64_t nx, uint64_t* prev, int* flag, int force) {
+                              int64_t ny, int64_t nx, uint64_t* prev, int* flag, int force,
+                              int guard) {
   const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (force && i == 0) atomicOr(flag + 1, 1);
-  if (i < padded) spread_key(idx, n, nz, ny, nx, prev, flag, i);
+  if (i < padded) spread_key(idx, n, nz, ny, nx, prev, flag, i, guard);
 }
 
 // Sort of the packed keys (unique 64-bit keys: the order is stable by construction) in one
@@ -976,6 +983,7 @@ __global__ void __launch_bounds__(256) k_spread_replay(
     const uint64_t* keys, int64_t n, const int* pcount, const int* poff, const uint32_t* plist,
     const uint32_t* owners, int* flag) {
 #pragma clang fp contract(off)
+  if (flag_load(flag + 5) != 0) return;  // a plan declared valid whose keys changed (uniform)
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicExch(flag + 1, 0);
   const int64_t count = flag_load(flag + 3);
   if (flag_load(flag + 4) <= kThreadReplayMax) {

@@ -122,6 +122,13 @@ class EulerianLagrangianGridCommunicator3D:
         )
         L.written(eul_grid_field)
 
+    def declare_support_unchanged(self):
+        """No counterpart in the reference: the next spreading call has the node count and
+        support indices of the previous one (a body at rest), so it replays the spreading plan
+        without the sort and plan kernels. A support that changed anyway is caught on the
+        device and reported by the next ``check_support_in_domain`` (``ValueError``)."""
+        L.call("sopht_spread3d_reuse_plan", self._spread_ws)
+
     def check_support_in_domain(self):
         """Synchronise and raise ``IndexError`` if a spread support block left the grid."""
         L.call("sopht_spread3d_check", self._spread_ws, L.stream_ptr())

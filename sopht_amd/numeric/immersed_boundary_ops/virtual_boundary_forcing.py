@@ -45,15 +45,18 @@ class _LagrangianUpload:
         self.dev = None
         self.pinned = None
         self.event = None
+        self.generation = 0  # bumped whenever the values may have changed
 
     def get(self, a):
         import torch
 
         if isinstance(a, torch.Tensor):
+            self.generation += 1  # a device tensor: its values are not tracked
             return as_device_f64(a)
         a = np.asarray(a, dtype=np.float64)
         if self.host is not None and self.host.shape == a.shape and np.array_equal(self.host, a):
             return self.dev
+        self.generation += 1
         if self.dev is None or tuple(self.dev.shape) != a.shape:
             self.dev = torch.empty(a.shape, dtype=torch.float64, device=_device())
             self.pinned = torch.empty(a.shape, dtype=torch.float64, pin_memory=True)
@@ -130,6 +133,10 @@ class VirtualBoundaryForcing:
         self._tag = L.dtype_tag(real_t)
         self._pos_upload = _LagrangianUpload()
         self._vel_upload = _LagrangianUpload()
+        # (position generation, workspace) of the last spreading: the same positions through
+        # the same workspace have the same support indices, and its plan is replayed without
+        # the sort and plan kernels
+        self._spread_key = None
         self.enable_eul_grid_forcing_reset = enable_eul_grid_forcing_reset
         if enable_eul_grid_forcing_reset:
             self.compute_interaction_forcing = (
@@ -206,6 +213,11 @@ class VirtualBoundaryForcing:
         self.compute_interaction_force_on_lag_grid(
             eul_grid_velocity_field, lag_grid_position_field, lag_grid_velocity_field
         )
+        comm = self.eul_lag_grid_communicator
+        key = (self._pos_upload.generation, comm._spread_ws.value)
+        if key == self._spread_key:
+            comm.declare_support_unchanged()
+        self._spread_key = key
         self.eul_lag_grid_communicator.lagrangian_to_eulerian_grid_interpolation_kernel(
             eul_grid_forcing_field,
             self.lag_grid_forcing_field,

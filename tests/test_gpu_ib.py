@@ -134,6 +134,47 @@ def test_spread_plan_reuse_and_rebuild(dev, real_t, rng):
     check(X2, (n, n, n), 6)
 
 
+def test_spread_declared_unchanged_support(dev, real_t, rng):
+    """sopht_spread3d_reuse_plan (a body at rest): a declared call replays the plan without the
+    sort and plan kernels and stays bit-exact; a declaration whose support changed is caught on
+    the device (nothing is spread) and reported by the next check as ValueError; the call after
+    it rebuilds the plan."""
+    n, m = 32, 300
+    dx = real_t(1.0 / n)
+    shift = real_t(dx / 2)
+    c = comm_cls()(dx, shift, m, 2, real_t, n_components=3)
+
+    def call(X, seed, declare):
+        r = np.random.default_rng(seed)
+        idx, d = ib.local_eulerian_grid_support(X, dx, shift, real_t)
+        w, _ = ib.cosine_weights(d, dx, real_t)
+        Fl = r.standard_normal((3, m)).astype(real_t)
+        F0 = r.standard_normal((3, n, n, n)).astype(real_t)
+        ref = F0.copy()
+        ib.spread(ref, Fl, w, idx)
+        F = to_dev(F0, dev)
+        if declare:
+            c.declare_support_unchanged()
+        c.lagrangian_to_eulerian_grid_interpolation_kernel(F, to_dev(Fl, dev), to_dev(w, dev), to_dev(idx, dev))
+        return to_host(F), ref, F0
+
+    X1 = random_nodes(rng, m, 0.25, 0.75)
+    call(X1, 1, False)
+    c.check_support_in_domain()
+    got, ref, _ = call(X1, 2, True)  # same supports, declared
+    c.check_support_in_domain()
+    np.testing.assert_array_equal(got, ref)
+    X2 = X1 + 2.5 * dx  # moved: the declaration is wrong
+    got, _, F0 = call(X2, 3, True)
+    np.testing.assert_array_equal(got, F0)  # nothing spread
+    with pytest.raises(ValueError, match="reuse_plan"):
+        c.check_support_in_domain()
+    c.check_support_in_domain()  # cleared
+    got, ref, _ = call(X2, 4, False)  # the plan is rebuilt
+    c.check_support_in_domain()
+    np.testing.assert_array_equal(got, ref)
+
+
 def test_spread_scalar_and_out_of_domain(dev, rng):
     n, m = 16, 5
     real_t = np.float64
