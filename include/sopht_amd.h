@@ -370,6 +370,10 @@ int sopht_ns3d_stage_times(sopht_ns3d* plan, double* ms, int n, int64_t* steps, 
 /* sopht_poisson3d_schedule of the step's Poisson plan (0 and 0 with the fast-diagonalisation
  * solver; no counterpart in the reference). */
 int sopht_ns3d_poisson_schedule(const sopht_ns3d* plan, int* passes, int* zpass_jobs);
+/* How the step's first sweep (vorticity forcing, advection and stretching) runs: *vector = 1
+ * for 16-byte row accesses, 0 for the scalar sweep; *zchunk = z planes per work item. Both
+ * give the same bits (no counterpart in the reference). */
+int sopht_ns3d_s1_schedule(const sopht_ns3d* plan, int* vector, int* zchunk);
 const char* sopht_ns3d_stage_name(const sopht_ns3d* plan, int stage);
 
 /* ---- batched step (no counterpart in the reference: many independent simulators, the

@@ -116,6 +116,7 @@ SIGNATURES = {
     "sopht_ns3d_stage_times": [_vp, _pd, _i, _pi64, _i],
     "sopht_ns3d_stage_name": [_vp, _i],
     "sopht_ns3d_poisson_schedule": [_vp, _pi, _pi],
+    "sopht_ns3d_s1_schedule": [_vp, _pi, _pi],
     "sopht_ns3d_batch_class_schedule": [_vp, _i, _pi64, _pi, _pi, _pi],
     "sopht_ns3d_batch_create": [_i, _i, C.POINTER(BatchMember), _vp, C.POINTER(_vp)],
     "sopht_ns3d_batch_step": [_vp, _i, _i, _vp],
@@ -191,9 +192,11 @@ def torch_dtype(real_t):
 
 
 def stream_ptr():
+    """The current HIP stream of the current device (torch's raw stream handle: building a
+    ``torch.cuda.Stream`` object per call costs a few microseconds of host time per launch)."""
     import torch
 
-    return _vp(torch.cuda.current_stream().cuda_stream)
+    return _vp(torch._C._cuda_getCurrentRawStream(torch._C._cuda_getDevice()))
 
 
 def strides3(t) -> C.Array:
@@ -207,8 +210,10 @@ def ptr(t):
 
 
 def comps(v):
-    """Three component pointers of a (3, nz, ny, nx) tensor."""
-    return _p3(*(v[i].data_ptr() for i in range(3)))
+    """Three component pointers of a (3, nz, ny, nx) tensor (from its base pointer and first
+    stride, without building the component views)."""
+    base, step = v.data_ptr(), v.stride(0) * v.element_size()
+    return _p3(base, base + step, base + 2 * step)
 
 
 def require_device_tensor(t, real_t, name, ndim=None):
@@ -239,8 +244,9 @@ def written(*tensors) -> None:
 
 
 def schedule(fn: str, handle) -> tuple[int, int]:
-    """(passes, zpass_jobs) of a Poisson plan (``sopht_poisson3d_schedule``) or of a step plan
-    (``sopht_ns3d_poisson_schedule``)."""
+    """Two integers a plan reports: (passes, zpass_jobs) of a Poisson plan
+    (``sopht_poisson3d_schedule``) or of a step plan (``sopht_ns3d_poisson_schedule``), or
+    (vector, zchunk) of a step plan's first sweep (``sopht_ns3d_s1_schedule``)."""
     passes, jobs = C.c_int(0), C.c_int(0)
     check(getattr(lib(), fn)(handle, C.byref(passes), C.byref(jobs)))
     return passes.value, jobs.value

@@ -216,7 +216,7 @@ __global__ void __launch_bounds__(kMarchThreads) k_sc_batch(const Member<T>* M, 
   const T pD = T(S.nu * S.dt / S.dx / S.dx);
   // F is zeroed only in the S1 tiles that saw a bit of it set (elsewhere it is zero already)
   sc_body<T, VEC>(S.g, S.psi, S.w, S.F, pD, S.pen, S.with_forcing, S.m_tx, S.m_ty, item - off[m],
-                  kMarchChunk, fflags + off_s1[m], S.s1_tx, S.s1_ty);
+                  kMarchChunk, fflags + off_s1[m], S1Map{S.s1_tx, S.s1_ty, S1Tile<T>::TY, kS1Chunk});
 }
 
 template <typename T, int VEC>

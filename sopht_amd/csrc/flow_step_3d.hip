@@ -12,6 +12,7 @@
 // The sweeps march up z in (x, y) tiles with x-fastest lanes (coalesced rows).
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <new>
 
 #include "common.h"
@@ -65,18 +66,24 @@ int step_impl(sopht_ns3d* P, const void* vort_in, void* vort_out, const void* ve
   const T pC = T(0.5 / double(dx));
   StageTimer& tm = P->timer;
   tm.begin(s);
-  using ST = S1Tile<T>;
-  const int s1_tx = int(ceil_div(g.nx, ST::TX)), s1_ty = int(ceil_div(g.ny, ST::TY));
-  const bool flags_ok = int64_t(s1_tx) * s1_ty * ceil_div(g.nz, kS1Chunk) <= P->s1flags_cap;
-  const int* fflags = flags_ok ? P->s1flags : nullptr;
+  // S1 with 16-byte accesses when the rows allow them, else the scalar sweep
+  const bool s1v = P->s1_vec != 0;
+  const int s1_rows = s1v ? S1VTile<T>::TY : S1Tile<T>::TY;
+  const int s1_chunk = s1v ? P->s1_chunk : kS1Chunk;
+  const S1Map s1m{int(ceil_div(g.nx, 64)), int(ceil_div(g.ny, s1_rows)), s1_rows, s1_chunk};
+  const int64_t s1_items = int64_t(s1m.tiles_x) * s1m.tiles_y * ceil_div(g.nz, s1_chunk);
+  const bool flags_ok = s1_items <= P->s1flags_cap;
+  int* const fflags = flags_ok ? P->s1flags : nullptr;
   {
-    const int zchunk = kS1Chunk;
-    const int tiles_x = s1_tx, tiles_y = s1_ty;
-    const int64_t items = int64_t(tiles_x) * tiles_y * ceil_div(g.nz, zchunk);
-    k_s1<T><<<unsigned(items), ST::NT, 0, s>>>(g, w_in, u_in, F, ps, pF, pB, wf, zchunk, tiles_x,
-                                              tiles_y, items % 8 == 0 ? 1 : 0,
-                                              static_cast<T*>(vort1_out),
-                                              items <= P->s1flags_cap ? P->s1flags : nullptr);
+    const int remap = s1_items % 8 == 0 ? 1 : 0;
+    if (s1v)
+      k_s1v<T><<<unsigned(s1_items), S1VTile<T>::NT, 0, s>>>(g, w_in, u_in, F, ps, pF, pB, wf, s1_chunk,
+                                                            s1m.tiles_x, s1m.tiles_y, remap,
+                                                            static_cast<T*>(vort1_out), fflags);
+    else
+      k_s1<T><<<unsigned(s1_items), S1Tile<T>::NT, 0, s>>>(g, w_in, u_in, F, ps, pF, pB, wf, kS1Chunk,
+                                                          s1m.tiles_x, s1m.tiles_y, remap,
+                                                          static_cast<T*>(vort1_out), fflags);
     SOPHT_CHECK_LAUNCH();
   }
   tm.mark(0, s);
@@ -86,7 +93,7 @@ int step_impl(sopht_ns3d* P, const void* vort_in, void* vort_out, const void* ve
     // penalisation (navier_stokes_flow_simulators.py:466-474)
     Pen none = make_pen(P);
     none.w = 0;
-    launch_sc<T>(ml, g, ps, w, F, pD, none, wf && zero_forcing, s, fflags, s1_tx, s1_ty);
+    launch_sc<T>(ml, g, ps, w, F, pD, none, wf && zero_forcing, s, fflags, s1m);
     SOPHT_CHECK_LAUNCH();
     const int64_t n = g.nz * g.ny * g.nx;
     const int64_t st[3] = {g.ny * g.nx, g.nx, 1};
@@ -110,7 +117,7 @@ int step_impl(sopht_ns3d* P, const void* vort_in, void* vort_out, const void* ve
       if (rc) return rc;
     }
   } else {
-    launch_sc<T>(ml, g, ps, w, F, pD, make_pen(P), wf && zero_forcing, s, fflags, s1_tx, s1_ty);
+    launch_sc<T>(ml, g, ps, w, F, pD, make_pen(P), wf && zero_forcing, s, fflags, s1m);
     SOPHT_CHECK_LAUNCH();
   }
   tm.mark(1, s);
@@ -227,6 +234,31 @@ int sopht_ns3d_create_with_poisson_solver(int dtype, int64_t nz, int64_t ny, int
   P->filter_order = -1;
   P->filter_type = 0;
   {
+    // The vector sweep needs whole 16-byte row vectors and component byte offsets below 2^32.
+    // Its work items hold 32 planes while that still makes two per CU: at 256^3 FP32 the
+    // resident tiles then cover one z chunk per XCD (16 planes put two chunks on each XCD and
+    // doubled its L2 misses; 64 left one workgroup per CU); else 16 planes, and grids with
+    // fewer than 256 such items keep the scalar sweep (twice the workgroups). The environment
+    // can force either: SOPHT_S1_SCALAR=1 (the two sweeps give the same bits) and
+    // SOPHT_S1_CHUNK=16/32/64.
+    const int64_t vec = 16 / (dtype == SOPHT_F32 ? 4 : 8);
+    const int64_t bytes = nz * ny * nx * (dtype == SOPHT_F32 ? 4 : 8);
+    const int64_t rows = dtype == SOPHT_F32 ? S1VTile<float>::TY : S1VTile<double>::TY;
+    const int64_t tiles = ceil_div(nx, 64) * ceil_div(ny, rows);
+    const char* scalar = std::getenv("SOPHT_S1_SCALAR");
+    const char* chunk = std::getenv("SOPHT_S1_CHUNK");
+    P->s1_chunk = tiles * ceil_div(nz, 32) >= 512 ? 32 : 16;
+    bool fill = tiles * ceil_div(nz, 16) >= 256;
+    if (chunk) {
+      const int v = std::atoi(chunk);
+      if (v == 16 || v == 32 || v == 64) {
+        P->s1_chunk = v;
+        fill = true;
+      }
+    }
+    P->s1_vec = nx % vec == 0 && bytes < (int64_t(1) << 32) && fill && !(scalar && std::atoi(scalar) == 1);
+  }
+  {
     const int64_t i32 = ceil_div(nx, S1Tile<float>::TX) * ceil_div(ny, S1Tile<float>::TY) * ceil_div(nz, kS1Chunk);
     const int64_t i64 = ceil_div(nx, S1Tile<double>::TX) * ceil_div(ny, S1Tile<double>::TY) * ceil_div(nz, kS1Chunk);
     P->s1flags_cap = dtype == SOPHT_F32 ? i32 : i64;
@@ -274,6 +306,13 @@ int sopht_ns3d_poisson_schedule(const sopht_ns3d* P, int* passes, int* zpass_job
     return SOPHT_OK;
   }
   return sopht_poisson3d_schedule(P->poisson, passes, zpass_jobs);
+}
+
+int sopht_ns3d_s1_schedule(const sopht_ns3d* P, int* vector, int* zchunk) {
+  SOPHT_REQUIRE(P != nullptr, "plan is null");
+  if (vector) *vector = P->s1_vec;
+  if (zchunk) *zchunk = P->s1_vec ? P->s1_chunk : sopht::kS1Chunk;
+  return SOPHT_OK;
 }
 
 const char* sopht_ns3d_stage_name(const sopht_ns3d* P, int i) {

@@ -23,6 +23,8 @@ struct sopht_ns3d {
   int64_t s1flags_cap;
   int filter_order;    // vorticity filter: order (< 0: off) and type (0 multiplicative, 1 convolution)
   int filter_type;
+  int s1_vec;          // S1 with 16-byte accesses (rows a multiple of 16 bytes), else the scalar sweep
+  int s1_chunk;        // z planes per work item of the vector S1 sweep (16, 32 or 64)
   sopht::StageTimer timer;
 };
 

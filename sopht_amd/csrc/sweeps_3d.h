@@ -328,6 +328,308 @@ __device__ __forceinline__ void x_nbrs(const VecT<T, VEC>& v, const T* __restric
   }
 }
 
+// S1 with 16-byte accesses (rows whose length is a multiple of VEC). The scalar sweep above is
+// bound by the texture-address unit: one instruction per array, plane and cell. Here a thread
+// holds VEC neighbouring cells of one row, so each array and plane costs one 16-byte load per
+// VEC cells. The tile is 64 cells by TY rows and marches a chunk of z planes as s1_body does.
+// Per plane i:
+//   - the tile threads form omega1 and B on their cells; the x neighbours of F come from the
+//     adjacent lanes of the row, its y neighbours are row loads, and F0 and F1 ride along z in
+//     registers;
+//   - wavefront 0 forms B on the y ring rows (y0 - 1 and y0 + TY);
+//   - wavefront 1 forms B on the x edge columns (x0 - 1 and x0 + 64).
+// LDS keeps three planes of B: the B0 and B2 rows of the tile and ring (y derivatives), the B1
+// rows of the tile, and the B1 and B2 edge columns (x derivatives at the tile's first and last
+// lanes). After the one barrier per plane, plane i - 1 is consumed: its x derivatives from the
+// adjacent lanes, its y and z derivatives from LDS. Only omega1 of the consumed plane stays in
+// registers from one plane to the next. Every expression keeps
+// the operand order of s1_body, so both sweeps give the same bits.
+template <typename T>
+struct S1VTile {
+  static constexpr int VEC = 16 / int(sizeof(T));
+  static constexpr int TX = 64, MXT = TX / VEC;            // cells and lanes per row
+  static constexpr int TY = sizeof(T) == 4 ? 16 : 8, NT = MXT * TY;
+  static constexpr int RW = TY + 2, ROWS = RW * TX;        // a component's row box (ring rows included)
+  static constexpr int B1OFF = 2 * ROWS, EOFF = B1OFF + TY * TX;
+  static constexpr int PL = EOFF + 4 * TY;  // B0 and B2 row boxes, B1 tile rows, B1 and B2 edge columns
+  static_assert(NT >= 64 + 2 * TY && 2 * MXT <= 64 && MXT == MarchGeom<VEC>::MXT,
+                "ring rows in wavefront 0, edge columns in wavefront 1");
+};
+
+// Where a flow step's S1 tiles lie, for the F-zeroing flags it leaves per tile (SC reads them).
+struct S1Map {
+  int tiles_x, tiles_y, rows, chunk;  // 64-cell-wide tiles of `rows` rows and `chunk` planes
+};
+
+template <typename T>
+__device__ __forceinline__ bool any_bits(const VecT<T, 16 / int(sizeof(T))>& v) {
+  bool r = false;
+  for (int k = 0; k < 16 / int(sizeof(T)); ++k) r = r || nonzero_bits(v.v[k]);
+  return r;
+}
+
+template <typename T>
+__device__ __forceinline__ void s1v_body(Grid g, const T* __restrict__ w, const T* __restrict__ u,
+                                         const T* __restrict__ F, T* __restrict__ w2out, T pF, T pB,
+                                         int with_forcing, int zchunk, int tiles_x, int tiles_y,
+                                         int64_t item, T* __restrict__ w1out = nullptr,
+                                         int* fflags = nullptr) {
+  using S = S1VTile<T>;
+  constexpr int VEC = S::VEC, TX = S::TX, MXT = S::MXT, TY = S::TY, ROWS = S::ROWS;
+  constexpr int B1OFF = S::B1OFF, EOFF = S::EOFF;
+  using V = VecT<T, VEC>;
+  __shared__ __align__(16) T bs[3][S::PL];
+  const int tid = threadIdx.x;
+  const int nz = int(g.nz), ny = int(g.ny), nx = int(g.nx);
+  const int n = nz * ny * nx, sz = ny * nx, sy = nx;
+  const int x0 = int(item % tiles_x) * TX;
+  const int y0 = int((item / tiles_x) % tiles_y) * TY;
+  const int z0 = int(item / (int64_t(tiles_x) * tiles_y)) * zchunk;
+  const int z1 = z0 + zchunk < nz ? z0 + zchunk : nz;
+  const bool fw = with_forcing != 0;
+  const T* const F0 = F;
+  const T* const F1 = F + n;
+  const T* const F2 = F + 2 * n;
+  // tile role: cells xs .. xs + VEC - 1 of row yt
+  const int tx = tid % MXT, ty = tid / MXT;
+  const int xs = x0 + tx * VEC, yt = y0 + ty;
+  const bool tv = xs < nx && yt < ny;
+  const int colj = yt * sy + xs;
+  const int lrow = (ty + 1) * TX + tx * VEC;  // the cells' slot in a row box
+  const int l1 = B1OFF + ty * TX + tx * VEC;   // and in the B1 rows
+  // ring role (wavefront 0): cells xs .. of row yr
+  const bool ring = tid < 2 * MXT;
+  const int yr = tid < MXT ? y0 - 1 : y0 + TY;
+  const bool rv = ring && xs < nx && yr >= 0 && yr < ny;
+  const int rrow = (tid < MXT ? 0 : TY + 1) * TX + tx * VEC;
+  // edge role (wavefront 1): cell (xe, ye)
+  const int e = tid - 64;
+  const bool edge = e >= 0 && e < 2 * TY;
+  const int side = e >= TY ? 1 : 0, ey = y0 + (e - side * TY);
+  const int xe = side ? x0 + TX : x0 - 1;
+  const bool ev = edge && xe >= 0 && xe < nx && ey < ny;
+
+  // Accesses at a uniform base (a component's start) plus a 32-bit byte offset, so an address
+  // is one VGPR over an SGPR pair (the host keeps n sizeof(T) below 2^32).
+  auto at = [](const T* p, int off) -> const T* {
+    return reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + uint32_t(off) * uint32_t(sizeof(T)));
+  };
+  // one vector-typed load (a struct load is split per element and a conditionally used
+  // element sunk into a branch, which breaks the 16-byte access apart)
+  typedef T vecT __attribute__((ext_vector_type(VEC)));
+  auto vl = [&](const T* p, int off) -> V {
+    const vecT x = *reinterpret_cast<const vecT*>(at(p, off));
+    V r;
+    for (int k = 0; k < VEC; ++k) r.v[k] = x[k];
+    return r;
+  };
+  auto sl = [&](const T* p, int off) -> T { return *at(p, off); };
+  auto ld = [&](const T* p, int off, bool ok) -> V { return ok ? vl(p, off) : vzero<T, VEC>(); };
+  // x neighbours of a row vector at element offset `off` (adjacent lanes; the tile's first and
+  // last lanes load); every lane of the row takes part
+  auto xn = [&](const V& v, const T* p, int off, bool act, T& l, T& r) {
+    l = __shfl_up(v.v[VEC - 1], 1, MXT);
+    r = __shfl_down(v.v[0], 1, MXT);
+    if (tx == 0) l = (act && xs > 0) ? sl(p, off - 1) : T(0);
+    if (tx == MXT - 1) r = (act && xs + VEC < nx) ? sl(p, off + VEC) : T(0);
+  };
+  auto lds_ld = [&](const T* p) -> V {
+    const vecT x = *reinterpret_cast<const vecT*>(p);
+    V r;
+    for (int k = 0; k < VEC; ++k) r.v[k] = x[k];
+    return r;
+  };
+  auto lds_st = [&](T* p, const V& v) {
+    vecT x;
+    for (int k = 0; k < VEC; ++k) x[k] = v.v[k];
+    *reinterpret_cast<vecT*>(p) = x;
+  };
+  auto vst = [&](T* p, int off, const V& v, bool stream) {
+    T* q = const_cast<T*>(at(p, off));
+    vecT x;
+    for (int k = 0; k < VEC; ++k) x[k] = v.v[k];
+    if (stream) __builtin_nontemporal_store(x, reinterpret_cast<vecT*>(q));
+    else *reinterpret_cast<vecT*>(q) = x;
+  };
+  auto fz = [&](const T* f, int q) -> V { return ld(f, q * sz + colj, fw && tv && q >= 0 && q < nz); };
+  // omega1 of the consumed plane
+  V a1p[3] = {vzero<T, VEC>(), vzero<T, VEC>(), vzero<T, VEC>()};
+  bool fset = false;  // some bit of F set in this thread's cells (the tile's SC zeroes F then)
+#pragma unroll 1
+  for (int i = z0 - 1; i <= z1; ++i) {
+    const bool pin = i >= 0 && i < nz;
+    const bool qin = i > 0 && i < nz - 1;
+    // F of the thread's cells: F0 and F1 on planes i - 1 and i + 1, F1 and F2 on plane i (the
+    // flag looks at F0 and F1 of plane i + 1 and F2 of plane i, as s1_body does)
+    const V f0p = fz(F0, i + 1), f1p = fz(F1, i + 1), f2c = fz(F2, i), f1c = fz(F1, i);
+    if (fw) fset = fset || any_bits<T>(f0p) || any_bits<T>(f1p) || any_bits<T>(f2c);
+    // x neighbours of F1 and F2 on plane i (every lane of the row takes part in the shuffles)
+    const bool fact = fw && tv && pin;
+    T f1l, f1r, f2l, f2r;
+    xn(f1c, F1, i * sz + colj, fact, f1l, f1r);
+    xn(f2c, F2, i * sz + colj, fact, f2l, f2r);
+    T* const Bq = bs[(i + 3) % 3];
+    if (ring) {
+      // B0 and B2 of the ring row yr on plane i
+      const bool ok = rv && pin;
+      const int j = i * sz + yr * sy + xs;
+      const V r1 = ld(F1, j, fw && ok), r2 = ld(F2, j, fw && ok);
+      T l1, q1, l2, q2;
+      xn(r1, F1, j, fw && ok, l1, q1);
+      xn(r2, F2, j, fw && ok, l2, q2);
+      if (ok) {
+        V a0 = vl(w, j), a1 = vl(w + n, j), a2 = vl(w + 2 * n, j);
+        const V u0 = vl(u, j), u1 = vl(u + n, j), u2 = vl(u + 2 * n, j);
+        if (fw && qin && yr > 0 && yr < ny - 1) {
+          const V f2yp = vl(F2, j + sy), f2ym = vl(F2, j - sy);
+          const V f0yp = vl(F0, j + sy), f0ym = vl(F0, j - sy);
+          const V f1zp = vl(F1, j + sz), f1zm = vl(F1, j - sz);
+          const V f0zp = vl(F0, j + sz), f0zm = vl(F0, j - sz);
+          for (int k = 0; k < VEC; ++k) {
+            const int x = xs + k;
+            const bool inner = x > 0 && x < nx - 1;
+            const T f2xl = k > 0 ? r2.v[k - 1] : l2, f2xr = k < VEC - 1 ? r2.v[k + 1] : q2;
+            const T f1xl = k > 0 ? r1.v[k - 1] : l1, f1xr = k < VEC - 1 ? r1.v[k + 1] : q1;
+            const T c0 = f2yp.v[k] - f2ym.v[k] - f1zp.v[k] + f1zm.v[k];
+            const T c1 = f0zp.v[k] - f0zm.v[k] - f2xr + f2xl;
+            const T c2 = f1xr - f1xl - f0yp.v[k] + f0ym.v[k];
+            const T o0 = a0.v[k] + pF * c0, o1 = a1.v[k] + pF * c1, o2 = a2.v[k] + pF * c2;
+            a0.v[k] = inner ? o0 : a0.v[k];
+            a1.v[k] = inner ? o1 : a1.v[k];
+            a2.v[k] = inner ? o2 : a2.v[k];
+          }
+        }
+        V rb0, rb2;
+        for (int k = 0; k < VEC; ++k) {
+          rb0.v[k] = u1.v[k] * a2.v[k] - a1.v[k] * u2.v[k];
+          rb2.v[k] = u0.v[k] * a1.v[k] - a0.v[k] * u1.v[k];
+        }
+        lds_st(Bq + rrow, rb0);
+        lds_st(Bq + ROWS + rrow, rb2);
+      }
+    }
+    V an[3] = {vzero<T, VEC>(), vzero<T, VEC>(), vzero<T, VEC>()};
+    if (tv && pin) {
+      const int j = i * sz + colj;
+      an[0] = vl(w, j);
+      an[1] = vl(w + n, j);
+      an[2] = vl(w + 2 * n, j);
+      const V u0 = vl(u, j), u1 = vl(u + n, j), u2 = vl(u + 2 * n, j);
+      if (fw && qin && yt > 0 && yt < ny - 1) {
+        const V f2yp = vl(F2, j + sy), f2ym = vl(F2, j - sy);
+        const V f0yp = vl(F0, j + sy), f0ym = vl(F0, j - sy);
+        const V f0m = vl(F0, j - sz), f1m = vl(F1, j - sz);
+        // every element is formed and the border cells keep omega by a select, so the
+        // loads stay whole 16-byte ones
+        for (int k = 0; k < VEC; ++k) {
+          const int x = xs + k;
+          const bool inner = x > 0 && x < nx - 1;
+          const T f2xl = k > 0 ? f2c.v[k - 1] : f2l, f2xr = k < VEC - 1 ? f2c.v[k + 1] : f2r;
+          const T f1xl = k > 0 ? f1c.v[k - 1] : f1l, f1xr = k < VEC - 1 ? f1c.v[k + 1] : f1r;
+          const T c0 = f2yp.v[k] - f2ym.v[k] - f1p.v[k] + f1m.v[k];
+          const T c1 = f0p.v[k] - f0m.v[k] - f2xr + f2xl;
+          const T c2 = f1xr - f1xl - f0yp.v[k] + f0ym.v[k];
+          const T o0 = an[0].v[k] + pF * c0, o1 = an[1].v[k] + pF * c1, o2 = an[2].v[k] + pF * c2;
+          an[0].v[k] = inner ? o0 : an[0].v[k];
+          an[1].v[k] = inner ? o1 : an[1].v[k];
+          an[2].v[k] = inner ? o2 : an[2].v[k];
+        }
+      }
+      // elementwise_ops_3d.py:427-447: B_i = u_j w_k - w_j u_k
+      V b0n, b1n, b2n;
+      for (int k = 0; k < VEC; ++k) {
+        b0n.v[k] = u1.v[k] * an[2].v[k] - an[1].v[k] * u2.v[k];
+        b1n.v[k] = u2.v[k] * an[0].v[k] - an[2].v[k] * u0.v[k];
+        b2n.v[k] = u0.v[k] * an[1].v[k] - an[0].v[k] * u1.v[k];
+      }
+      lds_st(Bq + lrow, b0n);
+      lds_st(Bq + ROWS + lrow, b2n);
+      lds_st(Bq + l1, b1n);
+    }
+    if (ev && pin) {
+      // B1 and B2 of the edge cell (xe, ey) on plane i
+      const int j = i * sz + ey * sy + xe;
+      T a0 = sl(w, j), a1 = sl(w + n, j), a2 = sl(w + 2 * n, j);
+      const T u0 = sl(u, j), u1 = sl(u + n, j), u2 = sl(u + 2 * n, j);
+      if (fw && g.interior(i, ey, xe)) {
+        const T c0 = sl(F2, j + sy) - sl(F2, j - sy) - sl(F1, j + sz) + sl(F1, j - sz);
+        const T c1 = sl(F0, j + sz) - sl(F0, j - sz) - sl(F2, j + 1) + sl(F2, j - 1);
+        const T c2 = sl(F1, j + 1) - sl(F1, j - 1) - sl(F0, j + sy) + sl(F0, j - sy);
+        a0 = a0 + pF * c0;
+        a1 = a1 + pF * c1;
+        a2 = a2 + pF * c2;
+      }
+      Bq[EOFF + e] = u2 * a0 - a2 * u0;
+      Bq[EOFF + 2 * TY + e] = u0 * a1 - a0 * u1;
+    }
+    __syncthreads();
+    const int r = i - 1;  // consumed plane (omega1 in a1p)
+    const T* const Bc = bs[(r + 3) % 3];
+    const T* const Bm = bs[(r + 2) % 3];  // plane r - 1
+    // B1 and B2 of the thread's cells on plane r and their x neighbours (every lane shuffles)
+    const V b1c = lds_ld(Bc + l1);
+    const V b2c = lds_ld(Bc + ROWS + lrow);
+    T b1l = __shfl_up(b1c.v[VEC - 1], 1, MXT), b1r = __shfl_down(b1c.v[0], 1, MXT);
+    T b2l = __shfl_up(b2c.v[VEC - 1], 1, MXT), b2r = __shfl_down(b2c.v[0], 1, MXT);
+    if (r >= z0 && tv) {
+      if (tx == 0) {
+        b1l = Bc[EOFF + ty];
+        b2l = Bc[EOFF + 2 * TY + ty];
+      }
+      if (tx == MXT - 1) {
+        b1r = Bc[EOFF + TY + ty];
+        b2r = Bc[EOFF + 3 * TY + ty];
+      }
+      V o[3] = {a1p[0], a1p[1], a1p[2]};
+      if (r > 0 && r < nz - 1 && yt > 0 && yt < ny - 1) {
+        const V b0yp = lds_ld(Bc + lrow + TX);
+        const V b0ym = lds_ld(Bc + lrow - TX);
+        const V b2yp = lds_ld(Bc + ROWS + lrow + TX);
+        const V b2ym = lds_ld(Bc + ROWS + lrow - TX);
+        const V b0zp = lds_ld(Bq + lrow), b0zm = lds_ld(Bm + lrow);
+        const V b1zp = lds_ld(Bq + l1), b1zm = lds_ld(Bm + l1);
+        for (int k = 0; k < VEC; ++k) {
+          const int x = xs + k;
+          const bool inner = x > 0 && x < nx - 1;
+          const T b2xl = k > 0 ? b2c.v[k - 1] : b2l, b2xr = k < VEC - 1 ? b2c.v[k + 1] : b2r;
+          const T b1xl = k > 0 ? b1c.v[k - 1] : b1l, b1xr = k < VEC - 1 ? b1c.v[k + 1] : b1r;
+          const T c0 = b2yp.v[k] - b2ym.v[k] - b1zp.v[k] + b1zm.v[k];
+          const T c1 = b0zp.v[k] - b0zm.v[k] - b2xr + b2xl;
+          const T c2 = b1xr - b1xl - b0yp.v[k] + b0ym.v[k];
+          const T q0 = o[0].v[k] + pB * c0, q1 = o[1].v[k] + pB * c1, q2 = o[2].v[k] + pB * c2;
+          o[0].v[k] = inner ? q0 : o[0].v[k];
+          o[1].v[k] = inner ? q1 : o[1].v[k];
+          o[2].v[k] = inner ? q2 : o[2].v[k];
+        }
+      }
+      const int j = r * sz + colj;
+      vst(w2out, j, o[0], true);
+      vst(w2out + n, j, o[1], true);
+      vst(w2out + 2 * n, j, o[2], true);
+      if (w1out) {  // omega1 as well (trajectories recorded for the adjoint)
+        vst(w1out, j, a1p[0], false);
+        vst(w1out + n, j, a1p[1], false);
+        vst(w1out + 2 * n, j, a1p[2], false);
+      }
+    }
+    a1p[0] = an[0];
+    a1p[1] = an[1];
+    a1p[2] = an[2];
+  }
+  if (fflags) {  // per tile (logical item index): does the tile's F need zeroing
+    const int any = __syncthreads_or(fset ? 1 : 0);
+    if (tid == 0) fflags[item] = any;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(S1VTile<T>::NT) __attribute__((amdgpu_waves_per_eu(4)))
+k_s1v(Grid g, const T* w, const T* u, const T* F, T* w2out, T pF, T pB, int with_forcing,
+      int zchunk, int tiles_x, int tiles_y, int remap, T* w1out, int* fflags) {
+  s1v_body<T>(g, w, u, F, w2out, pF, pB, with_forcing, zchunk, tiles_x, tiles_y, xcd_item(remap),
+              w1out, fflags);
+}
+
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 // SC: diffusion of omega2 at the clamped index, penalised with the per-axis sine factors
@@ -338,13 +640,13 @@ __device__ __forceinline__ void sc_body(Grid g, const T* __restrict__ w2, T* __r
                                         T* __restrict__ F, T pD, const Pen& pen,
                                         int zero_forcing, int tiles_x, int tiles_y,
                                         int64_t item, int chunk = kMarchChunk,
-                                        const int* fflags = nullptr, int s1_tiles_x = 0,
-                                        int s1_tiles_y = 0) {
+                                        const int* fflags = nullptr,
+                                        S1Map s1 = S1Map{0, 0, 1, kS1Chunk}) {
   using V = VecT<T, VEC>;
   const MarchTile m = march_tile<VEC>(g, tiles_x, tiles_y, item, chunk);
   if (zero_forcing && fflags && m.x0 < int(g.nx) && m.y < int(g.ny)) {
     // the S1 tile holding these cells saw no bit of F set: F is zero there already
-    const int t = ((m.z0 / kS1Chunk) * s1_tiles_y + m.y / S1Tile<T>::TY) * s1_tiles_x + m.x0 / S1Tile<T>::TX;
+    const int t = ((m.z0 / s1.chunk) * s1.tiles_y + m.y / s1.rows) * s1.tiles_x + m.x0 / 64;
     if (fflags[t] == 0) zero_forcing = 0;
   }
   const int nz = int(g.nz), ny = int(g.ny), nx = int(g.nx);
@@ -450,10 +752,9 @@ __global__ void __launch_bounds__(kMarchThreads) k_sc(Grid g, const T* __restric
                                                       T* __restrict__ w, T* __restrict__ F, T pD,
                                                       Pen pen, int zero_forcing, int tiles_x,
                                                       int tiles_y, int remap, int chunk,
-                                                      const int* fflags, int s1_tiles_x,
-                                                      int s1_tiles_y) {
+                                                      const int* fflags, S1Map s1) {
   sc_body<T, VEC>(g, w2, w, F, pD, pen, zero_forcing, tiles_x, tiles_y, xcd_item(remap), chunk,
-                  fflags, s1_tiles_x, s1_tiles_y);
+                  fflags, s1);
 }
 
 // S2: u = p curl_h(psi) (interior), 0 on the border, + U_inf.
@@ -554,14 +855,13 @@ inline int march_vec(int64_t nx, int width) {
 
 template <typename T>
 void launch_sc(const MarchLaunch& l, Grid g, const T* w2, T* w, T* F, T pD, const Pen& pen, int wf,
-               hipStream_t s, const int* fflags = nullptr, int s1_tiles_x = 0, int s1_tiles_y = 0) {
+               hipStream_t s, const int* fflags = nullptr, S1Map s1 = S1Map{0, 0, 1, kS1Chunk}) {
   if (l.vec == 1)
     k_sc<T, 1><<<l.blocks, kMarchThreads, 0, s>>>(g, w2, w, F, pD, pen, wf, l.tiles_x, l.tiles_y, l.remap,
-                                                  l.chunk, fflags, s1_tiles_x, s1_tiles_y);
+                                                  l.chunk, fflags, s1);
   else
     k_sc<T, 16 / sizeof(T)><<<l.blocks, kMarchThreads, 0, s>>>(g, w2, w, F, pD, pen, wf, l.tiles_x,
-                                                                l.tiles_y, l.remap, l.chunk, fflags,
-                                                                s1_tiles_x, s1_tiles_y);
+                                                                l.tiles_y, l.remap, l.chunk, fflags, s1);
 }
 
 template <typename T>

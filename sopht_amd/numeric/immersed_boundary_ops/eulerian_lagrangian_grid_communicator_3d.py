@@ -96,7 +96,7 @@ class EulerianLagrangianGridCommunicator3D:
 
     def _eul_ptrs(self, eul_grid_field):
         if eul_grid_field.dim() == 4:
-            return L.comps(eul_grid_field), L.strides3(eul_grid_field[0]), eul_grid_field.shape[1:], 3
+            return L.comps(eul_grid_field), L.strides3(eul_grid_field), eul_grid_field.shape[1:], 3
         p = (C.c_void_p * 3)(eul_grid_field.data_ptr(), 0, 0)
         return p, L.strides3(eul_grid_field), eul_grid_field.shape, 1
 

@@ -191,7 +191,7 @@ class VirtualBoundaryForcing:
         u = eul_grid_velocity_field
         nz, ny, nx = (int(s) for s in u.shape[1:])
         L.call(
-            "sopht_vbf_lag_force_3d", self._tag, self._ktype, L.comps(u), L.strides3(u[0]), nz, ny, nx,
+            "sopht_vbf_lag_force_3d", self._tag, self._ktype, L.comps(u), L.strides3(u), nz, ny, nx,
             L.ptr(X), L.ptr(V), self.num_lag_nodes, float(self.dx), float(self.eul_grid_coord_shift),
             float(self.virtual_boundary_stiffness_coeff), float(self.virtual_boundary_damping_coeff),
             L.ptr(self.nearest_eul_grid_index_to_lag_grid), L.ptr(self.local_eul_grid_support_of_lag_grid),

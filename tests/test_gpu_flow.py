@@ -80,6 +80,58 @@ def test_one_step_non_cubic_matches_oracle(dev, shape, precision, rng):
         assert rel_l2(a, b) <= bound, (name, rel_l2(a, b))
 
 
+@pytest.mark.parametrize("shape", [(24, 24, 48), (12, 48, 96), (64, 12, 128), (48, 16, 8), (4, 4, 4)])
+@pytest.mark.parametrize("precision", ["single", "double"])
+@pytest.mark.parametrize("chunk", [16, 32])
+def test_vector_s1_sweep_bit_identical_to_scalar(dev, shape, precision, chunk, rng, monkeypatch):
+    """The S1 sweep with 16-byte accesses (rows a multiple of 16 bytes; forced here with
+    ``SOPHT_S1_CHUNK``, as these grids are below the size that selects it) against the scalar
+    sweep (``SOPHT_S1_SCALAR=1``): the same bits in every field after one step, with partial
+    x, y and z tiles, forcing nonzero only in a sub-box (the F-zeroing flags per tile, read by
+    SC's tiles of 16 planes), and the out-of-place step's omega1 output."""
+    import torch
+
+    real_t = PREC[precision]
+    kw = dict(kinematic_viscosity=1e-2, time=0.0, with_forcing=True, with_free_stream_flow=True)
+    sims = []
+    for scalar in (False, True):
+        if scalar:
+            monkeypatch.delenv("SOPHT_S1_CHUNK")
+            monkeypatch.setenv("SOPHT_S1_SCALAR", "1")
+        else:
+            monkeypatch.setenv("SOPHT_S1_CHUNK", str(chunk))
+        sims.append(sps().UnboundedNavierStokesFlowSimulator3D(grid_size=shape, x_range=1.0,
+                                                               real_t=real_t, **kw))
+    monkeypatch.delenv("SOPHT_S1_SCALAR")
+    from sopht_amd import _lib as L
+
+    assert L.schedule("sopht_ns3d_s1_schedule", sims[0]._plan) == (1, chunk)
+    assert L.schedule("sopht_ns3d_s1_schedule", sims[1]._plan) == (0, 16)
+    w0, u0 = (rng.standard_normal((3,) + shape).astype(real_t) for _ in range(2))
+    F0 = np.zeros((3,) + shape, dtype=real_t)
+    nz, ny, nx = shape
+    F0[:, nz // 4: nz // 2 + 1, ny // 3: ny // 2 + 2, nx // 5: nx // 2 + 1] = rng.standard_normal(
+        (3, nz // 2 + 1 - nz // 4, ny // 2 + 2 - ny // 3, nx // 2 + 1 - nx // 5)).astype(real_t)
+    U = np.array([1.0, 0.5, -0.25])
+    for g in sims:
+        g.vorticity_field.copy_(to_dev(w0, dev))
+        g.velocity_field.copy_(to_dev(u0, dev))
+        g.eul_grid_forcing_field.copy_(to_dev(F0, dev))
+        g.time_step(dt=1e-2, free_stream_velocity=U)
+    a, b = sims
+    for name in ("vorticity_field", "velocity_field", "stream_func_field", "eul_grid_forcing_field"):
+        assert torch.equal(getattr(a, name), getattr(b, name)), name
+    assert np.all(to_host(a.eul_grid_forcing_field) == 0)
+    outs = []
+    for g in sims:
+        win, uin, F = (to_dev(x, dev) for x in (w0, u0, F0))
+        wo, uo, w1 = torch.empty_like(win), torch.empty_like(uin), torch.empty_like(win)
+        g.trajectory_step(win, uin, F, wo, uo, 1e-2, U, vorticity1_out=w1)
+        outs.append((wo, uo, w1))
+    for x, y in zip(*outs):
+        assert torch.equal(x, y)
+
+
 @pytest.mark.parametrize("precision", ["single", "double"])
 def test_stable_timestep(dev, precision):
     real_t = PREC[precision]
