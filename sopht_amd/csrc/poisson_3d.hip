@@ -131,6 +131,13 @@ struct SlabGeom {
   __device__ __forceinline__ int hi(int c, int z, int ky, int kx) const {
     return ((c * int(nz) + z) * (2 * int(ny)) + ky) * int(rs) + kx;
   }
+  // The three-pass schedule keeps the slab in 128-byte column tiles (FP32: 16 columns):
+  // (c, ky, kx / 16, z, kx % 16). A tile's z line is then one contiguous run, which the z pass
+  // reads and writes whole, while the x/y passes still move whole 128-byte lines (rs is a
+  // multiple of 16 there).
+  __device__ __forceinline__ int ti(int c, int z, int ky, int kx) const {
+    return (((c * 2 * int(ny) + ky) * (int(rs) >> 4) + (kx >> 4)) * int(nz) + z) * 16 + (kx & 15);
+  }
   __device__ __forceinline__ int64_t row_h(int64_t r) const {  // H row of input row r
     const int c = int(r / (nz * ny));
     const int64_t rem = r % (nz * ny);
@@ -612,8 +619,6 @@ __global__ void __launch_bounds__((XYPass<T, LX, LY>::NT)) ka_xy_fwd(SlabGeom g,
   Cplx<T>* const creg = U + col * F::S2;
   // staging rows to HBM: column sc of rows sr + q RSTEP
   const int sc = tid % CW, sr = tid / CW;
-  Cplx<T>* const hrow = H + g.hi(c, z, y1 * L2 + sr, sc);
-  const int rstride = F::RSTEP * int(g.rs);
   static_for<0, F::NCH>([&](auto ch_) {
     constexpr int ch = decltype(ch_)::value;
     constexpr bool last = ch == EX / 2;  // kx = nx alone
@@ -654,7 +659,7 @@ __global__ void __launch_bounds__((XYPass<T, LX, LY>::NT)) ka_xy_fwd(SlabGeom g,
     if (!last || sc == 0) {
       static_for<0, F::NQ>([&](auto q_) {
         constexpr int q = decltype(q_)::value;
-        store_stream(hrow + q * rstride + TX * ch, U[(sr + q * F::RSTEP) * P + sc]);
+        store_stream(H + g.ti(c, z, y1 * L2 + sr + q * F::RSTEP, TX * ch + sc), U[(sr + q * F::RSTEP) * P + sc]);
       });
     }
     __syncthreads();
@@ -697,7 +702,7 @@ __global__ void __launch_bounds__((XYPass<T, LX, LY>::NT)) kc_yx_inv(SlabGeom g,
     static_for<0, F::NQ>([&](auto q_) {
       constexpr int q = decltype(q_)::value;
       const int i = tid + q * NT;
-      if (i < L2 * ncol) pre[q] = H[g.hi(c, z, y1 * L2 + i / ncol, TX * ch + i % ncol)];
+      if (i < L2 * ncol) pre[q] = H[g.ti(c, z, y1 * L2 + i / ncol, TX * ch + i % ncol)];
     });
   };
   load_chunk(std::integral_constant<int, 0>{});
@@ -878,7 +883,7 @@ __global__ void __launch_bounds__((ZRowTile<T, L>::NT)) __attribute__((amdgpu_wa
       if (it + 1 < NIT || i < NG) gl[i] = gt[it];
     });
   }
-  const int zs = 2 * int(g.ny) * int(g.rs);
+  constexpr int zs = 16;  // consecutive z of a column tile (ti layout)
   using P3 = Plan3<typename LinePlan<L>::R>;
   constexpr int R1 = at_least_1(P3::r1), R2 = at_least_1(P3::r2);
   constexpr int PER2 = at_least_1(E / R1), PER3 = at_least_1(E / R2);
@@ -897,7 +902,7 @@ __global__ void __launch_bounds__((ZRowTile<T, L>::NT)) __attribute__((amdgpu_wa
     const int j = jbeg + jl;
     const int cc = j % g.ncomp;
     const int cls = j < g.ncomp ? int(k2) : L2 - int(k2);
-    return g.hi(cc, 0, y1 * L2 + cls, live ? int(kx) : 0);
+    return g.ti(cc, 0, y1 * L2 + cls, live ? int(kx) : 0);
   };
   Cplx<T> pf[4][KL];
   auto load = [&](int j) {
@@ -1474,7 +1479,7 @@ int poisson_create(int dtype, int64_t nz, int64_t ny, int64_t nx, double x_range
   // keeps the five-pass one
   const char* forced = std::getenv("SOPHT_POISSON_PASSES");
   const bool five = forced != nullptr && std::atoi(forced) == 5;
-  P->passes = (!five && dtype == SOPHT_F32 && nx == ny && xy_length_supported(2 * nx) &&
+  P->passes = (!five && kRowPad && dtype == SOPHT_F32 && nx == ny && xy_length_supported(2 * nx) &&
                zrow_length_supported(2 * nz)) ? 3 : 5;
   int rc = SOPHT_OK;
   auto fail = [&](int code) {
