@@ -71,6 +71,15 @@ constexpr int kLineCplx = 128 / int(sizeof(Cplx<T>));
 // other, so the second half is read from that XCD's L2.
 constexpr int kColHalf = SOPHT_COL_HALF;
 constexpr int64_t kRowPad = SOPHT_ROW_PAD;
+// Columns per slab tile of the three-pass schedule. Measured at 256^3 FP32 (pass A + z pass +
+// pass C): 16 columns 1.21 ms, 8 columns 1.20 ms (the z pass faster, passes A and C slower),
+// 4 columns 2.2 ms (pass A's writes become 32-byte pieces of lines from four planes, which
+// other workgroups write).
+#ifndef SOPHT_SLAB_TILE
+#define SOPHT_SLAB_TILE 16
+#endif
+constexpr int kSlabTile = SOPHT_SLAB_TILE;
+static_assert(16 % kSlabTile == 0, "slab tiles divide a 128-byte line");
 
 // (row, first kx) of the column tile of workgroup b; false for the padding workgroups.
 __device__ __forceinline__ bool col_tile(int64_t b, int64_t ncols, int64_t nrows, int wc,
@@ -131,12 +140,13 @@ struct SlabGeom {
   __device__ __forceinline__ int hi(int c, int z, int ky, int kx) const {
     return ((c * int(nz) + z) * (2 * int(ny)) + ky) * int(rs) + kx;
   }
-  // The three-pass schedule keeps the slab in 128-byte column tiles (FP32: 16 columns):
-  // (c, ky, kx / 16, z, kx % 16). A tile's z line is then one contiguous run, which the z pass
-  // reads and writes whole, while the x/y passes still move whole 128-byte lines (rs is a
-  // multiple of 16 there).
+  // The three-pass schedule keeps the slab in column tiles of kSlabTile columns with z inside
+  // a tile: (c, ky, kx / kSlabTile, z, kx % kSlabTile). The z pass then reads and writes each
+  // tile's z line as one contiguous run; with 16 columns (one 128-byte line per plane) the x/y
+  // passes still move whole lines (rs is a multiple of 16 there).
   __device__ __forceinline__ int ti(int c, int z, int ky, int kx) const {
-    return (((c * 2 * int(ny) + ky) * (int(rs) >> 4) + (kx >> 4)) * int(nz) + z) * 16 + (kx & 15);
+    return (((c * 2 * int(ny) + ky) * (int(rs) / kSlabTile) + kx / kSlabTile) * int(nz) + z) * kSlabTile +
+           kx % kSlabTile;
   }
   __device__ __forceinline__ int64_t row_h(int64_t r) const {  // H row of input row r
     const int c = int(r / (nz * ny));
@@ -883,7 +893,7 @@ __global__ void __launch_bounds__((ZRowTile<T, L>::NT)) __attribute__((amdgpu_wa
       if (it + 1 < NIT || i < NG) gl[i] = gt[it];
     });
   }
-  constexpr int zs = 16;  // consecutive z of a column tile (ti layout)
+  constexpr int zs = kSlabTile;  // consecutive z of a column tile (ti layout)
   using P3 = Plan3<typename LinePlan<L>::R>;
   constexpr int R1 = at_least_1(P3::r1), R2 = at_least_1(P3::r2);
   constexpr int PER2 = at_least_1(E / R1), PER3 = at_least_1(E / R2);
