@@ -144,9 +144,12 @@ struct SlabGeom {
   // a tile: (c, ky, kx / kSlabTile, z, kx % kSlabTile). The z pass then reads and writes each
   // tile's z line as one contiguous run; with 16 columns (one 128-byte line per plane) the x/y
   // passes still move whole lines (rs is a multiple of 16 there).
+  // (Unsigned arithmetic: the divisions by the power-of-two tile width are shifts.) Moving by
+  // dr rows adds dr rs nz; moving by a whole number of tiles, dk columns, adds dk nz.
   __device__ __forceinline__ int ti(int c, int z, int ky, int kx) const {
-    return (((c * 2 * int(ny) + ky) * (int(rs) / kSlabTile) + kx / kSlabTile) * int(nz) + z) * kSlabTile +
-           kx % kSlabTile;
+    const unsigned k = unsigned(kx);
+    return int((((unsigned(c) * 2u * unsigned(ny) + unsigned(ky)) * (unsigned(rs) / kSlabTile) + k / kSlabTile) *
+                    unsigned(nz) + unsigned(z)) * kSlabTile + k % kSlabTile);
   }
   __device__ __forceinline__ int64_t row_h(int64_t r) const {  // H row of input row r
     const int c = int(r / (nz * ny));
@@ -706,13 +709,20 @@ __global__ void __launch_bounds__((XYPass<T, LX, LY>::NT)) kc_yx_inv(SlabGeom g,
   // the rows of a chunk: thread tid holds elements tid + NT q of the chunk's L2 x ncol block
   // (addresses formed per load: a hoisted row pointer costs the registers of the prefetch)
   Cplx<T> pre[F::NQ];
+  const int hsrc = g.ti(c, z, y1 * L2 + tid / CW, tid % CW);
+  const int rowel = int(g.rs) * int(g.nz);  // elements per slab row
   auto load_chunk = [&](auto ch_) {
     constexpr int ch = decltype(ch_)::value;
     constexpr int ncol = ch == EX / 2 ? 1 : CW;
     static_for<0, F::NQ>([&](auto q_) {
       constexpr int q = decltype(q_)::value;
       const int i = tid + q * NT;
-      if (i < L2 * ncol) pre[q] = H[g.ti(c, z, y1 * L2 + i / ncol, TX * ch + i % ncol)];
+      if constexpr (ncol == CW && TX % kSlabTile == 0) {
+        // row i / CW = tid / CW + q NT / CW, column TX ch + tid % CW
+        pre[q] = H[hsrc + q * (NT / CW) * rowel + TX * ch * int(g.nz)];
+      } else {
+        if (i < L2 * ncol) pre[q] = H[g.ti(c, z, y1 * L2 + i / ncol, TX * ch + i % ncol)];
+      }
     });
   };
   load_chunk(std::integral_constant<int, 0>{});
