@@ -934,6 +934,9 @@ __global__ void __launch_bounds__(64 * kMergeWaves) k_spread_merge(
                        stage[w], table, bb, use_table);
 }
 
+// contributors per cell whose list entries a replay thread loads all at once
+constexpr int kReplayGather = 32;
+
 // Plan replay for the batched step (every call; one thread per support slot): each owned cell
 // adds its contributors F_lag[node] w[slot][node] in increasing node order, every product and
 // sum rounded separately, exactly as the reference's serial `F[block] += F_n * w_n`.
@@ -954,14 +957,47 @@ __device__ __forceinline__ void spread_replay_slot(int ncomp, const EulW3<T>& F,
   T acc[3];
   for (int c = 0; c < ncomp; ++c) acc[c] = F.p[c][cidx];
   const int o = poff[gid];
-  // unrolled so that the loads of several contributions are in flight together; the sums
-  // still run one after another in list order
+  if (m <= kReplayGather) {
+    // every list entry of the cell is loaded first and then the products in groups of eight,
+    // so the loads of many contributions are in flight together; the sums still run one after
+    // another in list order
+    uint32_t es[kReplayGather];
+    static_for_sort<0, kReplayGather>([&](auto k_) {
+      constexpr int k = decltype(k_)::value;
+      es[k] = k < m ? plist[o + k] : 0u;
+    });
+    static_for_sort<0, kReplayGather / 8>([&](auto g_) {
+      constexpr int g0 = 8 * decltype(g_)::value;
+      if (g0 < m) {
+        T pr[8][3];
+        static_for_sort<0, 8>([&](auto k_) {
+          constexpr int k = decltype(k_)::value;
+          const int64_t node = int64_t(es[g0 + k] >> 6);
+          const bool live = g0 + k < m;
+          const T w = live ? weights[int64_t(es[g0 + k] & 63u) * n + node] : T(0);
+          static_for_sort<0, 3>([&](auto c_) {
+            constexpr int c = decltype(c_)::value;
+            pr[k][c] = (live && c < ncomp) ? lag[c * n + node] * w : T(0);
+          });
+        });
+        static_for_sort<0, 8>([&](auto k_) {
+          constexpr int k = decltype(k_)::value;
+          if (g0 + k < m)
+            static_for_sort<0, 3>([&](auto c_) {
+              constexpr int c = decltype(c_)::value;
+              if (c < ncomp) acc[c] = acc[c] + pr[k][c];
+            });
+        });
+      }
+    });
+  } else {
 #pragma unroll 4
-  for (int k = 0; k < m; ++k) {
-    const uint32_t e = plist[o + k];
-    const int64_t node = int64_t(e >> 6);
-    const T w = weights[int64_t(e & 63u) * n + node];
-    for (int c = 0; c < ncomp; ++c) acc[c] = acc[c] + lag[c * n + node] * w;
+    for (int k = 0; k < m; ++k) {
+      const uint32_t e = plist[o + k];
+      const int64_t node = int64_t(e >> 6);
+      const T w = weights[int64_t(e & 63u) * n + node];
+      for (int c = 0; c < ncomp; ++c) acc[c] = acc[c] + lag[c * n + node] * w;
+    }
   }
   for (int c = 0; c < ncomp; ++c) F.p[c][cidx] = acc[c];
 }
