@@ -14,6 +14,7 @@
 // the members one by one.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <map>
 #include <new>
 #include <tuple>
@@ -206,17 +207,29 @@ k_s1_batch(const Member<T>* M, const int64_t* off, int count, int remap, int* ff
              item - off[m], nullptr, fflags + off[m]);
 }
 
+template <typename T>
+__global__ void __launch_bounds__(S1VTile<T>::NT) __attribute__((amdgpu_waves_per_eu(2)))
+k_s1v_batch(const Member<T>* M, const int64_t* off, int count, int remap, int* fflags) {
+  const int64_t item = xcd_item(remap);
+  const int m = find_member(off, count, item);
+  const Member<T>& S = M[m];
+  const T pF = T(S.dt / (2.0 * S.dx * S.rho));
+  const T pB = T(S.dt / (2.0 * S.dx));
+  s1v_body<T>(S.g, S.w, S.u, S.F, S.psi, pF, pB, S.with_forcing, kS1Chunk, S.s1_tx, S.s1_ty,
+              item - off[m], nullptr, fflags + off[m]);
+}
+
 template <typename T, int VEC>
 __global__ void __launch_bounds__(kMarchThreads) k_sc_batch(const Member<T>* M, const int64_t* off,
                                                             int count, int remap, const int* fflags,
-                                                            const int64_t* off_s1) {
+                                                            const int64_t* off_s1, int s1_rows) {
   const int64_t item = xcd_item(remap);
   const int m = find_member(off, count, item);
   const Member<T>& S = M[m];
   const T pD = T(S.nu * S.dt / S.dx / S.dx);
   // F is zeroed only in the S1 tiles that saw a bit of it set (elsewhere it is zero already)
   sc_body<T, VEC>(S.g, S.psi, S.w, S.F, pD, S.pen, S.with_forcing, S.m_tx, S.m_ty, item - off[m],
-                  kMarchChunk, fflags + off_s1[m], S1Map{S.s1_tx, S.s1_ty, S1Tile<T>::TY, kS1Chunk});
+                  kMarchChunk, fflags + off_s1[m], S1Map{S.s1_tx, S.s1_ty, s1_rows, kS1Chunk});
 }
 
 template <typename T, int VEC>
@@ -260,6 +273,8 @@ struct sopht_ns3d_batch {
   void* s2max = nullptr;    // per S2 work item maxima of the last step's velocity
   int plans_built = 0;      // the spreading plans of the (static) bodies exist
   int* s1flags = nullptr;   // per S1 work item: did it see a bit of F set
+  int s1v = 0;              // S1 with 16-byte accesses (k_s1v_batch)
+  int s1_rows = 0;          // rows per S1 tile
   std::vector<BatchClass> classes;
   std::vector<sopht_spread3d*> ws;  // spread workspaces of the members with bodies
   int max_padded = 0;
@@ -286,6 +301,15 @@ int batch_build(sopht_ns3d_batch* B, const sopht_batch_member* mem, hipStream_t 
   int vec = 16 / int(sizeof(T));
   for (int m = 0; m < count; ++m) vec = std::min(vec, march_vec<T>(mem[m].nx, mem[m].width));
   B->vec = vec;
+  // the S1 sweep with 16-byte accesses when every member's rows allow it (the same bits as
+  // the scalar sweep; component byte offsets must stay below 2^32)
+  B->s1v = vec > 1;
+  for (int m = 0; m < count; ++m)
+    if (mem[m].nz * mem[m].ny * mem[m].nx * int64_t(sizeof(T)) >= (int64_t(1) << 32)) B->s1v = 0;
+  if (const char* scalar = std::getenv("SOPHT_S1_SCALAR"))
+    if (std::atoi(scalar) == 1) B->s1v = 0;
+  const int s1_rows = B->s1v ? S1VTile<T>::TY : S1Tile<T>::TY;
+  B->s1_rows = s1_rows;
   std::map<std::tuple<int64_t, int64_t, int64_t, double>, std::vector<int>> by_grid;
   for (int m = 0; m < count; ++m) {
     const sopht_batch_member& q = mem[m];
@@ -318,8 +342,8 @@ int batch_build(sopht_ns3d_batch* B, const sopht_batch_member* mem, hipStream_t 
         S.pen.f[a][i] = std::sin(pre * (double(i) * S.dx));
         S.pen.f[a][q.width + i] = std::sin(pre * (double(q.width - 1 - i) * S.dx));
       }
-    S.s1_tx = int(ceil_div(q.nx, S1Tile<T>::TX));
-    S.s1_ty = int(ceil_div(q.ny, S1Tile<T>::TY));
+    S.s1_tx = int(ceil_div(q.nx, 64));
+    S.s1_ty = int(ceil_div(q.ny, s1_rows));
     const MarchLaunch ml = march_launch(q.nz, q.ny, q.nx, vec);
     S.m_tx = ml.tiles_x;
     S.m_ty = ml.tiles_y;
@@ -469,15 +493,18 @@ int batch_step(sopht_ns3d_batch* B, hipStream_t s, bool fused_cfl) {
   }
   // flow
   const int r1 = B->tot[kOffS1] % 8 == 0 ? 1 : 0;
-  k_s1_batch<T><<<nblocks(B->tot[kOffS1]), S1Tile<T>::NT, 0, s>>>(M, O(kOffS1), count, r1, B->s1flags);
+  if (B->s1v)
+    k_s1v_batch<T><<<nblocks(B->tot[kOffS1]), S1VTile<T>::NT, 0, s>>>(M, O(kOffS1), count, r1, B->s1flags);
+  else
+    k_s1_batch<T><<<nblocks(B->tot[kOffS1]), S1Tile<T>::NT, 0, s>>>(M, O(kOffS1), count, r1, B->s1flags);
   SOPHT_CHECK_LAUNCH();
   const int rm = B->tot[kOffMarch] % 8 == 0 ? 1 : 0;
   if (B->vec == 1)
     k_sc_batch<T, 1><<<nblocks(B->tot[kOffMarch]), kMarchThreads, 0, s>>>(M, O(kOffMarch), count, rm,
-                                                                         B->s1flags, O(kOffS1));
+                                                                         B->s1flags, O(kOffS1), B->s1_rows);
   else
     k_sc_batch<T, 16 / sizeof(T)><<<nblocks(B->tot[kOffMarch]), kMarchThreads, 0, s>>>(
-        M, O(kOffMarch), count, rm, B->s1flags, O(kOffS1));
+        M, O(kOffMarch), count, rm, B->s1flags, O(kOffS1), B->s1_rows);
   SOPHT_CHECK_LAUNCH();
   for (BatchClass& C : B->classes) {
     int rc = poisson_table_solve(C.plan, C.ncomp, C.sol_tab, C.strides, C.rhs_tab, C.strides, s);
