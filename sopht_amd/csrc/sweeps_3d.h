@@ -328,10 +328,11 @@ __device__ __forceinline__ void x_nbrs(const VecT<T, VEC>& v, const T* __restric
   }
 }
 
-// S1 with 16-byte accesses (rows whose length is a multiple of VEC). The scalar sweep above is
-// bound by the texture-address unit: one instruction per array, plane and cell. Here a thread
-// holds VEC neighbouring cells of one row, so each array and plane costs one 16-byte load per
-// VEC cells. The tile is 64 cells by TY rows and marches a chunk of z planes as s1_body does.
+// S1 with 16-byte accesses (rows whose length is a multiple of VEC). The scalar sweep above
+// issues one load instruction per array, plane and cell. Here a thread holds VEC neighbouring
+// cells of one row, so each array and plane costs one 16-byte load per VEC cells (3.4x fewer
+// load instructions; the kernel is then bound by HBM traffic, 1.5x its algorithmic bytes at
+// 256^3). The tile is 64 cells by TY rows and marches a chunk of z planes as s1_body does.
 // Per plane i:
 //   - the tile threads form omega1 and B on their cells; the x neighbours of F come from the
 //     adjacent lanes of the row, its y neighbours are row loads, and F0 and F1 ride along z in
@@ -344,8 +345,8 @@ __device__ __forceinline__ void x_nbrs(const VecT<T, VEC>& v, const T* __restric
 // adjacent lanes, its y and z derivatives from LDS. From one plane to the next, registers keep
 // omega1 of the consumed plane and F0 and F1 of the thread's cells (their z window). The
 // F0/F2 rows also going through LDS (for their y neighbours) cut the sweep's HBM fetch from
-// 500 to 435 MB at 256^3 FP32 but not its time (0.229 against 0.224 ms), and were not kept. Every expression keeps
-// the operand order of s1_body, so both sweeps give the same bits.
+// 500 to 435 MB at 256^3 FP32 but not its time (0.229 against 0.224 ms), and were not kept.
+// Every expression keeps the operand order of s1_body, so both sweeps give the same bits.
 template <typename T>
 struct S1VTile {
   static constexpr int VEC = 16 / int(sizeof(T));
