@@ -176,7 +176,7 @@ __global__ void __launch_bounds__(512) k_spread_replay_batch(const Member<T>* M,
   const int64_t gid = (int64_t(blockIdx.x) - off[m]) * 512 + threadIdx.x;
   if (gid == 0) atomicExch(S.flag + 1, 0);
   if (gid < 64 * S.vb.n)
-    spread_replay_slot<T>(3, S.Fw, S.g.nz, S.g.ny, S.g.nx, S.vb.F, S.vb.weights, S.keys, S.vb.n,
+    spread_replay_slot<T, false>(3, S.Fw, S.g.nz, S.g.ny, S.g.nx, S.vb.F, S.vb.weights, S.keys, S.vb.n,
                           S.pcount, S.poff, S.plist, gid);
 }
 
