@@ -531,11 +531,24 @@ __global__ void __launch_bounds__((RowTile<T, L>::NT)) k5_xc2r(SlabGeom g, const
 // twiddle and the inner sums and runs the inverse x transforms. So the x-pass outputs
 // (K1 -> K2) and the inverse y-pass outputs (K4 -> K5) of the five-pass schedule never reach
 // HBM: 24 words per cell fewer.
-//
+
+// Elements of a thread's half spectrum per chunk of the y sums: the most, up to 64 / TX, that
+// divide EX / 2 and keep the chunk's column teams and staging rows within the workgroup.
+constexpr int xy_chunk_elems(int EX, int TX, int T2, int NT, int L2) {
+  int best = 1;
+  for (int k = 1; k == 1 || k * TX <= 64; ++k) {
+    const int cw = k * TX;
+    if ((EX / 2) % k == 0 && cw * T2 <= NT && NT % cw == 0 && L2 % (NT / cw) == 0) best = k;
+  }
+  return best;
+}
+
 // A workgroup of pass A or C holds the ny / 4 rows of its (c, z, y1) as row pairs (two real
 // rows per complex x transform), PPT pairs per team of TX threads. The y sums run on chunks of
-// TX columns (kx = TX ch + t, the ch-th element of every x thread; the last chunk is kx = nx
-// alone), one team of T2 threads per column, staged through LDS.
+// CW = KC TX columns (kx = CW ch + TX kk + t, element KC ch + kk of every x thread; the last
+// chunk is kx = nx alone), one team of T2 threads per column, staged through LDS. Lines of
+// fewer than 64 x threads take KC > 1, so a chunk still spans a wavefront's width of columns
+// and the workgroup passes fewer chunk barriers.
 template <typename T, int LX, int LY>
 struct XYPass {
   static constexpr int EX = LinePlan<LX>::E, TX = LinePlan<LX>::T;
@@ -546,8 +559,9 @@ struct XYPass {
   static constexpr int NTEAM = NPAIR * TX <= 1024 ? NPAIR : 1024 / TX;
   static constexpr int PPT = NPAIR / NTEAM;  // row pairs per x team
   static constexpr int NT = NTEAM * TX;
-  static constexpr int CW = TX;              // columns per chunk
-  static constexpr int NCH = EX / 2 + 1;
+  static constexpr int KC = xy_chunk_elems(EX, TX, T2, NT, L2);
+  static constexpr int CW = KC * TX;         // columns per chunk: kx = CW ch + kk TX + t, kk < KC
+  static constexpr int NCH = EX / 2 / KC + 1;  // the last chunk is kx = nx alone
   static constexpr int SX = region_stride<LX, false>();
   static constexpr int S2 = region_stride<L2, true>();
   // staging tile [row][P]: the column teams' accesses (T2 rows of 32 / T2 columns per half
@@ -563,7 +577,7 @@ struct XYPass {
   // tid % CW of the rows tid / CW + q NT / CW
   static constexpr int RSTEP = NT / CW, NQ = L2 / RSTEP;
   static_assert(LY % 8 == 0 && NPAIR % NTEAM == 0 && NTEAM >= T2 && T2 <= 32, "three-pass geometry");
-  static_assert(NT % CW == 0 && L2 % RSTEP == 0, "whole staging rows per thread");
+  static_assert(NT % CW == 0 && L2 % RSTEP == 0 && CW * T2 <= NT, "whole staging rows per thread");
 };
 
 // Pass A: x real-to-complex of the rows y = 4 y2 + y1 of (c, z) (two rows per complex FFT,
@@ -634,13 +648,25 @@ __global__ void __launch_bounds__((XYPass<T, LX, LY>::NT)) ka_xy_fwd(SlabGeom g,
   const int sc = tid % CW, sr = tid / CW;
   static_for<0, F::NCH>([&](auto ch_) {
     constexpr int ch = decltype(ch_)::value;
-    constexpr bool last = ch == EX / 2;  // kx = nx alone
-    if (!last || t == 0) {
-      static_for<0, PPT>([&](auto p_) {
-        constexpr int p = decltype(p_)::value;
-        const int y2 = 2 * (team * PPT + p);
-        U[y2 * P + t] = A[p][ch];
-        U[(y2 + 1) * P + t] = B[p][ch];
+    constexpr bool last = ch == F::NCH - 1;  // kx = nx alone
+    if constexpr (last) {
+      if (t == 0) {
+        static_for<0, PPT>([&](auto p_) {
+          constexpr int p = decltype(p_)::value;
+          const int y2 = 2 * (team * PPT + p);
+          U[y2 * P] = A[p][EX / 2];
+          U[(y2 + 1) * P] = B[p][EX / 2];
+        });
+      }
+    } else {
+      static_for<0, F::KC>([&](auto k_) {
+        constexpr int kk = decltype(k_)::value;
+        static_for<0, PPT>([&](auto p_) {
+          constexpr int p = decltype(p_)::value;
+          const int y2 = 2 * (team * PPT + p);
+          U[y2 * P + kk * TX + t] = A[p][ch * F::KC + kk];
+          U[(y2 + 1) * P + kk * TX + t] = B[p][ch * F::KC + kk];
+        });
       });
     }
     __syncthreads();
@@ -672,7 +698,7 @@ __global__ void __launch_bounds__((XYPass<T, LX, LY>::NT)) ka_xy_fwd(SlabGeom g,
     if (!last || sc == 0) {
       static_for<0, F::NQ>([&](auto q_) {
         constexpr int q = decltype(q_)::value;
-        store_stream(H + g.ti(c, z, y1 * L2 + sr + q * F::RSTEP, TX * ch + sc), U[(sr + q * F::RSTEP) * P + sc]);
+        store_stream(H + g.ti(c, z, y1 * L2 + sr + q * F::RSTEP, CW * ch + sc), U[(sr + q * F::RSTEP) * P + sc]);
       });
     }
     __syncthreads();
@@ -713,15 +739,15 @@ __global__ void __launch_bounds__((XYPass<T, LX, LY>::NT)) kc_yx_inv(SlabGeom g,
   const int rowel = int(g.rs) * int(g.nz);  // elements per slab row
   auto load_chunk = [&](auto ch_) {
     constexpr int ch = decltype(ch_)::value;
-    constexpr int ncol = ch == EX / 2 ? 1 : CW;
+    constexpr int ncol = ch == F::NCH - 1 ? 1 : CW;
     static_for<0, F::NQ>([&](auto q_) {
       constexpr int q = decltype(q_)::value;
       const int i = tid + q * NT;
-      if constexpr (ncol == CW && TX % kSlabTile == 0) {
-        // row i / CW = tid / CW + q NT / CW, column TX ch + tid % CW
-        pre[q] = H[hsrc + q * (NT / CW) * rowel + TX * ch * int(g.nz)];
+      if constexpr (ncol == CW && CW % kSlabTile == 0) {
+        // row i / CW = tid / CW + q NT / CW, column CW ch + tid % CW
+        pre[q] = H[hsrc + q * (NT / CW) * rowel + CW * ch * int(g.nz)];
       } else {
-        if (i < L2 * ncol) pre[q] = H[g.ti(c, z, y1 * L2 + i / ncol, TX * ch + i % ncol)];
+        if (i < L2 * ncol) pre[q] = H[g.ti(c, z, y1 * L2 + i / ncol, CW * ch + i % ncol)];
       }
     });
   };
@@ -729,7 +755,7 @@ __global__ void __launch_bounds__((XYPass<T, LX, LY>::NT)) kc_yx_inv(SlabGeom g,
   Cplx<T> v[PPT][EX];
   static_for<0, F::NCH>([&](auto ch_) {
     constexpr int ch = decltype(ch_)::value;
-    constexpr bool last = ch == EX / 2;
+    constexpr bool last = ch == F::NCH - 1;
     constexpr int ncol = last ? 1 : CW;
     static_for<0, F::NQ>([&](auto q_) {
       constexpr int q = decltype(q_)::value;
@@ -756,30 +782,34 @@ __global__ void __launch_bounds__((XYPass<T, LX, LY>::NT)) kc_yx_inv(SlabGeom g,
       });
     }
     __syncthreads();
-    static_for<0, PPT>([&](auto p_) {
-      constexpr int p = decltype(p_)::value;
-      const int y2 = 2 * (team * PPT + p);
-      Cplx<T> a = U[y2 * P + t], b = U[(y2 + 1) * P + t];
-      if ((ch == 0 && t == 0) || last) {  // kx = 0 and kx = nx: imaginary parts dropped
-        a.im = T(0);
-        b.im = T(0);
-      }
-      const Cplx<T> zk = {a.re - b.im, a.im + b.re};  // Z[k] = A + i B
-      const Cplx<T> zm = {a.re + b.im, b.re - a.im};  // Z[LX - k] = conj(A) + i conj(B)
-      if constexpr (last) {
-        if (t == 0) v[p][EX / 2] = zk;
-      } else {
-        Cplx<T> r;
-        r.re = __shfl(zm.re, mirror_lane, TX);
-        r.im = __shfl(zm.im, mirror_lane, TX);
-        v[p][ch] = zk;
-        // Z[LX - k'] of lane TX - t (k' = TX ch + TX - t) is element EX - 1 - ch of lane t;
-        // lane 0 keeps its own mirror, element EX - ch
-        if (t != 0) v[p][EX - 1 - ch] = r;
-        if constexpr (ch > 0) {
-          if (t == 0) v[p][EX - ch] = zm;
+    static_for<0, (last ? 1 : F::KC)>([&](auto k_) {
+      constexpr int kk = decltype(k_)::value;
+      constexpr int e = last ? EX / 2 : ch * F::KC + kk;  // the element of kx = TX e + t
+      static_for<0, PPT>([&](auto p_) {
+        constexpr int p = decltype(p_)::value;
+        const int y2 = 2 * (team * PPT + p);
+        Cplx<T> a = U[y2 * P + kk * TX + t], b = U[(y2 + 1) * P + kk * TX + t];
+        if ((e == 0 && t == 0) || last) {  // kx = 0 and kx = nx: imaginary parts dropped
+          a.im = T(0);
+          b.im = T(0);
         }
-      }
+        const Cplx<T> zk = {a.re - b.im, a.im + b.re};  // Z[k] = A + i B
+        const Cplx<T> zm = {a.re + b.im, b.re - a.im};  // Z[LX - k] = conj(A) + i conj(B)
+        if constexpr (last) {
+          if (t == 0) v[p][EX / 2] = zk;
+        } else {
+          Cplx<T> r;
+          r.re = __shfl(zm.re, mirror_lane, TX);
+          r.im = __shfl(zm.im, mirror_lane, TX);
+          v[p][e] = zk;
+          // Z[LX - k'] of lane TX - t (k' = TX e + TX - t) is element EX - 1 - e of lane t;
+          // lane 0 keeps its own mirror, element EX - e
+          if (t != 0) v[p][EX - 1 - e] = r;
+          if constexpr (e > 0) {
+            if (t == 0) v[p][EX - e] = zm;
+          }
+        }
+      });
     });
     __syncthreads();
   });
