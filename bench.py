@@ -650,7 +650,7 @@ def main():
     names = [lib.sopht_ns3d_stage_name(flow._plan, i).decode() for i in range(nst)]
     avg_ms = {names[i]: ms[i] / max(cnt.value, 1) for i in range(nst)}
     passes, zjobs = L.schedule("sopht_ns3d_poisson_schedule", flow._plan)
-    s1_vec, s1_chunk = L.schedule("sopht_ns3d_s1_schedule", flow._plan)
+    s1_vec, s1_chunk, s1_cols = L.schedule("sopht_ns3d_s1_schedule", flow._plan)
     inter.check_support_in_domain()
     if not torch.isfinite(flow.vorticity_field).all():
         print("non-finite vorticity after the timed steps", file=sys.stderr)
@@ -735,7 +735,7 @@ def main():
         "stage_ms": {k: round(v, 4) for k, v in avg_ms.items()},
         "stage_gbs": kernels_gbs,
         "poisson_schedule": {"passes": passes, "zpass_jobs": zjobs},
-        "s1_sweep": {"vector": s1_vec, "zchunk": s1_chunk},
+        "s1_sweep": {"vector": s1_vec, "zchunk": s1_chunk, "cols": s1_cols},
         "roofline": roof,
         "batched": batched,
         "adjoint": adjoint,

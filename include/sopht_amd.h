@@ -371,9 +371,10 @@ int sopht_ns3d_stage_times(sopht_ns3d* plan, double* ms, int n, int64_t* steps, 
  * solver; no counterpart in the reference). */
 int sopht_ns3d_poisson_schedule(const sopht_ns3d* plan, int* passes, int* zpass_jobs);
 /* How the step's first sweep (vorticity forcing, advection and stretching) runs: *vector = 1
- * for 16-byte row accesses, 0 for the scalar sweep; *zchunk = z planes per work item. Both
- * give the same bits (no counterpart in the reference). */
-int sopht_ns3d_s1_schedule(const sopht_ns3d* plan, int* vector, int* zchunk);
+ * for 16-byte row accesses, 0 for the scalar sweep; *zchunk = z planes per work item; *cols =
+ * cells per tile row (64, or 128 for FP32 rows a multiple of 128 cells). Every choice gives
+ * the same bits (no counterpart in the reference). Null outputs are skipped. */
+int sopht_ns3d_s1_schedule(const sopht_ns3d* plan, int* vector, int* zchunk, int* cols);
 const char* sopht_ns3d_stage_name(const sopht_ns3d* plan, int stage);
 
 /* ---- batched step (no counterpart in the reference: many independent simulators, the

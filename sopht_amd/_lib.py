@@ -116,7 +116,7 @@ SIGNATURES = {
     "sopht_ns3d_stage_times": [_vp, _pd, _i, _pi64, _i],
     "sopht_ns3d_stage_name": [_vp, _i],
     "sopht_ns3d_poisson_schedule": [_vp, _pi, _pi],
-    "sopht_ns3d_s1_schedule": [_vp, _pi, _pi],
+    "sopht_ns3d_s1_schedule": [_vp, _pi, _pi, _pi],
     "sopht_ns3d_batch_class_schedule": [_vp, _i, _pi64, _pi, _pi, _pi],
     "sopht_ns3d_batch_create": [_i, _i, C.POINTER(BatchMember), _vp, C.POINTER(_vp)],
     "sopht_ns3d_batch_step": [_vp, _i, _i, _vp],
@@ -243,10 +243,10 @@ def written(*tensors) -> None:
         torch.autograd.graph.increment_version(live)
 
 
-def schedule(fn: str, handle) -> tuple[int, int]:
-    """Two integers a plan reports: (passes, zpass_jobs) of a Poisson plan
+def schedule(fn: str, handle) -> tuple[int, ...]:
+    """The integers a plan reports: (passes, zpass_jobs) of a Poisson plan
     (``sopht_poisson3d_schedule``) or of a step plan (``sopht_ns3d_poisson_schedule``), or
-    (vector, zchunk) of a step plan's first sweep (``sopht_ns3d_s1_schedule``)."""
-    passes, jobs = C.c_int(0), C.c_int(0)
-    check(getattr(lib(), fn)(handle, C.byref(passes), C.byref(jobs)))
-    return passes.value, jobs.value
+    (vector, zchunk, cols) of a step plan's first sweep (``sopht_ns3d_s1_schedule``)."""
+    outs = [C.c_int(0) for _ in range(len(SIGNATURES[fn]) - 1)]
+    check(getattr(lib(), fn)(handle, *(C.byref(o) for o in outs)))
+    return tuple(o.value for o in outs)

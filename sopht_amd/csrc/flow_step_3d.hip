@@ -68,19 +68,28 @@ int step_impl(sopht_ns3d* P, const void* vort_in, void* vort_out, const void* ve
   tm.begin(s);
   // S1 with 16-byte accesses when the rows allow them, else the scalar sweep
   const bool s1v = P->s1_vec != 0;
-  const int s1_rows = s1v ? S1VTile<T>::TY : S1Tile<T>::TY;
+  const int s1_rows = s1v ? P->s1_rows : S1Tile<T>::TY;
+  const int s1_cols = s1v ? P->s1_cols : S1Tile<T>::TX;
   const int s1_chunk = s1v ? P->s1_chunk : kS1Chunk;
-  const S1Map s1m{int(ceil_div(g.nx, 64)), int(ceil_div(g.ny, s1_rows)), s1_rows, s1_chunk};
+  const S1Map s1m{int(ceil_div(g.nx, s1_cols)), int(ceil_div(g.ny, s1_rows)), s1_rows, s1_chunk, s1_cols};
   const int64_t s1_items = int64_t(s1m.tiles_x) * s1m.tiles_y * ceil_div(g.nz, s1_chunk);
   const bool flags_ok = s1_items <= P->s1flags_cap;
   int* const fflags = flags_ok ? P->s1flags : nullptr;
   {
     const int remap = s1_items % 8 == 0 ? 1 : 0;
-    if (s1v)
+    if constexpr (sizeof(T) == 4) {
+      if (s1v && s1_cols == 128) {
+        using W = S1VTile<T, 128, 8>;
+        k_s1v<T, W::TX, W::TY><<<unsigned(s1_items), W::NT, 0, s>>>(g, w_in, u_in, F, ps, pF, pB, wf, s1_chunk,
+                                                                    s1m.tiles_x, s1m.tiles_y, remap,
+                                                                    static_cast<T*>(vort1_out), fflags);
+      }
+    }
+    if (s1v && s1_cols == 64)
       k_s1v<T><<<unsigned(s1_items), S1VTile<T>::NT, 0, s>>>(g, w_in, u_in, F, ps, pF, pB, wf, s1_chunk,
                                                             s1m.tiles_x, s1m.tiles_y, remap,
                                                             static_cast<T*>(vort1_out), fflags);
-    else
+    else if (!s1v)
       k_s1<T><<<unsigned(s1_items), S1Tile<T>::NT, 0, s>>>(g, w_in, u_in, F, ps, pF, pB, wf, kS1Chunk,
                                                           s1m.tiles_x, s1m.tiles_y, remap,
                                                           static_cast<T*>(vort1_out), fflags);
@@ -240,11 +249,18 @@ int sopht_ns3d_create_with_poisson_solver(int dtype, int64_t nz, int64_t ny, int
     // doubled its L2 misses; 64 left one workgroup per CU); else 16 planes, and grids with
     // fewer than 256 such items keep the scalar sweep (twice the workgroups). The environment
     // can force either: SOPHT_S1_SCALAR=1 (the two sweeps give the same bits) and
-    // SOPHT_S1_CHUNK=16/32/64.
+    // SOPHT_S1_CHUNK=16/32/64. FP32 rows that are a multiple of 128 cells take tiles of 128
+    // cells by 8 rows, others 64 by 16 (FP64: 64 by 8); SOPHT_S1_COLS=64/128 forces the width.
     const int64_t vec = 16 / (dtype == SOPHT_F32 ? 4 : 8);
     const int64_t bytes = nz * ny * nx * (dtype == SOPHT_F32 ? 4 : 8);
-    const int64_t rows = dtype == SOPHT_F32 ? S1VTile<float>::TY : S1VTile<double>::TY;
-    const int64_t tiles = ceil_div(nx, 64) * ceil_div(ny, rows);
+    P->s1_cols = dtype == SOPHT_F32 && nx % 128 == 0 ? 128 : 64;
+    if (const char* cols = std::getenv("SOPHT_S1_COLS")) {
+      const int v = std::atoi(cols);
+      if (dtype == SOPHT_F32 && (v == 64 || v == 128)) P->s1_cols = v;
+    }
+    P->s1_rows = P->s1_cols == 128 ? S1VTile<float, 128, 8>::TY
+                                   : (dtype == SOPHT_F32 ? S1VTile<float>::TY : S1VTile<double>::TY);
+    const int64_t tiles = ceil_div(nx, P->s1_cols) * ceil_div(ny, P->s1_rows);
     const char* scalar = std::getenv("SOPHT_S1_SCALAR");
     const char* chunk = std::getenv("SOPHT_S1_CHUNK");
     P->s1_chunk = tiles * ceil_div(nz, 32) >= 512 ? 32 : 16;
@@ -308,10 +324,11 @@ int sopht_ns3d_poisson_schedule(const sopht_ns3d* P, int* passes, int* zpass_job
   return sopht_poisson3d_schedule(P->poisson, passes, zpass_jobs);
 }
 
-int sopht_ns3d_s1_schedule(const sopht_ns3d* P, int* vector, int* zchunk) {
+int sopht_ns3d_s1_schedule(const sopht_ns3d* P, int* vector, int* zchunk, int* cols) {
   SOPHT_REQUIRE(P != nullptr, "plan is null");
   if (vector) *vector = P->s1_vec;
   if (zchunk) *zchunk = P->s1_vec ? P->s1_chunk : sopht::kS1Chunk;
+  if (cols) *cols = P->s1_vec ? P->s1_cols : sopht::S1Tile<float>::TX;
   return SOPHT_OK;
 }
 

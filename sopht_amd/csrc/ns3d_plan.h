@@ -25,6 +25,8 @@ struct sopht_ns3d {
   int filter_type;
   int s1_vec;          // S1 with 16-byte accesses (rows a multiple of 16 bytes), else the scalar sweep
   int s1_chunk;        // z planes per work item of the vector S1 sweep (16, 32 or 64)
+  int s1_cols;         // its tile: s1_cols cells (64, or 128 for FP32) by s1_rows rows
+  int s1_rows;
   sopht::StageTimer timer;
 };
 

@@ -332,7 +332,9 @@ __device__ __forceinline__ void x_nbrs(const VecT<T, VEC>& v, const T* __restric
 // issues one load instruction per array, plane and cell. Here a thread holds VEC neighbouring
 // cells of one row, so each array and plane costs one 16-byte load per VEC cells (3.4x fewer
 // load instructions; the kernel is then bound by HBM traffic, 1.5x its algorithmic bytes at
-// 256^3). The tile is 64 cells by TY rows and marches a chunk of z planes as s1_body does.
+// 256^3). The tile is TX = 64 or 128 cells by TY rows and marches a chunk of z planes as
+// s1_body does. 128-cell rows of 8 (FP32) fetch fewer x edge lines per cell than 64-cell rows
+// of 16: 0.210 against 0.225 ms at 256^3 FP32.
 // Per plane i:
 //   - the tile threads form omega1 and B on their cells; the x neighbours of F come from the
 //     adjacent lanes of the row, its y neighbours are row loads, and F0 and F1 ride along z in
@@ -347,21 +349,22 @@ __device__ __forceinline__ void x_nbrs(const VecT<T, VEC>& v, const T* __restric
 // F0/F2 rows also going through LDS (for their y neighbours) cut the sweep's HBM fetch from
 // 500 to 435 MB at 256^3 FP32 but not its time (0.229 against 0.224 ms), and were not kept.
 // Every expression keeps the operand order of s1_body, so both sweeps give the same bits.
-template <typename T>
+template <typename T, int TXW = 64, int TYW = sizeof(T) == 4 ? 16 : 8>
 struct S1VTile {
   static constexpr int VEC = 16 / int(sizeof(T));
-  static constexpr int TX = 64, MXT = TX / VEC;            // cells and lanes per row
-  static constexpr int TY = sizeof(T) == 4 ? 16 : 8, NT = MXT * TY;
+  static constexpr int TX = TXW, MXT = TX / VEC;           // cells and lanes per row
+  static constexpr int TY = TYW, NT = MXT * TY;
   static constexpr int RW = TY + 2, ROWS = RW * TX;        // a component's row box (ring rows included)
   static constexpr int B1OFF = 2 * ROWS, EOFF = B1OFF + TY * TX;
   static constexpr int PL = EOFF + 4 * TY;  // B0 and B2 row boxes, B1 tile rows, B1 and B2 edge columns
-  static_assert(NT >= 64 + 2 * TY && 2 * MXT <= 64 && MXT == MarchGeom<VEC>::MXT,
+  static_assert(NT >= 64 + 2 * TY && 2 * MXT <= 64 && TX % MX == 0,
                 "ring rows in wavefront 0, edge columns in wavefront 1");
 };
 
 // Where a flow step's S1 tiles lie, for the F-zeroing flags it leaves per tile (SC reads them).
 struct S1Map {
-  int tiles_x, tiles_y, rows, chunk;  // 64-cell-wide tiles of `rows` rows and `chunk` planes
+  int tiles_x, tiles_y, rows, chunk;  // tiles of `cols` cells by `rows` rows and `chunk` planes
+  int cols = 64;
 };
 
 template <typename T>
@@ -371,13 +374,13 @@ __device__ __forceinline__ bool any_bits(const VecT<T, 16 / int(sizeof(T))>& v) 
   return r;
 }
 
-template <typename T>
+template <typename T, int TXW = 64, int TYW = sizeof(T) == 4 ? 16 : 8>
 __device__ __forceinline__ void s1v_body(Grid g, const T* __restrict__ w, const T* __restrict__ u,
                                          const T* __restrict__ F, T* __restrict__ w2out, T pF, T pB,
                                          int with_forcing, int zchunk, int tiles_x, int tiles_y,
                                          int64_t item, T* __restrict__ w1out = nullptr,
                                          int* fflags = nullptr) {
-  using S = S1VTile<T>;
+  using S = S1VTile<T, TXW, TYW>;
   constexpr int VEC = S::VEC, TX = S::TX, MXT = S::MXT, TY = S::TY, ROWS = S::ROWS;
   constexpr int B1OFF = S::B1OFF, EOFF = S::EOFF;
   using V = VecT<T, VEC>;
@@ -629,11 +632,11 @@ __device__ __forceinline__ void s1v_body(Grid g, const T* __restrict__ w, const 
   }
 }
 
-template <typename T>
-__global__ void __launch_bounds__(S1VTile<T>::NT) __attribute__((amdgpu_waves_per_eu(2)))
+template <typename T, int TXW = 64, int TYW = sizeof(T) == 4 ? 16 : 8>
+__global__ void __launch_bounds__((S1VTile<T, TXW, TYW>::NT)) __attribute__((amdgpu_waves_per_eu(2)))
 k_s1v(Grid g, const T* w, const T* u, const T* F, T* w2out, T pF, T pB, int with_forcing,
       int zchunk, int tiles_x, int tiles_y, int remap, T* w1out, int* fflags) {
-  s1v_body<T>(g, w, u, F, w2out, pF, pB, with_forcing, zchunk, tiles_x, tiles_y, xcd_item(remap),
+  s1v_body<T, TXW, TYW>(g, w, u, F, w2out, pF, pB, with_forcing, zchunk, tiles_x, tiles_y, xcd_item(remap),
               w1out, fflags);
 }
 
@@ -653,7 +656,7 @@ __device__ __forceinline__ void sc_body(Grid g, const T* __restrict__ w2, T* __r
   const MarchTile m = march_tile<VEC>(g, tiles_x, tiles_y, item, chunk);
   if (zero_forcing && fflags && m.x0 < int(g.nx) && m.y < int(g.ny)) {
     // the S1 tile holding these cells saw no bit of F set: F is zero there already
-    const int t = ((m.z0 / s1.chunk) * s1.tiles_y + m.y / s1.rows) * s1.tiles_x + m.x0 / 64;
+    const int t = ((m.z0 / s1.chunk) * s1.tiles_y + m.y / s1.rows) * s1.tiles_x + m.x0 / s1.cols;
     if (fflags[t] == 0) zero_forcing = 0;
   }
   const int nz = int(g.nz), ny = int(g.ny), nx = int(g.nx);

@@ -83,12 +83,14 @@ def test_one_step_non_cubic_matches_oracle(dev, shape, precision, rng):
 @pytest.mark.parametrize("shape", [(24, 24, 48), (12, 48, 96), (64, 12, 128), (48, 16, 8), (4, 4, 4)])
 @pytest.mark.parametrize("precision", ["single", "double"])
 @pytest.mark.parametrize("chunk", [16, 32])
-def test_vector_s1_sweep_bit_identical_to_scalar(dev, shape, precision, chunk, rng, monkeypatch):
+@pytest.mark.parametrize("cols", [64, 128])
+def test_vector_s1_sweep_bit_identical_to_scalar(dev, shape, precision, chunk, cols, rng, monkeypatch):
     """The S1 sweep with 16-byte accesses (rows a multiple of 16 bytes; forced here with
     ``SOPHT_S1_CHUNK``, as these grids are below the size that selects it) against the scalar
     sweep (``SOPHT_S1_SCALAR=1``): the same bits in every field after one step, with partial
     x, y and z tiles, forcing nonzero only in a sub-box (the F-zeroing flags per tile, read by
-    SC's tiles of 16 planes), and the out-of-place step's omega1 output."""
+    SC's tiles of 16 planes), and the out-of-place step's omega1 output. FP32 runs both tile
+    widths (``SOPHT_S1_COLS``: 64 cells by 16 rows, 128 by 8); FP64 has the 64-cell tile."""
     import torch
 
     real_t = PREC[precision]
@@ -100,13 +102,16 @@ def test_vector_s1_sweep_bit_identical_to_scalar(dev, shape, precision, chunk, r
             monkeypatch.setenv("SOPHT_S1_SCALAR", "1")
         else:
             monkeypatch.setenv("SOPHT_S1_CHUNK", str(chunk))
+            monkeypatch.setenv("SOPHT_S1_COLS", str(cols))
         sims.append(sps().UnboundedNavierStokesFlowSimulator3D(grid_size=shape, x_range=1.0,
                                                                real_t=real_t, **kw))
     monkeypatch.delenv("SOPHT_S1_SCALAR")
+    monkeypatch.delenv("SOPHT_S1_COLS")
     from sopht_amd import _lib as L
 
-    assert L.schedule("sopht_ns3d_s1_schedule", sims[0]._plan) == (1, chunk)
-    assert L.schedule("sopht_ns3d_s1_schedule", sims[1]._plan) == (0, 16)
+    want_cols = cols if precision == "single" else 64
+    assert L.schedule("sopht_ns3d_s1_schedule", sims[0]._plan) == (1, chunk, want_cols)
+    assert L.schedule("sopht_ns3d_s1_schedule", sims[1]._plan) == (0, 16, 64)
     w0, u0 = (rng.standard_normal((3,) + shape).astype(real_t) for _ in range(2))
     F0 = np.zeros((3,) + shape, dtype=real_t)
     nz, ny, nx = shape
