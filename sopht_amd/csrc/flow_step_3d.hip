@@ -244,12 +244,14 @@ int sopht_ns3d_create_with_poisson_solver(int dtype, int64_t nz, int64_t ny, int
   P->filter_type = 0;
   {
     // The vector sweep needs whole 16-byte row vectors and component byte offsets below 2^32.
-    // Its work items hold 32 planes while that still makes two per CU: at 256^3 FP32 the
-    // resident tiles then cover one z chunk per XCD (16 planes put two chunks on each XCD and
-    // doubled its L2 misses; 64 left one workgroup per CU); else 16 planes, and grids with
-    // fewer than 256 such items keep the scalar sweep (twice the workgroups). The environment
-    // can force either: SOPHT_S1_SCALAR=1 (the two sweeps give the same bits) and
-    // SOPHT_S1_CHUNK=16/32/64. FP32 rows that are a multiple of 128 cells take tiles of 128
+    // Its work items hold the most planes (64, 32 or 16) that still make 384 items, one round
+    // of workgroups at one and a half to three per CU. Measured S1 times: 256^3 FP32 (128 x 8
+    // tiles) 0.21 ms at 32 planes against 0.24 at 16 (two chunks per XCD resident at once
+    // doubled the L2 misses) and 0.29 at 64; 256^3 FP64 0.38 ms at 64 against 0.43 at 32;
+    // 192^3 FP64 0.19 ms at 32 against 0.22 at 16; 192^3 FP32 0.11 ms at 16 against 0.16 at
+    // 32. Grids with fewer than 256 items of 16 planes keep the scalar sweep (twice the
+    // workgroups). The environment can force either: SOPHT_S1_SCALAR=1 (the two sweeps give
+    // the same bits) and SOPHT_S1_CHUNK=16/32/64. FP32 rows that are a multiple of 128 cells take tiles of 128
     // cells by 8 rows, others 64 by 16 (FP64: 64 by 8); SOPHT_S1_COLS=64/128 forces the width.
     const int64_t vec = 16 / (dtype == SOPHT_F32 ? 4 : 8);
     const int64_t bytes = nz * ny * nx * (dtype == SOPHT_F32 ? 4 : 8);
@@ -263,7 +265,12 @@ int sopht_ns3d_create_with_poisson_solver(int dtype, int64_t nz, int64_t ny, int
     const int64_t tiles = ceil_div(nx, P->s1_cols) * ceil_div(ny, P->s1_rows);
     const char* scalar = std::getenv("SOPHT_S1_SCALAR");
     const char* chunk = std::getenv("SOPHT_S1_CHUNK");
-    P->s1_chunk = tiles * ceil_div(nz, 32) >= 512 ? 32 : 16;
+    P->s1_chunk = 16;
+    for (const int c : {64, 32})
+      if (tiles * ceil_div(nz, int64_t(c)) >= 384) {
+        P->s1_chunk = c;
+        break;
+      }
     bool fill = tiles * ceil_div(nz, 16) >= 256;
     if (chunk) {
       const int v = std::atoi(chunk);
