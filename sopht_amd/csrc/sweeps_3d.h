@@ -331,16 +331,16 @@ __device__ __forceinline__ void x_nbrs(const VecT<T, VEC>& v, const T* __restric
 // S1 with 16-byte accesses (rows whose length is a multiple of VEC). The scalar sweep above
 // issues one load instruction per array, plane and cell. Here a thread holds VEC neighbouring
 // cells of one row, so each array and plane costs one 16-byte load per VEC cells (3.4x fewer
-// load instructions; the kernel is then bound by HBM traffic, 1.5x its algorithmic bytes at
-// 256^3). The tile is TX = 64 or 128 cells by TY rows and marches a chunk of z planes as
-// s1_body does. 128-cell rows of 8 (FP32) fetch fewer x edge lines per cell than 64-cell rows
-// of 16: 0.210 against 0.225 ms at 256^3 FP32.
+// load instructions; the kernel is then bound by HBM traffic). The tile is TX = 64 or 128
+// cells by TY rows and marches a chunk of z planes as s1_body does. 128-cell rows of 8 (FP32)
+// fetch fewer x edge lines per cell than 64-cell rows of 16: at 256^3 FP32 0.210 against
+// 0.225 ms, 1.19x against 1.54x its algorithmic bytes.
 // Per plane i:
 //   - the tile threads form omega1 and B on their cells; the x neighbours of F come from the
 //     adjacent lanes of the row, its y neighbours are row loads, and F0 and F1 ride along z in
 //     registers;
 //   - wavefront 0 forms B on the y ring rows (y0 - 1 and y0 + TY);
-//   - wavefront 1 forms B on the x edge columns (x0 - 1 and x0 + 64).
+//   - wavefront 1 forms B on the x edge columns (x0 - 1 and x0 + TX).
 // LDS keeps three planes of B: the B0 and B2 rows of the tile and ring (y derivatives), the B1
 // rows of the tile, and the B1 and B2 edge columns (x derivatives at the tile's first and last
 // lanes). After the one barrier per plane, plane i - 1 is consumed: its x derivatives from the
