@@ -155,6 +155,18 @@ int sopht_advection_flux_conservative_eno3_3d(int dtype, void* flux, const int64
                                               int64_t ny, int64_t nx, double inv_dx,
                                               void* stream);
 
+/* Adjoint of gen_advection_flux_conservative_eno3_pyst_kernel_3d
+ * (.../advection_flux_3d.py:12-233; the reference has no adjoint): the transpose of the flux
+ * with every face's upwind stencil choice frozen at `velocity`. For a flux adjoint read on
+ * [2:-2]^3, accumulates field_adj += J_field^T flux_adj and velocity_adj[c] += J_u_c^T flux_adj
+ * over the whole grid. Either output may be NULL. Deterministic (gather, no atomics). */
+int sopht_advection_flux_conservative_eno3_adjoint_3d(
+    int dtype, void* field_adj, const int64_t* field_adj_strides, void* const velocity_adj[3],
+    const int64_t* velocity_adj_strides, const void* flux_adj, const int64_t* flux_adj_strides,
+    const void* field, const int64_t* field_strides, const void* const velocity[3],
+    const int64_t* velocity_strides, int64_t nz, int64_t ny, int64_t nx, double inv_dx,
+    void* stream);
+
 /* gen_vorticity_stretching_flux_pyst_kernel_3d — .../vorticity_stretching_flux_3d.py:13-107. */
 int sopht_vorticity_stretching_flux_3d(int dtype, void* const flux[3], const int64_t* flux_strides,
                                        const void* const vorticity[3],

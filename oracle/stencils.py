@@ -278,6 +278,47 @@ def advection_flux_conservative_eno3(advection_flux, field, velocity, inv_dx):
         advection_flux[J] = advection_flux[J] - inv_dx * np.where(up, a, b)
 
 
+def advection_flux_conservative_eno3_adjoint(field_adj, velocity_adj, flux_adj, field, velocity, inv_dx):
+    """Transpose of ``advection_flux_conservative_eno3`` (``advection_flux_3d.py:12-233``) with
+    each face's stencil choice frozen at the forward's ``velocity``. The reference has no
+    adjoint; this restates the forward's transpose in scatter form.
+
+    The forward adds ``inv_dx * sum_o W(c, o) * g[c + o]`` to the flux at ``c`` in
+    ``[2:-2]^3``, with ``g = field * u_axis`` and the face weights ``W`` picked by the
+    upwind switches (which read only ``velocity``). So for a flux adjoint ``a``,
+    ``gbar[c + o] += inv_dx * W(c, o) * a[c]``, then ``field_adj += gbar * u_axis`` and
+    ``velocity_adj[axis] += gbar * field``. Either output may be None."""
+    n = field.shape
+    J = (slice(2, n[0] - 2), slice(2, n[1] - 2), slice(2, n[2] - 2))
+
+    def sh(a, axis, off):
+        sl = list(J)
+        s = sl[axis]
+        sl[axis] = slice(s.start + off, s.stop + off)
+        return a[tuple(sl)]
+
+    a = flux_adj[J]
+    for axis, comp in ((2, 0), (1, 1), (0, 2)):
+        u = velocity[comp]
+        gbar = np.zeros(n, dtype=np.result_type(field, flux_adj))
+        upf = sh(u, axis, 0) > -sh(u, axis, 1)
+        upb = sh(u, axis, 0) > -sh(u, axis, -1)
+        # (offset, weight when the switch holds, weight when it does not); back faces negated
+        terms = (
+            (upf, ((1, 1 / 3), (0, 5 / 6), (-1, -1 / 6)), ((0, 1 / 3), (1, 5 / 6), (2, -1 / 6)), 1.0),
+            (upb, ((0, 1 / 3), (-1, 5 / 6), (-2, -1 / 6)), ((-1, 1 / 3), (0, 5 / 6), (1, -1 / 6)), -1.0),
+        )
+        for mask, taken, other, sign in terms:
+            for off, w in taken:
+                sh(gbar, axis, off)[...] += np.where(mask, sign * w * inv_dx * a, 0)
+            for off, w in other:
+                sh(gbar, axis, off)[...] += np.where(mask, 0, sign * w * inv_dx * a)
+        if field_adj is not None:
+            field_adj += gbar * u
+        if velocity_adj is not None:
+            velocity_adj[comp] += gbar * field
+
+
 def advection_timestep_euler_forward_conservative_eno3(field, advection_flux, velocity, dt_by_dx):
     """``advection_timestep_3d.py:12-99``: per scalar (or per component, x, y, z): zero the
     flux, accumulate the ENO3 flux with ``inv_dx = -dt_by_dx``, then ``field += flux``."""

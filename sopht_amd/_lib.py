@@ -75,6 +75,8 @@ SIGNATURES = {
     "sopht_elementwise_complex_product_3d": [_i, _vp, _pi64, _vp, _pi64, _vp, _pi64, _i64, _i64, _i64, _vp],
     "sopht_advection_flux_conservative_eno3_3d": [
         _i, _vp, _pi64, _vp, _pi64, _p3, _pi64, _i64, _i64, _i64, _d, _vp],
+    "sopht_advection_flux_conservative_eno3_adjoint_3d": [
+        _i, _vp, _pi64, _vp, _pi64, _vp, _pi64, _vp, _pi64, _p3, _pi64, _i64, _i64, _i64, _d, _vp],
     "sopht_passive_transport_step_3d": [_i, _i, _p3, _p3, _p3, _i64, _i64, _i64, _d, _d, _vp],
     "sopht_velocity_abs_sum_max_3d": [_i, _p3, _i64, _vp, _vp, _i64, _vp, _vp],
     "sopht_vorticity_stretching_flux_3d": [_i, _p3, _pi64, _p3, _pi64, _p3, _pi64, _i64, _i64, _i64, _d, _vp],

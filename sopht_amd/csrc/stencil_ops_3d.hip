@@ -341,6 +341,82 @@ int eno3_impl(void* flux, const int64_t* fs, const void* field, const int64_t* i
       s);
 }
 
+// Weight of g[c + o] in the ENO3 flux at c along one axis (front face minus back face), for
+// the face switches up_front = u[c] > -u[c+1] and up_back = u[c] > -u[c-1] of eno3_impl.
+template <typename T>
+__device__ __forceinline__ T eno3_weight(bool up_front, bool up_back, int o) {
+  const T c13 = T(1.0 / 3.0), c56 = T(5.0 / 6.0), c16 = T(1.0 / 6.0);
+  T w = T(0);
+  if (up_front) {
+    w = o == 1 ? c13 : (o == 0 ? c56 : (o == -1 ? -c16 : T(0)));
+  } else {
+    w = o == 0 ? c13 : (o == 1 ? c56 : (o == 2 ? -c16 : T(0)));
+  }
+  if (up_back) {
+    w -= o == 0 ? c13 : (o == -1 ? c56 : (o == -2 ? -c16 : T(0)));
+  } else {
+    w -= o == -1 ? c13 : (o == 0 ? c56 : (o == 1 ? -c16 : T(0)));
+  }
+  return w;
+}
+
+// Transpose of eno3_impl with the switches frozen at `vel`: for a flux adjoint a (read on
+// [2:-2]^3), gbar_axis(m) = inv_dx * sum_o W(m - o, o) a(m - o) over m - o in [2:-2]^3, then
+// field_adj += sum_axis gbar_axis * u_axis and vel_adj[axis] += gbar_axis * field. One thread
+// gathers for its own cell (no atomics, the same bits on every run). Either output may be
+// absent (has_f / has_v false).
+template <typename T>
+int eno3_adjoint_impl(void* field_adj, const int64_t* fas, void* const vel_adj[3],
+                      const int64_t* vas, const void* flux_adj, const int64_t* as,
+                      const void* field, const int64_t* is, const void* const vel[3],
+                      const int64_t* vs, int64_t nz, int64_t ny, int64_t nx, T inv_dx,
+                      hipStream_t s) {
+  if (nz <= 4 || ny <= 4 || nx <= 4) return SOPHT_OK;
+  const bool has_f = field_adj != nullptr, has_v = vel_adj != nullptr;
+  if (!has_f && !has_v) return SOPHT_OK;
+  const int64_t zero3[3] = {0, 0, 0};
+  View3<T> fa = make_view<T>(field_adj, has_f ? fas : zero3);
+  View3<T> va0 = make_view<T>(has_v ? vel_adj[0] : nullptr, has_v ? vas : zero3);
+  View3<T> va1 = make_view<T>(has_v ? vel_adj[1] : nullptr, has_v ? vas : zero3);
+  View3<T> va2 = make_view<T>(has_v ? vel_adj[2] : nullptr, has_v ? vas : zero3);
+  View3<T> a = make_view<T>(flux_adj, as), f = make_view<T>(field, is);
+  View3<T> u0 = make_view<T>(vel[0], vs), u1 = make_view<T>(vel[1], vs), u2 = make_view<T>(vel[2], vs);
+  return launch_box(
+      0, 0, 0, nz, ny, nx,
+      [=] __device__(int64_t z, int64_t y, int64_t x) {
+        const int64_t m[3] = {x, y, z}, n[3] = {nx, ny, nz};
+        const bool in_j[3] = {x >= 2 && x < nx - 2, y >= 2 && y < ny - 2, z >= 2 && z < nz - 2};
+        T gbar[3] = {T(0), T(0), T(0)};
+        for (int ax = 0; ax < 3; ++ax) {
+          // the flux cells that read this one along ax share its other two coordinates
+          if (!in_j[(ax + 1) % 3] || !in_j[(ax + 2) % 3]) continue;
+          const View3<T>& u = ax == 0 ? u0 : (ax == 1 ? u1 : u2);
+          auto U = [&](int64_t k) -> T {
+            return ax == 0 ? u(z, y, x + k) : (ax == 1 ? u(z, y + k, x) : u(z + k, y, x));
+          };
+          auto A = [&](int64_t k) -> T {
+            return ax == 0 ? a(z, y, x + k) : (ax == 1 ? a(z, y + k, x) : a(z + k, y, x));
+          };
+          T g = T(0);
+          for (int o = -2; o <= 2; ++o) {
+            const int64_t c = m[ax] - o;
+            if (c < 2 || c >= n[ax] - 2) continue;
+            const T uc = U(-o);
+            g += eno3_weight<T>(uc > -U(1 - o), uc > -U(-1 - o), o) * A(-o);
+          }
+          gbar[ax] = inv_dx * g;
+        }
+        if (has_f) fa(z, y, x) += gbar[0] * u0(z, y, x) + gbar[1] * u1(z, y, x) + gbar[2] * u2(z, y, x);
+        if (has_v) {
+          const T fv = f(z, y, x);
+          va0(z, y, x) += gbar[0] * fv;
+          va1(z, y, x) += gbar[1] * fv;
+          va2(z, y, x) += gbar[2] * fv;
+        }
+      },
+      s);
+}
+
 template <typename T>
 int stretching_impl(void* const flux[3], const int64_t* fs, const void* const w[3],
                     const int64_t* ws, const void* const u[3], const int64_t* us, int64_t nz,
@@ -649,6 +725,20 @@ int sopht_advection_flux_conservative_eno3_3d(int dtype, void* flux, const int64
   REQUIRE_GRID(nz, ny, nx);
   DISPATCH(dtype, eno3_impl<T>(flux, fs, field, is, vel, vs, nz, ny, nx, T(inv_dx),
                                as_stream(stream)));
+}
+
+int sopht_advection_flux_conservative_eno3_adjoint_3d(
+    int dtype, void* field_adj, const int64_t* fas, void* const velocity_adj[3],
+    const int64_t* vas, const void* flux_adj, const int64_t* as, const void* field,
+    const int64_t* is, const void* const vel[3], const int64_t* vs, int64_t nz, int64_t ny,
+    int64_t nx, double inv_dx, void* stream) {
+  REQUIRE_GRID(nz, ny, nx);
+  SOPHT_REQUIRE(field_adj == nullptr || fas != nullptr, "field_adj needs its strides");
+  SOPHT_REQUIRE(velocity_adj == nullptr || (vas != nullptr && velocity_adj[0] != nullptr &&
+                                            velocity_adj[1] != nullptr && velocity_adj[2] != nullptr),
+                "velocity_adj needs three components and their strides");
+  DISPATCH(dtype, eno3_adjoint_impl<T>(field_adj, fas, velocity_adj, vas, flux_adj, as, field, is,
+                                       vel, vs, nz, ny, nx, T(inv_dx), as_stream(stream)));
 }
 
 int sopht_vorticity_stretching_flux_3d(int dtype, void* const flux[3], const int64_t* fs,

@@ -481,6 +481,68 @@ def gen_advection_flux_conservative_eno3_pyst_kernel_3d(
     return advection_flux_conservative_eno3_pyst_kernel_3d
 
 
+def gen_advection_flux_conservative_eno3_adjoint_kernel_3d(real_t: type) -> Callable:
+    """Transpose of the ENO3 flux above (``advection_flux_3d.py:12-233``; the reference has no
+    adjoint), with each face's upwind stencil choice frozen at ``velocity``.
+
+    ``kernel(field_adj, velocity_adj, flux_adj, field, velocity, inv_dx)`` accumulates
+    ``J_field^T flux_adj`` into ``field_adj`` and ``J_velocity^T flux_adj`` into
+    ``velocity_adj`` (either may be None). ``flux_adj`` is read on ``[2:-2]^3``, the cells
+    the forward writes."""
+    tag = L.dtype_tag(real_t)
+
+    def advection_flux_conservative_eno3_adjoint_kernel_3d(
+        field_adj, velocity_adj, flux_adj, field, velocity, inv_dx: float
+    ) -> None:
+        nz, ny, nx = _shape3(field)
+        L.call("sopht_advection_flux_conservative_eno3_adjoint_3d", tag, L.ptr(field_adj),
+               L.strides3(field_adj) if field_adj is not None else None,
+               L.comps(velocity_adj) if velocity_adj is not None else None,
+               L.strides3(velocity_adj[0]) if velocity_adj is not None else None,
+               L.ptr(flux_adj), L.strides3(flux_adj), L.ptr(field), L.strides3(field),
+               L.comps(velocity), L.strides3(velocity[0]), nz, ny, nx, float(inv_dx),
+               L.stream_ptr())
+        written = [t for t in (field_adj, velocity_adj) if t is not None]
+        L.written(*written)
+
+    return advection_flux_conservative_eno3_adjoint_kernel_3d
+
+
+def eno3_advection_flux(field, velocity, inv_dx: float):
+    """Differentiable ENO3 conservative advection flux (a ``torch.autograd.Function``).
+
+    Returns a new flux field: zero outside ``[2:-2]^3`` and the reference's ENO3 flux inside.
+    The backward runs the transpose kernel with the forward's stencil choices (they depend
+    on ``velocity`` only), so gradients flow to both ``field`` and ``velocity``; the switch
+    itself is piecewise constant and contributes nothing."""
+    import torch
+
+    real_t = np.float32 if field.dtype == torch.float32 else np.float64
+    forward_kernel = gen_advection_flux_conservative_eno3_pyst_kernel_3d(real_t)
+    adjoint_kernel = gen_advection_flux_conservative_eno3_adjoint_kernel_3d(real_t)
+
+    class _ENO3Flux(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, f, u):
+            f, u = f.contiguous(), u.contiguous()
+            flux = torch.zeros_like(f)
+            forward_kernel(flux, f, u, inv_dx)
+            ctx.save_for_backward(f, u)
+            return flux
+
+        @staticmethod
+        def backward(ctx, grad_flux):
+            f, u = ctx.saved_tensors
+            grad_flux = grad_flux.contiguous()
+            gf = torch.zeros_like(f) if ctx.needs_input_grad[0] else None
+            gu = torch.zeros_like(u) if ctx.needs_input_grad[1] else None
+            if gf is not None or gu is not None:
+                adjoint_kernel(gf, gu, grad_flux, f, u, inv_dx)
+            return gf, gu
+
+    return _ENO3Flux.apply(field, velocity)
+
+
 def gen_advection_timestep_euler_forward_conservative_eno3_pyst_kernel_3d(
     real_t: type,
     num_threads: bool | int = False,

@@ -192,6 +192,60 @@ def test_eno3_flux_and_timestep(dev, real_t, rng):
     np.testing.assert_allclose(to_host(vd), refv, atol=tol(real_t), rtol=0)
 
 
+@pytest.mark.parametrize("shape", SHAPES)
+def test_eno3_adjoint_matches_oracle(dev, real_t, shape, rng):
+    # accumulates onto nonzero adjoints; the switches compare the same values in both (the
+    # oracle runs in FP64 on the FP32 inputs), so only the rounding of the sums differs
+    f = rng.standard_normal(shape).astype(real_t)
+    u = rng.standard_normal((3, *shape)).astype(real_t)
+    a = rng.standard_normal(shape).astype(real_t)
+    fa0 = rng.standard_normal(shape).astype(real_t)
+    ua0 = rng.standard_normal((3, *shape)).astype(real_t)
+    inv_dx = 0.3
+    fa_ref, ua_ref = fa0.astype(np.float64), ua0.astype(np.float64)
+    S.advection_flux_conservative_eno3_adjoint(fa_ref, ua_ref, a.astype(np.float64), f.astype(np.float64),
+                                               u.astype(np.float64), inv_dx)
+    fa, ua = to_dev(fa0, dev), to_dev(ua0, dev)
+    k = spne().gen_advection_flux_conservative_eno3_adjoint_kernel_3d(real_t)
+    k(fa, ua, to_dev(a, dev), to_dev(f, dev), to_dev(u, dev), inv_dx)
+    atol = 1e-12 if real_t is np.float64 else 4 * tol(real_t)
+    np.testing.assert_allclose(to_host(fa), fa_ref, atol=atol, rtol=0)
+    np.testing.assert_allclose(to_host(ua), ua_ref, atol=atol, rtol=0)
+    # field adjoint only, into a strided view; the untouched columns stay as they were
+    big = to_dev(np.full((shape[0], shape[1], 2 * shape[2]), 5.0, dtype=real_t), dev)
+    k(big[..., ::2], None, to_dev(a, dev), to_dev(f, dev), to_dev(u, dev), inv_dx)
+    fa_ref2 = np.full(shape, 5.0)
+    S.advection_flux_conservative_eno3_adjoint(fa_ref2, None, a.astype(np.float64), f.astype(np.float64),
+                                               u.astype(np.float64), inv_dx)
+    np.testing.assert_allclose(to_host(big[..., ::2]), fa_ref2, atol=atol, rtol=0)
+    assert np.all(to_host(big[..., 1::2]) == 5.0)
+
+
+@pytest.mark.parametrize("n", [16, 96])
+def test_eno3_flux_autograd_dot_products(dev, n, rng):
+    # through torch.autograd: <F(f; u), a> = <f, df> = <u, du> (the flux is linear in the
+    # field and, with the switches frozen, in the velocity), and the gradients match the
+    # oracle's transpose at the small size
+    import torch
+
+    f = to_dev(rng.standard_normal((n, n, n)), dev).requires_grad_()
+    u = to_dev(rng.standard_normal((3, n, n, n)), dev).requires_grad_()
+    a = to_dev(rng.standard_normal((n, n, n)), dev)
+    flux = spne().eno3_advection_flux(f, u, 0.5)
+    lhs = (flux * a).sum()
+    lhs.backward()
+    lv = float(lhs.detach())
+    assert abs(lv) > 1e-3
+    np.testing.assert_allclose(float((f * f.grad).sum()), lv, rtol=1e-11)
+    np.testing.assert_allclose(float((u * u.grad).sum()), lv, rtol=1e-11)
+    if n == 16:
+        fr, ur = np.zeros((n, n, n)), np.zeros((3, n, n, n))
+        S.advection_flux_conservative_eno3_adjoint(fr, ur, to_host(a), to_host(f), to_host(u), 0.5)
+        np.testing.assert_allclose(to_host(f.grad), fr, atol=1e-12, rtol=0)
+        np.testing.assert_allclose(to_host(u.grad), ur, atol=1e-12, rtol=0)
+    assert flux.dtype == torch.float64
+
+
 def test_vorticity_stretching(dev, real_t, rng):
     n = 16
     w = rng.standard_normal((3, n, n, n)).astype(real_t)

@@ -166,6 +166,56 @@ def test_eno3_constant_velocity_linear_field():
     np.testing.assert_allclose(flux[2:-2, 2:-2, 2:-2], -6.0, atol=1e-12)
 
 
+def _eno3_masks(u):
+    n = u.shape[1:]
+    J = (slice(2, n[0] - 2), slice(2, n[1] - 2), slice(2, n[2] - 2))
+    out = []
+    for axis, comp in ((2, 0), (1, 1), (0, 2)):
+        for off in (1, -1):
+            sl = list(J)
+            sl[axis] = slice(J[axis].start + off, J[axis].stop + off)
+            out.append(u[comp][J] > -u[comp][tuple(sl)])
+    return np.stack(out)
+
+
+def test_eno3_adjoint_dot_products():
+    # the forward is linear in the field and, with the switches frozen, in the velocity, so
+    # <F(f; u), a> = <f, J_f^T a> = <u, J_u^T a>, and a small velocity change that flips no
+    # switch moves <F, a> by exactly <du, J_u^T a>
+    rng = np.random.default_rng(7)
+    shape = (8, 9, 10)
+    f = rng.standard_normal(shape)
+    u = rng.standard_normal((3, *shape))
+    a = rng.standard_normal(shape)
+    inv_dx = 0.7
+    flux = np.zeros(shape)
+    S.advection_flux_conservative_eno3(flux, f, u, inv_dx)
+    fa = np.zeros(shape)
+    ua = np.zeros((3, *shape))
+    S.advection_flux_conservative_eno3_adjoint(fa, ua, a, f, u, inv_dx)
+    lhs = np.vdot(flux, a)
+    assert abs(lhs) > 0.1
+    np.testing.assert_allclose(np.vdot(f, fa), lhs, rtol=1e-12)
+    np.testing.assert_allclose(np.vdot(u, ua), lhs, rtol=1e-12)
+    du = 1e-7 * rng.standard_normal(u.shape)
+    assert np.array_equal(_eno3_masks(u), _eno3_masks(u + du))
+    flux2 = np.zeros(shape)
+    S.advection_flux_conservative_eno3(flux2, f, u + du, inv_dx)
+    np.testing.assert_allclose(np.vdot(flux2 - flux, a), np.vdot(du, ua), rtol=1e-6)
+    # each unit field adjoint vector: a column of the transpose equals a row of the forward
+    for c in [(2, 2, 2), (5, 4, 7), (3, 6, 2)]:
+        e = np.zeros(shape)
+        e[c] = 1.0
+        fa = np.zeros(shape)
+        S.advection_flux_conservative_eno3_adjoint(fa, None, e, f, u, inv_dx)
+        for m in [(c[0], c[1], c[2] - 2), (c[0], c[1] + 1, c[2]), (c[0] - 1, c[1], c[2]), c]:
+            g = np.zeros(shape)
+            g[m] = 1.0
+            fl = np.zeros(shape)
+            S.advection_flux_conservative_eno3(fl, g, u, inv_dx)
+            np.testing.assert_allclose(fa[m], fl[c], rtol=1e-12, atol=1e-15)
+
+
 def test_vorticity_stretching_linear_velocity():
     n = 7
     x, y, z = coords(n)
