@@ -26,8 +26,10 @@ against forward with checkpoints plus the adjoint sweep, 128^3, 200 steps) and
 the FP32 MFMA peak, the flow step with that solver, and the numpy oracle of the solve),
 "rod_coupling" (the flexible-filament driver loop at 128^3 with a heaving rod: rod substeps with
 the flow forces, one spreading per flow step), "other_sizes" (the sphere workload at 192^3
-fp32 and fp64 and at 256^3 fp64, each with its roofline) and "filtered_step" (the 256^3 step
-with the order-5 convolution vorticity filter of the flexible-filament example).
+fp32 and fp64 and at 256^3 fp64, each with its roofline), "filtered_step" (the 256^3 step
+with the order-5 convolution vorticity filter of the flexible-filament example) and
+"standalone_factories" (the drop-in stencil factories off the step path at 256^3 FP32: ENO3
+flux and its adjoint, vorticity stretching, curl, diffusion, each against HBM peak).
 """
 
 from __future__ import annotations
@@ -512,6 +514,51 @@ def other_sizes(steps):
     return out
 
 
+def standalone_factories(n=256, reps=20):
+    """The drop-in factories off the step path, one (n, n, n) FP32 field each, timed with HIP
+    events on the stream they launch on. Algorithmic words per cell: ENO3 flux 6 (field,
+    three velocity components, flux read and written), its adjoint 13 (flux adjoint, field,
+    velocity, field adjoint and velocity adjoint read and written), vorticity stretching 9,
+    curl 6, vector diffusion flux 6."""
+    import torch
+
+    from sopht_amd.numeric import eulerian_grid_ops as ops
+
+    rt = np.float32
+    gen = torch.Generator(device="cuda").manual_seed(3)
+    r = lambda *s: torch.randn(s, dtype=torch.float32, device="cuda", generator=gen)  # noqa: E731
+    f, flux, fa = r(n, n, n), r(n, n, n), r(n, n, n)
+    u, w, ua, out = r(3, n, n, n), r(3, n, n, n), r(3, n, n, n), r(3, n, n, n)
+    eno3 = ops.gen_advection_flux_conservative_eno3_pyst_kernel_3d(rt)
+    eno3_adj = ops.gen_advection_flux_conservative_eno3_adjoint_kernel_3d(rt)
+    stretch = ops.gen_vorticity_stretching_flux_pyst_kernel_3d(rt)
+    curl = ops.gen_curl_pyst_kernel_3d(rt)
+    diff = ops.gen_diffusion_flux_pyst_kernel_3d(rt, field_type="vector")
+    cases = (
+        ("eno3_flux", 6, lambda: eno3(flux, f, u, 0.1)),
+        ("eno3_flux_adjoint", 13, lambda: eno3_adj(fa, ua, flux, f, u, 0.1)),
+        ("vorticity_stretching_flux", 9, lambda: stretch(out, w, u, 0.1)),
+        ("curl", 6, lambda: curl(curl=out, field=w, prefactor=0.1)),
+        ("diffusion_flux_vector", 6, lambda: diff(out, w, 0.1)),
+    )
+    res = {}
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    for name, words, fn in cases:
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        ev[0].record()
+        for _ in range(reps):
+            fn()
+        ev[1].record()
+        torch.cuda.synchronize()
+        ms = ev[0].elapsed_time(ev[1]) / reps
+        gbs = words * 4 * n**3 / (ms * 1e-3) / 1e9
+        res[name] = {"ms": round(ms, 4), "words_per_cell": words, "achieved_gbs": round(gbs, 1),
+                     "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    return {"grid": [n, n, n], "dtype": "f32", "reps": reps, "ops": res}
+
+
 def copy_probe(n, reps):
     """A plain device copy of known size (one (n, n, n) fp32 field per launch, read once
     and written once) through the library's elementwise copy kernel: the calibration run
@@ -572,9 +619,12 @@ def main():
     ap.add_argument("--sizes-steps", type=int, default=20,
                     help="steps of the 192^3 fp32/fp64 and 256^3 fp64 lines (0 = skip)")
     ap.add_argument("--sizes-only", action="store_true", help="run only the other sizes and precisions")
+    ap.add_argument("--factory-reps", type=int, default=20,
+                    help="timed launches of each standalone factory at 256^3 fp32 (0 = skip)")
     ap.add_argument("--filter-steps", type=int, default=20,
                     help="steps of the 256^3 step with the order-5 convolution filter (0 = skip)")
     ap.add_argument("--filter-only", action="store_true", help="run only the filtered step")
+    ap.add_argument("--factory-only", action="store_true", help="run only the standalone factories")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -614,6 +664,12 @@ def main():
 
         torch.cuda.set_device(0)
         print(json.dumps(filtered_step(max(1, args.filter_steps))))
+        return 0
+    if args.factory_only:
+        import torch
+
+        torch.cuda.set_device(0)
+        print(json.dumps(standalone_factories(reps=max(1, args.factory_reps))))
         return 0
     if args.batch_only:
         import torch
@@ -705,6 +761,10 @@ def main():
     if args.filter_steps > 0:
         torch.cuda.empty_cache()
         filtered = filtered_step(args.filter_steps)
+    factories = None
+    if args.factory_reps > 0:
+        torch.cuda.empty_cache()
+        factories = standalone_factories(reps=args.factory_reps)
     cpu = None
     if args.cpu_steps > 0:
         try:
@@ -743,6 +803,7 @@ def main():
         "rod_coupling": rod,
         "other_sizes": sizes,
         "filtered_step": filtered,
+        "standalone_factories": factories,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
