@@ -306,43 +306,88 @@ int penalise_impl(void* field, const int64_t* fs, int64_t nz, int64_t ny, int64_
       s);
 }
 
+// Strides of one (z, y, x) field, with the neighbour step along each axis (0 = x, 1 = y, 2 = z).
+struct Strides3 {
+  int64_t z, y, x;
+  __device__ __forceinline__ int64_t at(int64_t zz, int64_t yy, int64_t xx) const {
+    return zz * z + yy * y + xx * x;
+  }
+  __device__ __forceinline__ int64_t step(int a) const { return a == 0 ? x : (a == 1 ? y : z); }
+};
+
+inline Strides3 strides_of(const int64_t* s) { return Strides3{s[0], s[1], s[2]}; }
+
+// ENO3 launches: 64 x 4 threads per workgroup, one z plane per grid slice, so each thread
+// finds its cell without division and forms every neighbour address from one base offset.
+constexpr int kEnoTx = 64, kEnoTy = 4;
+
+inline bool eno3_grid_fits(int64_t bz, int64_t by) {
+  return bz > 0 && bz <= 65535 && ceil_div(by, kEnoTy) <= 65535;
+}
+
 // ENO3 conservative advection flux (advection_flux_3d.py:12-233): the six sub-stencils
 // accumulate in the reference order on [2:-2]^3; one thread computes all six for its cell.
+template <typename T>
+__device__ __forceinline__ T eno3_cell(const T* f, int64_t fo, Strides3 fs, const T* const u[3],
+                                       int64_t uo, Strides3 us, T acc, T inv_dx) {
+  const T c13 = T(1.0 / 3.0), c56 = T(5.0 / 6.0), c16 = T(1.0 / 6.0);
+  for (int a = 0; a < 3; ++a) {
+    const T* ua = u[a];
+    const int64_t sf = fs.step(a), su = us.step(a);
+    auto F = [&](int k) -> T { return f[fo + k * sf]; };
+    auto U = [&](int k) -> T { return ua[uo + k * su]; };
+    T front = (U(0) > -U(1)) ? (c13 * F(1) * U(1) + c56 * F(0) * U(0) - c16 * F(-1) * U(-1))
+                             : (c13 * F(0) * U(0) + c56 * F(1) * U(1) - c16 * F(2) * U(2));
+    acc = acc + inv_dx * front;
+    T back = (U(0) > -U(-1)) ? (c13 * F(0) * U(0) + c56 * F(-1) * U(-1) - c16 * F(-2) * U(-2))
+                             : (c13 * F(-1) * U(-1) + c56 * F(0) * U(0) - c16 * F(1) * U(1));
+    acc = acc - inv_dx * back;
+  }
+  return acc;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kEnoTx* kEnoTy)
+    k_eno3(T* flux, Strides3 os, const T* f, Strides3 fs, const T* u0, const T* u1, const T* u2,
+           Strides3 us, int64_t ny, int64_t nx, T inv_dx) {
+  const int64_t x = 2 + int64_t(blockIdx.x) * kEnoTx + threadIdx.x;
+  const int64_t y = 2 + int64_t(blockIdx.y) * kEnoTy + threadIdx.y;
+  const int64_t z = 2 + int64_t(blockIdx.z);
+  if (x >= nx - 2 || y >= ny - 2) return;
+  const T* const u[3] = {u0, u1, u2};
+  const int64_t oo = os.at(z, y, x);
+  flux[oo] = eno3_cell<T>(f, fs.at(z, y, x), fs, u, us.at(z, y, x), us, flux[oo], inv_dx);
+}
+
 template <typename T>
 int eno3_impl(void* flux, const int64_t* fs, const void* field, const int64_t* is,
               const void* const vel[3], const int64_t* vs, int64_t nz, int64_t ny, int64_t nx,
               T inv_dx, hipStream_t s) {
   if (nz <= 4 || ny <= 4 || nx <= 4) return SOPHT_OK;
-  View3<T> o = make_view<T>(flux, fs), f = make_view<T>(field, is);
-  View3<T> u0 = make_view<T>(vel[0], vs), u1 = make_view<T>(vel[1], vs), u2 = make_view<T>(vel[2], vs);
+  const Strides3 os3 = strides_of(fs), is3 = strides_of(is), vs3 = strides_of(vs);
+  T* o = static_cast<T*>(flux);
+  const T* f = static_cast<const T*>(field);
+  const T* u0 = static_cast<const T*>(vel[0]);
+  const T* u1 = static_cast<const T*>(vel[1]);
+  const T* u2 = static_cast<const T*>(vel[2]);
+  if (eno3_grid_fits(nz - 4, ny - 4)) {
+    dim3 grid(unsigned(ceil_div(nx - 4, kEnoTx)), unsigned(ceil_div(ny - 4, kEnoTy)), unsigned(nz - 4));
+    k_eno3<T><<<grid, dim3(kEnoTx, kEnoTy), 0, s>>>(o, os3, f, is3, u0, u1, u2, vs3, ny, nx, inv_dx);
+    SOPHT_CHECK_LAUNCH();
+    return SOPHT_OK;
+  }
   return launch_box(
       2, 2, 2, nz - 4, ny - 4, nx - 4,
       [=] __device__(int64_t z, int64_t y, int64_t x) {
-        const T c13 = T(1.0 / 3.0), c56 = T(5.0 / 6.0), c16 = T(1.0 / 6.0);
-        T acc = o(z, y, x);
-        // axis a: 0 = x (u0), 1 = y (u1), 2 = z (u2)
-        for (int a = 0; a < 3; ++a) {
-          const View3<T>& u = a == 0 ? u0 : (a == 1 ? u1 : u2);
-          auto F = [&](int k) -> T {
-            return a == 0 ? f(z, y, x + k) : (a == 1 ? f(z, y + k, x) : f(z + k, y, x));
-          };
-          auto U = [&](int k) -> T {
-            return a == 0 ? u(z, y, x + k) : (a == 1 ? u(z, y + k, x) : u(z + k, y, x));
-          };
-          T front = (U(0) > -U(1)) ? (c13 * F(1) * U(1) + c56 * F(0) * U(0) - c16 * F(-1) * U(-1))
-                                   : (c13 * F(0) * U(0) + c56 * F(1) * U(1) - c16 * F(2) * U(2));
-          acc = acc + inv_dx * front;
-          T back = (U(0) > -U(-1)) ? (c13 * F(0) * U(0) + c56 * F(-1) * U(-1) - c16 * F(-2) * U(-2))
-                                   : (c13 * F(-1) * U(-1) + c56 * F(0) * U(0) - c16 * F(1) * U(1));
-          acc = acc - inv_dx * back;
-        }
-        o(z, y, x) = acc;
+        const T* const u[3] = {u0, u1, u2};
+        const int64_t oo = os3.at(z, y, x);
+        o[oo] = eno3_cell<T>(f, is3.at(z, y, x), is3, u, vs3.at(z, y, x), vs3, o[oo], inv_dx);
       },
       s);
 }
 
 // Weight of g[c + o] in the ENO3 flux at c along one axis (front face minus back face), for
-// the face switches up_front = u[c] > -u[c+1] and up_back = u[c] > -u[c-1] of eno3_impl.
+// the face switches up_front = u[c] > -u[c+1] and up_back = u[c] > -u[c-1] of eno3_cell.
 template <typename T>
 __device__ __forceinline__ T eno3_weight(bool up_front, bool up_back, int o) {
   const T c13 = T(1.0 / 3.0), c56 = T(5.0 / 6.0), c16 = T(1.0 / 6.0);
@@ -360,11 +405,62 @@ __device__ __forceinline__ T eno3_weight(bool up_front, bool up_back, int o) {
   return w;
 }
 
-// Transpose of eno3_impl with the switches frozen at `vel`: for a flux adjoint a (read on
-// [2:-2]^3), gbar_axis(m) = inv_dx * sum_o W(m - o, o) a(m - o) over m - o in [2:-2]^3, then
+// Transpose of the ENO3 flux with the switches frozen at the velocity, for one cell m:
+// gbar_axis(m) = inv_dx * sum_o W(m - o, o) a(m - o) over m - o in [2:-2]^3, then
 // field_adj += sum_axis gbar_axis * u_axis and vel_adj[axis] += gbar_axis * field. One thread
 // gathers for its own cell (no atomics, the same bits on every run). Either output may be
-// absent (has_f / has_v false).
+// absent (null pointer).
+template <typename T>
+__device__ __forceinline__ void eno3_adjoint_cell(int64_t z, int64_t y, int64_t x, int64_t nz,
+                                                  int64_t ny, int64_t nx, T* fa, Strides3 fas,
+                                                  T* const va[3], Strides3 vas, const T* a,
+                                                  Strides3 as, const T* f, Strides3 is,
+                                                  const T* const u[3], Strides3 us, T inv_dx) {
+  const int64_t m[3] = {x, y, z}, n[3] = {nx, ny, nz};
+  const bool in_j[3] = {x >= 2 && x < nx - 2, y >= 2 && y < ny - 2, z >= 2 && z < nz - 2};
+  const int64_t ao = as.at(z, y, x), uo = us.at(z, y, x);
+  T gbar[3] = {T(0), T(0), T(0)};
+  for (int ax = 0; ax < 3; ++ax) {
+    // the flux cells that read this one along ax share its other two coordinates
+    if (!in_j[(ax + 1) % 3] || !in_j[(ax + 2) % 3]) continue;
+    const T* ua = u[ax];
+    const int64_t su = us.step(ax), sa = as.step(ax);
+    auto U = [&](int64_t k) -> T { return ua[uo + k * su]; };
+    T g = T(0);
+    for (int o = -2; o <= 2; ++o) {
+      const int64_t c = m[ax] - o;
+      if (c < 2 || c >= n[ax] - 2) continue;
+      const T uc = U(-o);
+      g += eno3_weight<T>(uc > -U(1 - o), uc > -U(-1 - o), o) * a[ao - o * sa];
+    }
+    gbar[ax] = inv_dx * g;
+  }
+  if (fa != nullptr) {
+    fa[fas.at(z, y, x)] += gbar[0] * u[0][uo] + gbar[1] * u[1][uo] + gbar[2] * u[2][uo];
+  }
+  if (va[0] != nullptr) {
+    const T fv = f[is.at(z, y, x)];
+    const int64_t vo = vas.at(z, y, x);
+    va[0][vo] += gbar[0] * fv;
+    va[1][vo] += gbar[1] * fv;
+    va[2][vo] += gbar[2] * fv;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kEnoTx* kEnoTy)
+    k_eno3_adjoint(T* fa, Strides3 fas, T* va0, T* va1, T* va2, Strides3 vas, const T* a,
+                   Strides3 as, const T* f, Strides3 is, const T* u0, const T* u1, const T* u2,
+                   Strides3 us, int64_t nz, int64_t ny, int64_t nx, T inv_dx) {
+  const int64_t x = int64_t(blockIdx.x) * kEnoTx + threadIdx.x;
+  const int64_t y = int64_t(blockIdx.y) * kEnoTy + threadIdx.y;
+  const int64_t z = blockIdx.z;
+  if (x >= nx || y >= ny) return;
+  T* const va[3] = {va0, va1, va2};
+  const T* const u[3] = {u0, u1, u2};
+  eno3_adjoint_cell<T>(z, y, x, nz, ny, nx, fa, fas, va, vas, a, as, f, is, u, us, inv_dx);
+}
+
 template <typename T>
 int eno3_adjoint_impl(void* field_adj, const int64_t* fas, void* const vel_adj[3],
                       const int64_t* vas, const void* flux_adj, const int64_t* as,
@@ -375,44 +471,30 @@ int eno3_adjoint_impl(void* field_adj, const int64_t* fas, void* const vel_adj[3
   const bool has_f = field_adj != nullptr, has_v = vel_adj != nullptr;
   if (!has_f && !has_v) return SOPHT_OK;
   const int64_t zero3[3] = {0, 0, 0};
-  View3<T> fa = make_view<T>(field_adj, has_f ? fas : zero3);
-  View3<T> va0 = make_view<T>(has_v ? vel_adj[0] : nullptr, has_v ? vas : zero3);
-  View3<T> va1 = make_view<T>(has_v ? vel_adj[1] : nullptr, has_v ? vas : zero3);
-  View3<T> va2 = make_view<T>(has_v ? vel_adj[2] : nullptr, has_v ? vas : zero3);
-  View3<T> a = make_view<T>(flux_adj, as), f = make_view<T>(field, is);
-  View3<T> u0 = make_view<T>(vel[0], vs), u1 = make_view<T>(vel[1], vs), u2 = make_view<T>(vel[2], vs);
+  const Strides3 fas3 = strides_of(has_f ? fas : zero3), vas3 = strides_of(has_v ? vas : zero3);
+  const Strides3 as3 = strides_of(as), is3 = strides_of(is), us3 = strides_of(vs);
+  T* fa = static_cast<T*>(field_adj);
+  T* va0 = has_v ? static_cast<T*>(vel_adj[0]) : nullptr;
+  T* va1 = has_v ? static_cast<T*>(vel_adj[1]) : nullptr;
+  T* va2 = has_v ? static_cast<T*>(vel_adj[2]) : nullptr;
+  const T* a = static_cast<const T*>(flux_adj);
+  const T* f = static_cast<const T*>(field);
+  const T* u0 = static_cast<const T*>(vel[0]);
+  const T* u1 = static_cast<const T*>(vel[1]);
+  const T* u2 = static_cast<const T*>(vel[2]);
+  if (eno3_grid_fits(nz, ny)) {
+    dim3 grid(unsigned(ceil_div(nx, kEnoTx)), unsigned(ceil_div(ny, kEnoTy)), unsigned(nz));
+    k_eno3_adjoint<T><<<grid, dim3(kEnoTx, kEnoTy), 0, s>>>(fa, fas3, va0, va1, va2, vas3, a, as3, f,
+                                                          is3, u0, u1, u2, us3, nz, ny, nx, inv_dx);
+    SOPHT_CHECK_LAUNCH();
+    return SOPHT_OK;
+  }
   return launch_box(
       0, 0, 0, nz, ny, nx,
       [=] __device__(int64_t z, int64_t y, int64_t x) {
-        const int64_t m[3] = {x, y, z}, n[3] = {nx, ny, nz};
-        const bool in_j[3] = {x >= 2 && x < nx - 2, y >= 2 && y < ny - 2, z >= 2 && z < nz - 2};
-        T gbar[3] = {T(0), T(0), T(0)};
-        for (int ax = 0; ax < 3; ++ax) {
-          // the flux cells that read this one along ax share its other two coordinates
-          if (!in_j[(ax + 1) % 3] || !in_j[(ax + 2) % 3]) continue;
-          const View3<T>& u = ax == 0 ? u0 : (ax == 1 ? u1 : u2);
-          auto U = [&](int64_t k) -> T {
-            return ax == 0 ? u(z, y, x + k) : (ax == 1 ? u(z, y + k, x) : u(z + k, y, x));
-          };
-          auto A = [&](int64_t k) -> T {
-            return ax == 0 ? a(z, y, x + k) : (ax == 1 ? a(z, y + k, x) : a(z + k, y, x));
-          };
-          T g = T(0);
-          for (int o = -2; o <= 2; ++o) {
-            const int64_t c = m[ax] - o;
-            if (c < 2 || c >= n[ax] - 2) continue;
-            const T uc = U(-o);
-            g += eno3_weight<T>(uc > -U(1 - o), uc > -U(-1 - o), o) * A(-o);
-          }
-          gbar[ax] = inv_dx * g;
-        }
-        if (has_f) fa(z, y, x) += gbar[0] * u0(z, y, x) + gbar[1] * u1(z, y, x) + gbar[2] * u2(z, y, x);
-        if (has_v) {
-          const T fv = f(z, y, x);
-          va0(z, y, x) += gbar[0] * fv;
-          va1(z, y, x) += gbar[1] * fv;
-          va2(z, y, x) += gbar[2] * fv;
-        }
+        T* const va[3] = {va0, va1, va2};
+        const T* const u[3] = {u0, u1, u2};
+        eno3_adjoint_cell<T>(z, y, x, nz, ny, nx, fa, fas3, va, vas3, a, as3, f, is3, u, us3, inv_dx);
       },
       s);
 }
