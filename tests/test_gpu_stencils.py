@@ -236,8 +236,8 @@ def test_eno3_flux_autograd_dot_products(dev, n, rng):
     lhs.backward()
     lv = float(lhs.detach())
     assert abs(lv) > 1e-3
-    np.testing.assert_allclose(float((f * f.grad).sum()), lv, rtol=1e-11)
-    np.testing.assert_allclose(float((u * u.grad).sum()), lv, rtol=1e-11)
+    np.testing.assert_allclose(float((f.detach() * f.grad).sum()), lv, rtol=1e-11)
+    np.testing.assert_allclose(float((u.detach() * u.grad).sum()), lv, rtol=1e-11)
     if n == 16:
         fr, ur = np.zeros((n, n, n)), np.zeros((3, n, n, n))
         S.advection_flux_conservative_eno3_adjoint(fr, ur, to_host(a), to_host(f), to_host(u), 0.5)
