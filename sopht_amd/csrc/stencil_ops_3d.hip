@@ -346,17 +346,68 @@ __device__ __forceinline__ T eno3_cell(const T* f, int64_t fo, Strides3 fs, cons
   return acc;
 }
 
+// The same flux marching z: a thread owns one (y, x) column of kEnoZc planes and keeps f
+// and u_z at z-2..z+2 in registers, so the z terms cost two loads per cell instead of nine.
+// The x and y terms are read as in eno3_cell, and every expression is the same, so the
+// results are the same bits.
+constexpr int kEnoZc = 16;
+
 template <typename T>
 __global__ void __launch_bounds__(kEnoTx* kEnoTy)
-    k_eno3(T* flux, Strides3 os, const T* f, Strides3 fs, const T* u0, const T* u1, const T* u2,
-           Strides3 us, int64_t ny, int64_t nx, T inv_dx) {
+    k_eno3_zmarch(T* flux, Strides3 os, const T* f, Strides3 fs, const T* u0, const T* u1,
+                  const T* u2, Strides3 us, int64_t nz, int64_t ny, int64_t nx, T inv_dx) {
   const int64_t x = 2 + int64_t(blockIdx.x) * kEnoTx + threadIdx.x;
   const int64_t y = 2 + int64_t(blockIdx.y) * kEnoTy + threadIdx.y;
-  const int64_t z = 2 + int64_t(blockIdx.z);
-  if (x >= nx - 2 || y >= ny - 2) return;
-  const T* const u[3] = {u0, u1, u2};
-  const int64_t oo = os.at(z, y, x);
-  flux[oo] = eno3_cell<T>(f, fs.at(z, y, x), fs, u, us.at(z, y, x), us, flux[oo], inv_dx);
+  const int64_t z0 = 2 + int64_t(blockIdx.z) * kEnoZc;
+  if (x >= nx - 2 || y >= ny - 2 || z0 >= nz - 2) return;
+  const int64_t z1 = z0 + kEnoZc < nz - 2 ? z0 + kEnoZc : nz - 2;
+  const T c13 = T(1.0 / 3.0), c56 = T(5.0 / 6.0), c16 = T(1.0 / 6.0);
+  T fw[5], uw[5];
+  {
+    const int64_t fo = fs.at(z0, y, x), uo = us.at(z0, y, x);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      fw[k] = f[fo + (k - 2) * fs.z];
+      uw[k] = u2[uo + (k - 2) * us.z];
+    }
+  }
+  for (int64_t z = z0; z < z1; ++z) {
+    const int64_t fo = fs.at(z, y, x), uo = us.at(z, y, x), oo = os.at(z, y, x);
+    T acc = flux[oo];
+    for (int a = 0; a < 2; ++a) {
+      const T* ua = a == 0 ? u0 : u1;
+      const int64_t sf = a == 0 ? fs.x : fs.y, su = a == 0 ? us.x : us.y;
+      auto F = [&](int k) -> T { return f[fo + k * sf]; };
+      auto U = [&](int k) -> T { return ua[uo + k * su]; };
+      T front = (U(0) > -U(1)) ? (c13 * F(1) * U(1) + c56 * F(0) * U(0) - c16 * F(-1) * U(-1))
+                               : (c13 * F(0) * U(0) + c56 * F(1) * U(1) - c16 * F(2) * U(2));
+      acc = acc + inv_dx * front;
+      T back = (U(0) > -U(-1)) ? (c13 * F(0) * U(0) + c56 * F(-1) * U(-1) - c16 * F(-2) * U(-2))
+                               : (c13 * F(-1) * U(-1) + c56 * F(0) * U(0) - c16 * F(1) * U(1));
+      acc = acc - inv_dx * back;
+    }
+    {
+      auto F = [&](int k) -> T { return fw[k + 2]; };
+      auto U = [&](int k) -> T { return uw[k + 2]; };
+      T front = (U(0) > -U(1)) ? (c13 * F(1) * U(1) + c56 * F(0) * U(0) - c16 * F(-1) * U(-1))
+                               : (c13 * F(0) * U(0) + c56 * F(1) * U(1) - c16 * F(2) * U(2));
+      acc = acc + inv_dx * front;
+      T back = (U(0) > -U(-1)) ? (c13 * F(0) * U(0) + c56 * F(-1) * U(-1) - c16 * F(-2) * U(-2))
+                               : (c13 * F(-1) * U(-1) + c56 * F(0) * U(0) - c16 * F(1) * U(1));
+      acc = acc - inv_dx * back;
+    }
+    flux[oo] = acc;
+    if (z + 1 < z1) {
+      // plane z + 3 exists: z + 1 < z1 <= nz - 2
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        fw[k] = fw[k + 1];
+        uw[k] = uw[k + 1];
+      }
+      fw[4] = f[fo + 3 * fs.z];
+      uw[4] = u2[uo + 3 * us.z];
+    }
+  }
 }
 
 template <typename T>
@@ -370,9 +421,11 @@ int eno3_impl(void* flux, const int64_t* fs, const void* field, const int64_t* i
   const T* u0 = static_cast<const T*>(vel[0]);
   const T* u1 = static_cast<const T*>(vel[1]);
   const T* u2 = static_cast<const T*>(vel[2]);
-  if (eno3_grid_fits(nz - 4, ny - 4)) {
-    dim3 grid(unsigned(ceil_div(nx - 4, kEnoTx)), unsigned(ceil_div(ny - 4, kEnoTy)), unsigned(nz - 4));
-    k_eno3<T><<<grid, dim3(kEnoTx, kEnoTy), 0, s>>>(o, os3, f, is3, u0, u1, u2, vs3, ny, nx, inv_dx);
+  if (eno3_grid_fits(ceil_div(nz - 4, kEnoZc), ny - 4)) {
+    dim3 grid(unsigned(ceil_div(nx - 4, kEnoTx)), unsigned(ceil_div(ny - 4, kEnoTy)),
+              unsigned(ceil_div(nz - 4, kEnoZc)));
+    k_eno3_zmarch<T><<<grid, dim3(kEnoTx, kEnoTy), 0, s>>>(o, os3, f, is3, u0, u1, u2, vs3, nz, ny,
+                                                         nx, inv_dx);
     SOPHT_CHECK_LAUNCH();
     return SOPHT_OK;
   }

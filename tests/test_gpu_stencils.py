@@ -192,7 +192,23 @@ def test_eno3_flux_and_timestep(dev, real_t, rng):
     np.testing.assert_allclose(to_host(vd), refv, atol=tol(real_t), rtol=0)
 
 
-@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("shape", [(41, 12, 70), (22, 9, 5)])
+def test_eno3_flux_several_z_chunks(dev, real_t, shape, rng):
+    # 37 interior planes: two whole 16-plane column chunks and a partial third; a strided
+    # flux view; the write region [2:-2]^3 only
+    f = rng.standard_normal(shape).astype(real_t)
+    u = rng.standard_normal((3, *shape)).astype(real_t)
+    flux0 = rng.standard_normal(shape).astype(real_t)
+    ref = flux0.copy()
+    S.advection_flux_conservative_eno3(ref, f, u, real_t(0.4))
+    big = to_dev(np.repeat(flux0, 2, axis=1), dev)
+    view = big[:, ::2]
+    spne().gen_advection_flux_conservative_eno3_pyst_kernel_3d(real_t)(view, to_dev(f, dev), to_dev(u, dev), real_t(0.4))
+    np.testing.assert_allclose(to_host(view), ref, atol=tol(real_t), rtol=0)
+    np.testing.assert_array_equal(to_host(big[:, 1::2]), flux0)
+
+
+@pytest.mark.parametrize("shape", [*SHAPES, (41, 12, 70)])
 def test_eno3_adjoint_matches_oracle(dev, real_t, shape, rng):
     # accumulates onto nonzero adjoints; the switches compare the same values in both (the
     # oracle runs in FP64 on the FP32 inputs), so only the rounding of the sums differs
