@@ -262,6 +262,18 @@ def test_eno3_flux_autograd_dot_products(dev, n, rng):
     assert flux.dtype == torch.float64
 
 
+@pytest.mark.parametrize("shape", [(41, 12, 70), (20, 7, 5)])
+def test_vorticity_stretching_several_z_chunks(dev, real_t, shape, rng):
+    # non-cubic grids with a long z axis and a thin one; whole arrays, so the border reset too
+    w = rng.standard_normal((3, *shape)).astype(real_t)
+    u = rng.standard_normal((3, *shape)).astype(real_t)
+    ref = np.ones_like(w)
+    S.vorticity_stretching_flux(ref, w, u, real_t(0.3))
+    out = to_dev(np.ones_like(w), dev)
+    spne().gen_vorticity_stretching_flux_pyst_kernel_3d(real_t)(out, to_dev(w, dev), to_dev(u, dev), real_t(0.3))
+    np.testing.assert_allclose(to_host(out), ref, atol=tol(real_t), rtol=0)
+
+
 def test_vorticity_stretching(dev, real_t, rng):
     n = 16
     w = rng.standard_normal((3, n, n, n)).astype(real_t)
